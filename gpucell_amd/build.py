@@ -1,0 +1,86 @@
+"""Builds libgpucell_hip.so (gfx950) in-tree and the oracle checker library.
+
+    python -m gpucell_amd.build            # both
+    python -m gpucell_amd.build --lib      # the HIP library only
+
+The HIP sources are compiled one object per file in parallel and linked
+with hipcc into gpucell_amd/libgpucell_hip.so.
+"""
+from __future__ import annotations
+
+import argparse
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libgpucell_hip.so")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+SOURCES = ["gc_abi.hip", "gc_sdpd.hip", "gc_membrane.hip", "gc_engine.hip"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# Packed-FP32 instructions are disabled for all device code.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+
+
+def _run(cmd):
+    p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in p.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if p.returncode != 0:
+        raise RuntimeError(f"command failed ({p.returncode}): {' '.join(cmd)}\n{out}")
+    return out
+
+
+def build_lib(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+
+    def comp(src):
+        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        return obj
+
+    with cf.ThreadPoolExecutor(max_workers=min(4, os.cpu_count() or 1)) as ex:
+        objs = list(ex.map(comp, SOURCES))
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+def build_oracle(verbose: bool = False) -> str:
+    if shutil.which("make") is None:
+        raise RuntimeError("make not found")
+    _run(["make", "-s", "-C", ORACLE_DIR, "-B", "liboracle.so"])
+    path = os.path.join(ORACLE_DIR, "liboracle.so")
+    if verbose:
+        print(f"built {path}")
+    return path
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lib", action="store_true", help="HIP library only")
+    ap.add_argument("--oracle", action="store_true", help="oracle only")
+    a = ap.parse_args(argv)
+    if not a.oracle:
+        build_lib(verbose=True)
+    if not a.lib:
+        build_oracle(verbose=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

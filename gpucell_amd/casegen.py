@@ -1,0 +1,446 @@
+"""Seeded synthetic cases in the reference's data model.
+
+A case holds what the reference's four loaders produce (Particle::LoadParInfo,
+src/Particle.cu:29-427): particle coordinates in load order
+(Bound1 | Bound2 | fluid | membrane), membrane triangles and edge records
+with 1-based ids offset per cell, per-edge and per-triangle moduli, the
+bounding box and the 96-byte parameter block.
+
+Geometry:
+  * fluid: simple-cubic lattice of spacing dx with uniform jitter;
+  * channel walls: the same lattice continued 3 layers past the fluid
+    region (frozen Bound2 particles);
+  * RBC: icosahedron subdivided 3x (642 vertices, 1280 triangles,
+    1920 edges) mapped onto the Evans & Fung (1972) biconcave profile
+    z = +-D0 sqrt(1 - 4 r^2/D0^2) (a0 + a1 r^2/D0^2 + a2 r^4/D0^4),
+    a0 = 0.0518, a1 = 2.0026, a2 = -4.491;
+  * RBC placement: random centres and orientations from the seed,
+    rejected when any vertex comes closer than `gap` to another cell's or
+    to a wall;
+  * fluid particles closer than `clear` to a membrane vertex are removed.
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from . import abi
+
+
+# ---------------------------------------------------------------------------
+# membrane mesh
+def icosphere(subdiv: int = 3):
+    """Unit icosphere with outward-oriented triangles (0-based)."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
+         (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+         (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+         (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, dtype=np.float64) / np.linalg.norm(p) for p in v]
+    faces = list(f)
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = np.array(verts)
+    F = np.array(faces, dtype=np.int64)
+    # orient outward
+    n = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
+    flip = np.einsum("ij,ij->i", n, V[F].mean(axis=1)) < 0
+    F[flip] = F[flip][:, [0, 2, 1]]
+    return V, F
+
+
+def edge_records(F: np.ndarray) -> np.ndarray:
+    """(a, b, c, d, t1, t2) per edge, 0-based: (a, b) shared, c the third
+    vertex of t1 and d the third vertex of t2 (Particle.cuh:35)."""
+    owner = {}
+    for t, (a, b, c) in enumerate(F):
+        for (p, q, r) in ((a, b, c), (b, c, a), (c, a, b)):
+            owner.setdefault((min(p, q), max(p, q)), []).append((t, r))
+    rec = []
+    for (p, q), lst in sorted(owner.items()):
+        assert len(lst) == 2, "mesh must be closed and manifold"
+        (t1, c), (t2, d) = lst
+        rec.append((p, q, c, d, t1, t2))
+    return np.array(rec, dtype=np.int64)
+
+
+def biconcave(V: np.ndarray, D0: float) -> np.ndarray:
+    """Maps the unit sphere onto the Evans-Fung profile of diameter D0."""
+    a0, a1, a2 = 0.0518, 2.0026, -4.491
+    rs2 = V[:, 0] ** 2 + V[:, 1] ** 2
+    u = rs2 / 4.0  # (rho / D0)^2 with rho = rs * D0 / 2
+    out = np.empty_like(V)
+    out[:, 0] = 0.5 * D0 * V[:, 0]
+    out[:, 1] = 0.5 * D0 * V[:, 1]
+    out[:, 2] = D0 * V[:, 2] * (a0 + a1 * u + a2 * u * u)
+    return out
+
+
+def random_rotation(rng: np.random.Generator) -> np.ndarray:
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    w, x, y, z = q
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+        [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+        [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+# ---------------------------------------------------------------------------
+@dataclasses.dataclass
+class Case:
+    name: str
+    coor: np.ndarray            # [N,3] float32, load order
+    vel: np.ndarray             # [N,3] float32
+    nBound1: int
+    nBound2: int
+    nFluid: int
+    bbox: np.ndarray            # [6] float32
+    consts: abi.Structconst
+    ext: abi.gc_ext_consts
+    # membrane (1-based ids as the reference's loader stores them)
+    tri: np.ndarray             # [T,3] int32
+    edge: np.ndarray            # [E,6] int32
+    celltab: np.ndarray         # [Nm] int32
+    beginendtri: np.ndarray     # [C,2] int32 inclusive
+    shearmod: np.ndarray        # [E] float32
+    bendmod: np.ndarray         # [E] float32
+    hlarea: np.ndarray          # [T]
+    hgarea: np.ndarray          # [T]
+    hvol: np.ndarray            # [T]
+
+    @property
+    def nCell(self) -> int:
+        return int(self.beginendtri.shape[0])
+
+    @property
+    def nCellPar(self) -> int:
+        return int(self.celltab.shape[0])
+
+    @property
+    def N(self) -> int:
+        return int(self.coor.shape[0])
+
+    def desc(self) -> "tuple[abi.gc_case_desc, list]":
+        """gc_case_desc pointing at this case's arrays (keep the list alive)."""
+        keep = []
+
+        def p(a, dt):
+            if a is None or a.size == 0:
+                return None
+            b = np.ascontiguousarray(a, dtype=dt)
+            keep.append(b)
+            return b.ctypes.data
+
+        d = abi.gc_case_desc()
+        d.nBound1Par, d.nBound2Par, d.nFluidPar = self.nBound1, self.nBound2, self.nFluid
+        d.nCell, d.nAllTri, d.nAllEdge = self.nCell, int(self.tri.shape[0]), int(self.edge.shape[0])
+        d.nCellPar = self.nCellPar
+        d.coor = p(self.coor, np.float32)
+        d.vel = p(self.vel, np.float32)
+        d.tri = p(self.tri, np.int32)
+        d.edge = p(self.edge, np.int32)
+        d.celltab = p(self.celltab, np.int32)
+        d.beginendtri = p(self.beginendtri, np.int32)
+        d.shearmod_para = p(self.shearmod, np.float32)
+        d.bendingmod_para = p(self.bendmod, np.float32)
+        d.hlarea_para = p(self.hlarea, np.float32)
+        d.hgarea_para = p(self.hgarea, np.float32)
+        d.hvol_para = p(self.hvol, np.float32)
+        for i in range(6):
+            d.bbox[i] = float(self.bbox[i])
+        d.consts = self.consts
+        d.ext = self.ext
+        return d, keep
+
+
+@dataclasses.dataclass
+class Physics:
+    h: float = 1.5
+    mass: float = 1.0
+    rho0: float = 1.0
+    c1: float = 4.0
+    c2: float = 4.0
+    eta: float = 1.0
+    zeta: float = 0.0
+    kT: float = 0.01
+    dt: float = 0.005
+    fext: float = 0.02
+    lam: float = 0.65
+    seed: int = 1
+    # membrane moduli (per RBC type)
+    mu0: float = 1.0
+    kc: float = 0.1
+    ka: float = 20.0
+    kd: float = 5.0
+    kv: float = 50.0
+    # Morse
+    De: float = 0.0
+    r0: float = 0.3
+    beta: float = 1.5
+    mcut: float = 1.5
+
+
+def make_consts(ph: Physics, nb1: int, nb2: int, nfl: int, nm: int) -> abi.Structconst:
+    c = abi.Structconst()
+    c.nBound2Start = nb1
+    c.nFluidStart = nb1 + nb2
+    c.nCellStart = nb1 + nb2 + nfl
+    c.nTotalPar = nb1 + nb2 + nfl + nm
+    c.nCellPar = nm
+    c.iSeed = ph.seed
+    c.CutRadius = ph.h
+    c.Mass = ph.mass
+    c.SoundSpeed1 = ph.c1
+    c.SoundSpeed2 = ph.c2
+    c.DenInit = ph.rho0
+    c.VVsigma = ph.lam
+    c.ShearVis = ph.eta
+    c.BulkVis = ph.zeta
+    c.ConstExtForce = ph.fext * ph.mass
+    c.TimeStep = ph.dt
+    c.surEnergy = ph.De
+    c.MaxCut = ph.mcut
+    c.zeroLen = ph.r0
+    c.scaFac = ph.beta
+    c.Temp = ph.kT
+    return c
+
+
+def _lattice(lo, hi, dx):
+    axes = [np.arange(lo[d] + 0.5 * dx, hi[d], dx) for d in range(3)]
+    g = np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1).reshape(-1, 3)
+    return g
+
+
+def _place_cells(rng, n_cells, shape, inside, gap, lo, hi, max_tries=20000):
+    """Random sequential placement of rigid copies of `shape` (Nv,3)."""
+    placed = []
+    rad = float(np.max(np.linalg.norm(shape, axis=1)))
+    centres = []
+    tries = 0
+    while len(placed) < n_cells:
+        tries += 1
+        if tries > max_tries:
+            raise RuntimeError(f"could place only {len(placed)} of {n_cells} cells")
+        c = lo + rng.random(3) * (hi - lo)
+        R = random_rotation(rng)
+        X = shape @ R.T + c
+        if not inside(X):
+            continue
+        ok = True
+        for c2, X2 in zip(centres, placed):
+            if np.linalg.norm(c2 - c) > 2 * rad + gap:
+                continue
+            d2 = ((X[:, None, :] - X2[None, :, :]) ** 2).sum(-1)
+            if d2.min() < gap * gap:
+                ok = False
+                break
+        if ok:
+            placed.append(X)
+            centres.append(c)
+    return placed
+
+
+def _remove_near(points: np.ndarray, verts: np.ndarray, dist: float) -> np.ndarray:
+    """Mask of points farther than `dist` from every vertex (grid hashing)."""
+    if verts.shape[0] == 0:
+        return np.ones(points.shape[0], dtype=bool)
+    cell = dist
+    keys = np.floor(verts / cell).astype(np.int64)
+    table = {}
+    for i, k in enumerate(map(tuple, keys)):
+        table.setdefault(k, []).append(i)
+    pk = np.floor(points / cell).astype(np.int64)
+    keep = np.ones(points.shape[0], dtype=bool)
+    cand = np.unique(pk, axis=0)
+    near_keys = set()
+    for k in map(tuple, cand):
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    if (k[0] + dx, k[1] + dy, k[2] + dz) in table:
+                        near_keys.add(k)
+    idx = np.nonzero([tuple(k) in near_keys for k in pk])[0]
+    for i in idx:
+        k = pk[i]
+        vs = []
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    vs += table.get((k[0] + dx, k[1] + dy, k[2] + dz), [])
+        if vs:
+            d2 = ((verts[vs] - points[i]) ** 2).sum(-1).min()
+            if d2 < dist * dist:
+                keep[i] = False
+    return keep
+
+
+def _assemble(name, ph, ext, bbox, bound1, bound2, fluid, cells, tri0, edge0, rng, jitter, dx):
+    nb1, nb2 = bound1.shape[0], bound2.shape[0]
+    if jitter > 0:
+        fluid = fluid + rng.uniform(-jitter * dx, jitter * dx, size=fluid.shape)
+    mverts = np.concatenate(cells, axis=0) if cells else np.zeros((0, 3))
+    if cells:
+        keep = _remove_near(fluid, mverts, 0.5 * dx)
+        fluid = fluid[keep]
+    nfl = fluid.shape[0]
+    nv = cells[0].shape[0] if cells else 0
+    nm = mverts.shape[0]
+    coor = np.concatenate([bound1, bound2, fluid, mverts], axis=0).astype(np.float32)
+    N = coor.shape[0]
+    vel = np.zeros((N, 3), dtype=np.float64)
+    if ph.kT > 0:
+        mv = slice(nb1 + nb2, N)
+        vel[mv] = rng.normal(0.0, math.sqrt(ph.kT / ph.mass), size=(N - nb1 - nb2, 3))
+        vel[mv] -= vel[mv].mean(axis=0)
+    ncell = len(cells)
+    T, E = tri0.shape[0], edge0.shape[0]
+    tri = np.zeros((ncell * T, 3), dtype=np.int32)
+    edge = np.zeros((ncell * E, 6), dtype=np.int32)
+    bet = np.zeros((ncell, 2), dtype=np.int32)
+    for c in range(ncell):
+        tri[c * T:(c + 1) * T] = tri0 + 1 + c * nv
+        e = edge0.copy()
+        e[:, :4] += 1 + c * nv
+        e[:, 4:] += 1 + c * T
+        edge[c * E:(c + 1) * E] = e
+        bet[c] = (c * T, (c + 1) * T - 1)
+    celltab = np.repeat(np.arange(ncell, dtype=np.int32), nv)
+    consts = make_consts(ph, nb1, nb2, nfl, nm)
+    return Case(name=name, coor=coor, vel=vel.astype(np.float32), nBound1=nb1, nBound2=nb2,
+                nFluid=nfl, bbox=np.asarray(bbox, dtype=np.float32), consts=consts, ext=ext,
+                tri=tri, edge=edge, celltab=celltab, beginendtri=bet,
+                shearmod=np.full(ncell * E, ph.mu0, np.float32),
+                bendmod=np.full(ncell * E, ph.kc, np.float32),
+                hlarea=np.full(ncell * T, ph.kd, np.float32),
+                hgarea=np.full(ncell * T, ph.ka, np.float32),
+                hvol=np.full(ncell * T, ph.kv, np.float32))
+
+
+def periodic_box(n: int = 32, n_rbc: int = 1, rbc_diameter: float = 12.0, seed: int = 20261203,
+                 physics: Physics | None = None, jitter: float = 0.1, subdiv: int = 3,
+                 reverse_poiseuille: bool = True, noise: bool = True) -> Case:
+    """Fully periodic n^3 box of unit-spaced fluid with `n_rbc` RBCs
+    (n=32, one RBC: the 32x32x32 / one-RBC reverse-Poiseuille case)."""
+    ph = physics or Physics()
+    rng = np.random.default_rng(seed)
+    L = float(n)
+    bbox = np.array([0, L, 0, L, 0, L], dtype=np.float32)
+    ext = abi.default_ext()
+    ext.periodic[0] = ext.periodic[1] = ext.periodic[2] = 1
+    ext.force_mode = 1 if reverse_poiseuille else 0
+    ext.force_split = 0.5 * L
+    ext.noise_on = 1 if noise else 0
+    fluid = _lattice([0, 0, 0], [L, L, L], 1.0)
+    cells = []
+    V0, F0 = icosphere(subdiv)
+    shape = biconcave(V0, rbc_diameter)
+    if n_rbc == 1:
+        cells = [shape + np.array([L / 2, L / 2, L / 2])]
+    elif n_rbc > 1:
+        cells = _place_cells(rng, n_rbc, shape, lambda X: True, 1.0,
+                             np.zeros(3), np.full(3, L))
+    E0 = edge_records(F0)
+    return _assemble(f"periodic{n}", ph, ext, bbox, np.zeros((0, 3)), np.zeros((0, 3)), fluid,
+                     cells, F0, E0, rng, jitter, 1.0)
+
+
+def channel(length: int, width: int, n_rbc: int, hematocrit: float | None = None,
+            rbc_diameter: float | None = None, seed: int = 20261203,
+            physics: Physics | None = None, jitter: float = 0.1, wall_layers: int = 3,
+            subdiv: int = 3, noise: bool = True, stenosis: float = 0.0,
+            name: str = "channel") -> Case:
+    """Square duct, periodic along x, `wall_layers` frozen Bound2 layers.
+
+    With `hematocrit` the RBC diameter is chosen so that n_rbc cells fill
+    that fraction of the fluid volume.  `stenosis` in [0, 1) narrows the
+    duct with a cosine constriction: the wall sits where the signed distance
+    max(|y|, |z|) - w(x) changes sign, w(x) = W/2 (1 - s/2 (1 + cos(2 pi
+    (x - L/2) / L))) within the middle half of the length."""
+    ph = physics or Physics()
+    rng = np.random.default_rng(seed)
+    W, Lx = float(width), float(length)
+    half = 0.5 * W
+    bbox = np.array([0, Lx, -half, half, -half, half], dtype=np.float32)
+    ext = abi.default_ext()
+    ext.noise_on = 1 if noise else 0
+    nl = wall_layers
+    pts = _lattice([0, -half - nl, -half - nl], [Lx, half + nl, half + nl], 1.0)
+
+    def wall_half(x):
+        if stenosis <= 0:
+            return np.full_like(x, half)
+        w = np.full_like(x, half)
+        m = np.abs(x - Lx / 2) < Lx / 4
+        w[m] = half * (1 - 0.5 * stenosis * (1 + np.cos(2 * np.pi * (x[m] - Lx / 2) / (Lx / 2))))
+        return w
+
+    sd = np.maximum(np.abs(pts[:, 1]), np.abs(pts[:, 2])) - wall_half(pts[:, 0])
+    fluid = pts[sd < 0]
+    walls = pts[(sd >= 0) & (sd < nl)]
+    V0, F0 = icosphere(subdiv)
+    if rbc_diameter is None:
+        if hematocrit is None or n_rbc == 0:
+            rbc_diameter = 8.0
+        else:
+            unit = biconcave(V0, 1.0)
+            vol_unit = _mesh_volume(unit, F0)
+            rbc_diameter = (hematocrit * fluid.shape[0] / (n_rbc * vol_unit)) ** (1.0 / 3.0)
+    shape = biconcave(V0, rbc_diameter)
+    gap = 1.0
+
+    def inside(X):
+        w = wall_half(X[:, 0])
+        return bool(np.all(np.maximum(np.abs(X[:, 1]), np.abs(X[:, 2])) < w - gap) and
+                    np.all((X[:, 0] >= 0) & (X[:, 0] < Lx)))
+
+    cells = _place_cells(rng, n_rbc, shape, inside, gap, np.array([0, -half, -half]),
+                         np.array([Lx, half, half])) if n_rbc else []
+    E0 = edge_records(F0)
+    case = _assemble(name, ph, ext, bbox, np.zeros((0, 3)), walls, fluid, cells, F0, E0, rng,
+                     jitter, 1.0)
+    case.rbc_diameter = rbc_diameter
+    return case
+
+
+def _mesh_volume(V, F):
+    a, b, c = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    return float(np.einsum("ij,ij->i", a, np.cross(b, c)).sum() / 6.0)
+
+
+def mesh_volume(V, F):
+    return _mesh_volume(V, F)
+
+
+def suspension_1m(seed: int = 20261203, physics: Physics | None = None, noise: bool = True) -> Case:
+    """1M fluid+wall particles, 100 RBCs at 20 % hematocrit, 3 wall layers:
+    a 472 x 40 x 40 duct (46 x 46 with walls)."""
+    return channel(472, 40, 100, hematocrit=0.20, seed=seed, physics=physics, noise=noise,
+                   name="suspension_1m_100rbc")
+
+
+def stenosis_2m(seed: int = 20261203, physics: Physics | None = None, noise: bool = True) -> Case:
+    """2M particles, 300 RBCs at 30 % hematocrit, 50 % cosine stenosis,
+    Morse aggregation De = 0.3 kT, r0 = 0.3, beta = 1.5."""
+    ph = physics or Physics()
+    ph = dataclasses.replace(ph, De=0.3 * ph.kT, r0=0.3, beta=1.5, mcut=min(1.5, ph.h))
+    return channel(944, 40, 300, hematocrit=0.30, seed=seed, physics=ph, noise=noise,
+                   stenosis=0.5, name="stenosis_2m_300rbc")

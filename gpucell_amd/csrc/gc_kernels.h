@@ -1,0 +1,78 @@
+// gc_kernels.h -- kernels shared between the entry-point translation units
+// and the fused engine (defined in gc_sdpd.hip, gc_membrane.hip, gc_abi.hip).
+#pragma once
+
+#include <vector>
+
+#include "gc_internal.h"
+
+namespace gc {
+
+// gc_sdpd.hip
+__global__ void k_density(int n, GcParams P, const tfloat3* __restrict__ coor,
+                          tfloat4* __restrict__ velrhop, const tint3* __restrict__ cc,
+                          const int* __restrict__ ncube, const float* __restrict__ outer,
+                          const int* __restrict__ sortpart, const tint2* __restrict__ be);
+__global__ void k_bound2_vel(int n, GcParams P, const tfloat3* __restrict__ coor,
+                             tfloat4* __restrict__ velrhop, const tint3* __restrict__ cc,
+                             const int* __restrict__ ncube, const float* __restrict__ outer,
+                             const int* __restrict__ sortpart, const tint2* __restrict__ be);
+__global__ void k_eos(int n, GcParams P, const int* __restrict__ sortpart,
+                      const tfloat4* __restrict__ velrhop, float* __restrict__ press);
+__global__ void k_sdpd_force(int n, GcParams P, const tfloat3* __restrict__ coor,
+                             const tfloat4* __restrict__ velrhop, const tint3* __restrict__ cc,
+                             const int* __restrict__ ncube, const int* __restrict__ sortpart,
+                             const float* __restrict__ outer, const tint2* __restrict__ be,
+                             tfloat3* __restrict__ force, const float* __restrict__ press,
+                             const unsigned long long* __restrict__ ctr);
+__global__ void k_rep_force(int n, GcParams P, const tfloat3* __restrict__ coor,
+                            const tint3* __restrict__ cc, const int* __restrict__ ncube,
+                            const int* __restrict__ sortpart, const float* __restrict__ outer,
+                            const tint2* __restrict__ be, tfloat3* __restrict__ force);
+__global__ void k_ext_force(int n, GcParams P, const int* __restrict__ sortpart,
+                            const tfloat3* __restrict__ coor, tfloat3* __restrict__ force);
+__global__ void k_correct(int n, GcParams P, const int* __restrict__ sortpart,
+                          const tfloat3* __restrict__ force, const tfloat3* __restrict__ velhalf,
+                          tfloat4* __restrict__ velrhop);
+
+// gc_membrane.hip
+struct DefArgs {
+    const int6* edge; const tint3* tri; const tfloat3* coor; const tint3* nthro; const int* curpos;
+    const float* lmax; const float* mexp; const float* p; const float* kp; const float* th0;
+    const float* kb; const float* reftriarea; const float* kd; const float* ka; const float* Ac;
+    const float* A0c; const float* kv; const float* Vc; const float* V0c;
+    const int* es; const int* ee; const int* ts; const int* te;
+    tfloat3* defforce; tfloat3* force;
+};
+
+__global__ void k_geo_cell(GcParams P, long stepcount, const float* __restrict__ bbox,
+                           const int* __restrict__ curpos, const tint3* __restrict__ tri,
+                           const tint2* __restrict__ betri, const tint3* __restrict__ nthro,
+                           const tfloat3* __restrict__ coor, tfloat3* __restrict__ trinor,
+                           float* __restrict__ triarea, float* __restrict__ reftriarea,
+                           tfloat3* __restrict__ tricen, float* __restrict__ refarea_para,
+                           float* __restrict__ refvol_para, float* __restrict__ area_para,
+                           float* __restrict__ vol_para, float* __restrict__ area,
+                           float* __restrict__ vol);
+__global__ void k_par_area(int nm, const int* __restrict__ ts, const int* __restrict__ te,
+                           const float* __restrict__ triarea, float* __restrict__ pararea);
+__global__ void k_mech_para(GcParams P, int nedge, const int6* __restrict__ edge,
+                            const int* __restrict__ curpos, const tfloat3* __restrict__ coor,
+                            float* __restrict__ l0o, float* __restrict__ lmaxo,
+                            const float* __restrict__ kc, const float* __restrict__ mu0,
+                            float* __restrict__ kbo, float* __restrict__ mo, float* __restrict__ po,
+                            float* __restrict__ kpo, float* __restrict__ th0o);
+__global__ void k_def_force(GcParams P, int nm, const float* __restrict__ bbox, DefArgs a);
+__global__ void k_agg_force(int n, GcParams P, const tfloat3* __restrict__ coor,
+                            tfloat3* __restrict__ aggforce, const tint3* __restrict__ cc,
+                            const int* __restrict__ ncube, const int* __restrict__ sortpart,
+                            const int* __restrict__ celltab, const float* __restrict__ outer,
+                            const tint2* __restrict__ be, tfloat3* __restrict__ force);
+int build_incidence(int nm, int nedge, const int6* h_edge, int ntri, const tint3* h_tri,
+                    std::vector<int>& es, std::vector<int>& ee, std::vector<int>& ts,
+                    std::vector<int>& te);
+
+// gc_abi.hip
+__global__ void k_fill_int(int* __restrict__ a, int v, int n);
+
+}  // namespace gc

@@ -1,0 +1,512 @@
+// gc_membrane.hip -- RBC membrane entry points replacing cupar::CalGeoPara,
+// SetMechPara, CalCellDefForce and CalCellAggForce (src/ParticleGPU.cuh:25-35).
+//
+// Membrane vertex v (0-based) is particle nCellStart + v; its slot is
+// curpos[nCellStart + v].  Triangle and edge records hold 1-based vertex ids
+// (the reference's loader adds the per-cell offset to 1-based file ids,
+// Particle.cu:252,268-284).  Unwrapped positions use the image counters:
+// u = x + nthrobound * L on periodic axes; differences of unwrapped
+// positions are formed in FP64 and rounded once.
+//
+// Mechanics (Fedosov, Caswell & Karniadakis 2010, Biophys J 98:2215):
+//   E = sum_edges [ kT l_max/(4p) (3x^2-2x^3)/(1-x) + k_p/((m-1) l^(m-1))
+//                   + k_b (1 - cos(theta - theta0)) ]
+//     + sum_tri k_d (A_t - A_t0)^2/(2 A_t0)
+//     + sum_cells [ k_a (A - A0)^2/(2 A0) + k_v (V - V0)^2/(2 V0) ],
+// x = l/l_max, theta the signed dihedral of the edge's two triangles.
+// Forces are -grad E, gathered per vertex over its incident edges and
+// triangles in a fixed order (deterministic, no float atomics).
+#include "gc_kernels.h"
+
+#include <algorithm>
+#include <vector>
+
+using namespace gc;
+
+namespace {
+
+constexpr int BS = GC_BS;
+
+struct Topo {
+    const void* edge_ptr = nullptr;
+    const void* tri_ptr = nullptr;
+    int nedge = -1, ntri = -1, nm = -1;
+    int* d_es = nullptr;   // [nm+1] edge-entry starts
+    int* d_ee = nullptr;   // [4*nedge] (edge << 2) | role (0=A,1=B,2=C,3=D)
+    int* d_ts = nullptr;   // [nm+1]
+    int* d_te = nullptr;   // [3*ntri] (tri << 2) | corner
+};
+Topo g_topo;
+
+void free_topo()
+{
+    if (g_topo.d_es) (void)hipFree(g_topo.d_es);
+    if (g_topo.d_ee) (void)hipFree(g_topo.d_ee);
+    if (g_topo.d_ts) (void)hipFree(g_topo.d_ts);
+    if (g_topo.d_te) (void)hipFree(g_topo.d_te);
+    g_topo = Topo();
+}
+
+}  // namespace
+
+namespace gc {
+
+// Builds vertex -> (edge, role) and vertex -> (triangle, corner) tables on
+// the host from 1-based records (edge records may be null).  Entries of a
+// vertex are in ascending element order.
+int build_incidence(int nm, int nedge, const int6* h_edge, int ntri, const tint3* h_tri,
+                    std::vector<int>& es, std::vector<int>& ee, std::vector<int>& ts,
+                    std::vector<int>& te)
+{
+    es.assign(nm + 1, 0); ts.assign(nm + 1, 0);
+    for (int e = 0; e < nedge; ++e) {
+        const int* r = &h_edge[e].x;
+        for (int k = 0; k < 4; ++k) {
+            int v = r[k] - 1;
+            if (v < 0 || v >= nm) return set_error(GC_ERR_TOPOLOGY, "edge vertex id out of range");
+            es[v + 1]++;
+        }
+    }
+    for (int t = 0; t < ntri; ++t) {
+        const int* r = &h_tri[t].x;
+        for (int k = 0; k < 3; ++k) {
+            int v = r[k] - 1;
+            if (v < 0 || v >= nm) return set_error(GC_ERR_TOPOLOGY, "triangle vertex id out of range");
+            ts[v + 1]++;
+        }
+    }
+    for (int v = 0; v < nm; ++v) { es[v + 1] += es[v]; ts[v + 1] += ts[v]; }
+    ee.assign((size_t)4 * nedge, 0); te.assign((size_t)3 * ntri, 0);
+    std::vector<int> fe(es.begin(), es.end() - 1), ft(ts.begin(), ts.end() - 1);
+    for (int e = 0; e < nedge; ++e) {
+        const int* r = &h_edge[e].x;
+        for (int k = 0; k < 4; ++k) ee[fe[r[k] - 1]++] = (e << 2) | k;
+    }
+    for (int t = 0; t < ntri; ++t) {
+        const int* r = &h_tri[t].x;
+        for (int k = 0; k < 3; ++k) te[ft[r[k] - 1]++] = (t << 2) | k;
+    }
+    return GC_OK;
+}
+
+int ensure_topology(int nm, int nedge, const int6* d_edge, int ntri, const tint3* d_tri)
+{
+    if (g_topo.edge_ptr == (const void*)d_edge && g_topo.tri_ptr == (const void*)d_tri &&
+        g_topo.nedge == nedge && g_topo.ntri == ntri && g_topo.nm == nm)
+        return GC_OK;
+    hipStream_t s = stream();
+    int st = hip_check(hipStreamSynchronize(s), "topology sync");
+    if (st) return st;
+    std::vector<int6> he(d_edge ? nedge : 0);
+    std::vector<tint3> ht(ntri);
+    if (d_edge && nedge > 0)
+        if ((st = hip_check(hipMemcpy(he.data(), d_edge, sizeof(int6) * nedge, hipMemcpyDeviceToHost), "edge D2H"))) return st;
+    if (ntri > 0)
+        if ((st = hip_check(hipMemcpy(ht.data(), d_tri, sizeof(tint3) * ntri, hipMemcpyDeviceToHost), "tri D2H"))) return st;
+    std::vector<int> es, ee, ts, te;
+    st = build_incidence(nm, d_edge ? nedge : 0, he.data(), ntri, ht.data(), es, ee, ts, te);
+    if (st) return st;
+    free_topo();
+    auto up = [&](int** dst, const std::vector<int>& v) -> int {
+        size_t b = sizeof(int) * (v.size() ? v.size() : 1);
+        if (hipMalloc(dst, b) != hipSuccess) return set_error(GC_ERR_NOMEM, "topology alloc");
+        if (!v.empty()) return hip_check(hipMemcpy(*dst, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice), "topology H2D");
+        return GC_OK;
+    };
+    if ((st = up(&g_topo.d_es, es)) || (st = up(&g_topo.d_ee, ee)) || (st = up(&g_topo.d_ts, ts)) ||
+        (st = up(&g_topo.d_te, te))) { free_topo(); return st; }
+    g_topo.edge_ptr = d_edge; g_topo.tri_ptr = d_tri;
+    g_topo.nedge = d_edge ? nedge : 0; g_topo.ntri = ntri; g_topo.nm = nm;
+    return GC_OK;
+}
+
+struct PerBox { float L[3]; int per[3]; };
+
+__device__ __forceinline__ PerBox load_box(const GcParams& P, const float* bbox)
+{
+    PerBox b;
+    for (int d = 0; d < 3; ++d) { b.L[d] = bbox[2 * d + 1] - bbox[2 * d]; b.per[d] = P.per[d]; }
+    return b;
+}
+
+// unwrapped(u) - unwrapped(o) in FP64, rounded to FP32
+__device__ __forceinline__ float3 urel(const tfloat3& xu, const tint3& nu, const tfloat3& xo,
+                                       const tint3& no, const PerBox& b)
+{
+    double dx = (double)xu.x - (double)xo.x, dy = (double)xu.y - (double)xo.y,
+           dz = (double)xu.z - (double)xo.z;
+    if (b.per[0]) dx += (double)(nu.x - no.x) * (double)b.L[0];
+    if (b.per[1]) dy += (double)(nu.y - no.y) * (double)b.L[1];
+    if (b.per[2]) dz += (double)(nu.z - no.z) * (double)b.L[2];
+    return make_float3((float)dx, (float)dy, (float)dz);
+}
+
+__device__ __forceinline__ float3 f3sub(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 f3cross(float3 a, float3 b)
+{
+    return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float f3dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 f3scale(float3 a, float s) { return make_float3(a.x * s, a.y * s, a.z * s); }
+
+// ---- CalGeoPara ---------------------------------------------------------
+// One block per cell: per-triangle geometry, FP64 block reduction of the
+// cell's area and volume (signed tetrahedra against the cell's first
+// triangle's first vertex), then the per-triangle broadcasts.
+__global__ __launch_bounds__(BS) void k_geo_cell(GcParams P, long stepcount,
+    const float* __restrict__ bbox, const int* __restrict__ curpos, const tint3* __restrict__ tri,
+    const tint2* __restrict__ betri, const tint3* __restrict__ nthro, const tfloat3* __restrict__ coor,
+    tfloat3* __restrict__ trinor, float* __restrict__ triarea, float* __restrict__ reftriarea,
+    tfloat3* __restrict__ tricen, float* __restrict__ refarea_para, float* __restrict__ refvol_para,
+    float* __restrict__ area_para, float* __restrict__ vol_para, float* __restrict__ area,
+    float* __restrict__ vol)
+{
+    __shared__ double sa[BS], sv[BS];
+    const int c = blockIdx.x;
+    const int tb = betri[c].x, te = betri[c].y;
+    PerBox b = load_box(P, bbox);
+    int so = curpos[P.nc + tri[tb].x - 1];
+    tfloat3 xo = coor[so];
+    tint3 no = nthro[so];
+    double A = 0.0, V = 0.0;
+    for (int t = tb + threadIdx.x; t <= te; t += BS) {
+        tint3 tr = tri[t];
+        int s0 = curpos[P.nc + tr.x - 1], s1 = curpos[P.nc + tr.y - 1], s2 = curpos[P.nc + tr.z - 1];
+        float3 p0 = urel(coor[s0], nthro[s0], xo, no, b);
+        float3 p1 = urel(coor[s1], nthro[s1], xo, no, b);
+        float3 p2 = urel(coor[s2], nthro[s2], xo, no, b);
+        float3 n = f3cross(f3sub(p1, p0), f3sub(p2, p0));
+        float nn = sqrtf(f3dot(n, n));
+        float at = 0.5f * nn;
+        float in = 1.0f / nn;
+        trinor[t] = tfloat3{n.x * in, n.y * in, n.z * in};
+        triarea[t] = at;
+        if (stepcount == -1) reftriarea[t] = at;
+        // centre in absolute unwrapped coordinates of the cell's frame
+        double ox = (double)xo.x + (b.per[0] ? (double)no.x * b.L[0] : 0.0);
+        double oy = (double)xo.y + (b.per[1] ? (double)no.y * b.L[1] : 0.0);
+        double oz = (double)xo.z + (b.per[2] ? (double)no.z * b.L[2] : 0.0);
+        tricen[t] = tfloat3{(float)(ox + ((double)p0.x + p1.x + p2.x) / 3.0),
+                            (float)(oy + ((double)p0.y + p1.y + p2.y) / 3.0),
+                            (float)(oz + ((double)p0.z + p1.z + p2.z) / 3.0)};
+        A += (double)at;
+        // signed tetrahedron (O, p0, p1, p2) in FP64
+        double ax = p0.x, ay = p0.y, az = p0.z, bx = p1.x, by = p1.y, bz = p1.z;
+        double cx = p2.x, cy = p2.y, cz = p2.z;
+        V += (ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx)) / 6.0;
+    }
+    sa[threadIdx.x] = A; sv[threadIdx.x] = V;
+    __syncthreads();
+    for (int o = BS / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) { sa[threadIdx.x] += sa[threadIdx.x + o]; sv[threadIdx.x] += sv[threadIdx.x + o]; }
+        __syncthreads();
+    }
+    float Ac = (float)sa[0], Vc = (float)sv[0];
+    if (threadIdx.x == 0) { area[c] = Ac; vol[c] = Vc; }
+    for (int t = tb + threadIdx.x; t <= te; t += BS) {
+        area_para[t] = Ac; vol_para[t] = Vc;
+        if (stepcount == -1) { refarea_para[t] = Ac; refvol_para[t] = Vc; }
+    }
+}
+
+__global__ __launch_bounds__(BS) void k_par_area(int nm, const int* __restrict__ ts,
+                                                 const int* __restrict__ te,
+                                                 const float* __restrict__ triarea,
+                                                 float* __restrict__ pararea)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nm) return;
+    float a = 0.f;
+    for (int q = ts[v]; q < ts[v + 1]; ++q) a += triarea[te[q] >> 2];
+    pararea[v] = a * (1.0f / 3.0f);
+}
+
+// ---- SetMechPara ----------------------------------------------------------
+// Dihedral of edge (A,B) with wings C (triangle v) and D (triangle w), all
+// relative to A: n1 = e x c, n2 = d x e, theta = atan2((n1^ x n2^).e^, n1^.n2^).
+__device__ __forceinline__ double dihedral_d(double3 e, double3 c, double3 d)
+{
+    double3 n1 = make_double3(e.y * c.z - e.z * c.y, e.z * c.x - e.x * c.z, e.x * c.y - e.y * c.x);
+    double3 n2 = make_double3(d.y * e.z - d.z * e.y, d.z * e.x - d.x * e.z, d.x * e.y - d.y * e.x);
+    double m1 = sqrt(n1.x * n1.x + n1.y * n1.y + n1.z * n1.z);
+    double m2 = sqrt(n2.x * n2.x + n2.y * n2.y + n2.z * n2.z);
+    double l = sqrt(e.x * e.x + e.y * e.y + e.z * e.z);
+    double3 u1 = make_double3(n1.x / m1, n1.y / m1, n1.z / m1);
+    double3 u2 = make_double3(n2.x / m2, n2.y / m2, n2.z / m2);
+    double3 x = make_double3(u1.y * u2.z - u1.z * u2.y, u1.z * u2.x - u1.x * u2.z, u1.x * u2.y - u1.y * u2.x);
+    return atan2((x.x * e.x + x.y * e.y + x.z * e.z) / l, u1.x * u2.x + u1.y * u2.y + u1.z * u2.z);
+}
+
+__global__ __launch_bounds__(BS) void k_mech_para(GcParams P, int nedge, const int6* __restrict__ edge,
+    const int* __restrict__ curpos, const tfloat3* __restrict__ coor, float* __restrict__ l0o,
+    float* __restrict__ lmaxo, const float* __restrict__ kc, const float* __restrict__ mu0,
+    float* __restrict__ kbo, float* __restrict__ mo, float* __restrict__ po, float* __restrict__ kpo,
+    float* __restrict__ th0o)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nedge) return;
+    int6 E = edge[e];
+    tfloat3 A = coor[curpos[P.nc + E.x - 1]], B = coor[curpos[P.nc + E.y - 1]];
+    tfloat3 C = coor[curpos[P.nc + E.z - 1]], D = coor[curpos[P.nc + E.k - 1]];
+    double3 be = make_double3((double)B.x - A.x, (double)B.y - A.y, (double)B.z - A.z);
+    double3 ce = make_double3((double)C.x - A.x, (double)C.y - A.y, (double)C.z - A.z);
+    double3 de = make_double3((double)D.x - A.x, (double)D.y - A.y, (double)D.z - A.z);
+    double L0 = sqrt(be.x * be.x + be.y * be.y + be.z * be.z);
+    double x0 = 1.0 / (double)P.wlc_ratio, m = P.pow_index, kT = P.kT;
+    double g = 1.0 / (4.0 * (1 - x0) * (1 - x0)) - 0.25 + x0;
+    double hh = x0 / (2.0 * (1 - x0) * (1 - x0) * (1 - x0)) - 1.0 / (4.0 * (1 - x0) * (1 - x0)) + 0.25;
+    double p = kT * sqrt(3.0) * (hh + (m + 1.0) * g) / (4.0 * (double)mu0[e] * L0);
+    l0o[e] = (float)L0;
+    lmaxo[e] = (float)(L0 * (double)P.wlc_ratio);
+    mo[e] = (float)m;
+    po[e] = (float)p;
+    kpo[e] = (float)(kT / p * g * pow(L0, m));
+    kbo[e] = (float)(2.0 * (double)kc[e] / sqrt(3.0));
+    th0o[e] = (float)dihedral_d(be, ce, de);
+}
+
+// ---- CalCellDefForce --------------------------------------------------------
+__device__ __forceinline__ float3 vpos(const DefArgs& a, const GcParams& P, const PerBox& b, int v1,
+                                       const tfloat3& xo, const tint3& no)
+{
+    int s = a.curpos[P.nc + v1 - 1];
+    return urel(a.coor[s], a.nthro[s], xo, no, b);
+}
+
+__global__ __launch_bounds__(BS) void k_def_force(GcParams P, int nm, const float* __restrict__ bbox,
+                                                  DefArgs a)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nm) return;
+    PerBox b = load_box(P, bbox);
+    int sv = a.curpos[P.nc + v];
+    tfloat3 xo = a.coor[sv];
+    tint3 no = a.nthro[sv];
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    // edges: WLC-POW spring (roles A, B) and bending (all four roles)
+    for (int q = a.es[v]; q < a.es[v + 1]; ++q) {
+        int code = a.ee[q], e = code >> 2, role = code & 3;
+        int6 E = a.edge[e];
+        float3 pA = vpos(a, P, b, E.x, xo, no), pB = vpos(a, P, b, E.y, xo, no);
+        float3 pC = vpos(a, P, b, E.z, xo, no), pD = vpos(a, P, b, E.k, xo, no);
+        if (role < 2) {
+            float3 d = f3sub(pA, pB);
+            float l = sqrtf(f3dot(d, d));
+            float x = l / a.lmax[e];
+            float om = 1.0f - x;
+            float fm = -(P.kT / a.p[e]) * (0.25f / (om * om) - 0.25f + x) + a.kp[e] / powf(l, a.mexp[e]);
+            float s = (role == 0 ? fm : -fm) / l;
+            fx += s * d.x; fy += s * d.y; fz += s * d.z;
+        }
+        float3 e3 = f3sub(pB, pA), ac = f3sub(pC, pA), ad = f3sub(pD, pA);
+        float3 n1 = f3cross(e3, ac), n2 = f3cross(ad, e3);
+        float l2 = f3dot(e3, e3), l = sqrtf(l2);
+        float q1 = f3dot(n1, n1), q2 = f3dot(n2, n2);
+        float im1 = rsqrtf(q1), im2 = rsqrtf(q2);
+        float3 u1 = f3scale(n1, im1), u2 = f3scale(n2, im2);
+        float3 xx = f3cross(u1, u2);
+        float th = atan2f(f3dot(xx, e3) / l, f3dot(u1, u2));
+        float cb = -a.kb[e] * sinf(th - a.th0[e]);
+        float3 gC = f3scale(n1, -l / q1), gD = f3scale(n2, -l / q2);
+        float3 g;
+        if (role == 2) g = gC;
+        else if (role == 3) g = gD;
+        else {
+            float tC = f3dot(ac, e3) / l2, tD = f3dot(ad, e3) / l2;
+            float wC = role == 0 ? -(1.0f - tC) : -tC;
+            float wD = role == 0 ? -(1.0f - tD) : -tD;
+            g = make_float3(wC * gC.x + wD * gD.x, wC * gC.y + wD * gD.y, wC * gC.z + wD * gD.z);
+        }
+        fx += cb * g.x; fy += cb * g.y; fz += cb * g.z;
+    }
+    // triangles: local + global area, volume (tetrahedra against this vertex)
+    for (int q = a.ts[v]; q < a.ts[v + 1]; ++q) {
+        int code = a.te[q], t = code >> 2, k = code & 3;
+        tint3 T = a.tri[t];
+        int id[3] = {T.x, T.y, T.z};
+        float3 P0 = vpos(a, P, b, id[0], xo, no), P1 = vpos(a, P, b, id[1], xo, no);
+        float3 P2 = vpos(a, P, b, id[2], xo, no);
+        float3 n = f3cross(f3sub(P1, P0), f3sub(P2, P0));
+        float nn = sqrtf(f3dot(n, n));
+        float At = 0.5f * nn;
+        float3 un = f3scale(n, 1.0f / nn);
+        float cA = a.ka[t] * (a.Ac[t] - a.A0c[t]) / a.A0c[t] + a.kd[t] * (At - a.reftriarea[t]) / a.reftriarea[t];
+        float cV = a.kv[t] * (a.Vc[t] - a.V0c[t]) / a.V0c[t];
+        float3 pn = k == 0 ? P1 : (k == 1 ? P2 : P0);   // next corner
+        float3 pnn = k == 0 ? P2 : (k == 1 ? P0 : P1);  // next-next corner
+        float3 dA = f3cross(un, f3sub(pnn, pn));        // 2 dA/dP_k
+        float3 dV = f3cross(pn, pnn);                   // 6 dV/dP_k (this vertex at origin)
+        fx += -0.5f * cA * dA.x - cV * dV.x * (1.0f / 6.0f);
+        fy += -0.5f * cA * dA.y - cV * dV.y * (1.0f / 6.0f);
+        fz += -0.5f * cA * dA.z - cV * dV.z * (1.0f / 6.0f);
+    }
+    a.defforce[v] = tfloat3{fx, fy, fz};
+    tfloat3 F = a.force[sv];
+    F.x += fx; F.y += fy; F.z += fz;
+    a.force[sv] = F;
+}
+
+// ---- CalCellAggForce ------------------------------------------------------
+__global__ __launch_bounds__(BS) void k_agg_force(int n, GcParams P, const tfloat3* __restrict__ coor,
+    tfloat3* __restrict__ aggforce, const tint3* __restrict__ cc, const int* __restrict__ ncube,
+    const int* __restrict__ sortpart, const int* __restrict__ celltab, const float* __restrict__ outer,
+    const tint2* __restrict__ be, tfloat3* __restrict__ force)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int id = sortpart[p];
+    if (id < P.nc) return;
+    int v = id - P.nc;
+    int cell = celltab[v];
+    GcGridV g;
+    g.nc[0] = ncube[0]; g.nc[1] = ncube[1]; g.nc[2] = ncube[2];
+    g.L[0] = outer[0]; g.L[1] = outer[1]; g.L[2] = outer[2];
+    tfloat3 xi = coor[p];
+    tint3 c = cc[p];
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    const float tb = 2.0f * P.beta * P.De;
+    visit_ranges(g, P.per, be, c.x, c.y, c.z, [&](int jb, int je, float sx, float sy, float sz) {
+        for (int j = jb; j < je; ++j) {
+            int idj = sortpart[j];
+            if (idj < P.nc || celltab[idj - P.nc] == cell) continue;
+            tfloat3 xj = coor[j];
+            float rx = xi.x - (xj.x + sx), ry = xi.y - (xj.y + sy), rz = xi.z - (xj.z + sz);
+            float r2 = rx * rx + ry * ry + rz * rz;
+            if (r2 >= P.mcut2 || r2 == 0.f) continue;
+            float r = sqrtf(r2);
+            float ex = expf(P.beta * (P.r0 - r));
+            float fm = tb * (ex * ex - ex) / r;
+            fx += fm * rx; fy += fm * ry; fz += fm * rz;
+        }
+    });
+    aggforce[v] = tfloat3{fx, fy, fz};
+    tfloat3 F = force[p];
+    F.x += fx; F.y += fy; F.z += fz;
+    force[p] = F;
+}
+
+}  // namespace
+
+#define GC_PARAMS_OR_RETURN(P)          \
+    GcParams P;                         \
+    do { int _s = params(&P); if (_s) return _s; } while (0)
+
+extern "C" int gc_InvalidateTopology(void)
+{
+    int st = hip_check(hipStreamSynchronize(stream()), "InvalidateTopology sync");
+    free_topo();
+    return st;
+}
+
+extern "C" int gc_CalGeoPara(long stepcount, int ntri, int ncell, float* d_bbox, int* d_curpos,
+                             tint3* d_tri, tint2* d_beginendtri, tint3* d_nthrobound,
+                             tfloat3* d_totalcoor, tfloat3* d_trinordir, float* d_triarea,
+                             float* d_reftriarea, tfloat3* d_tricenter, float* d_refarea_para,
+                             float* d_refvol_para, float* d_area_para, float* d_vol_para,
+                             float* d_area, float* d_vol, float* d_pararea)
+{
+    if (ncell <= 0) return GC_OK;  // the reference calls it with nCell == 0 too
+    GC_PARAMS_OR_RETURN(P);
+    if (ntri <= 0) return set_error(GC_ERR_INVALID, "CalGeoPara: ntri <= 0 with cells");
+    if (!d_bbox || !d_curpos || !d_tri || !d_beginendtri || !d_nthrobound || !d_totalcoor ||
+        !d_trinordir || !d_triarea || !d_reftriarea || !d_tricenter || !d_refarea_para ||
+        !d_refvol_para || !d_area_para || !d_vol_para || !d_area || !d_vol)
+        return set_error(GC_ERR_INVALID, "CalGeoPara: null pointer");
+    hipStream_t s = stream();
+    k_geo_cell<<<ncell, BS, 0, s>>>(P, stepcount, d_bbox, d_curpos, d_tri, d_beginendtri,
+                                    d_nthrobound, d_totalcoor, d_trinordir, d_triarea,
+                                    d_reftriarea, d_tricenter, d_refarea_para, d_refvol_para,
+                                    d_area_para, d_vol_para, d_area, d_vol);
+    GC_LAUNCH_CHECK("CalGeoPara");
+    if (d_pararea) {
+        bool cached = g_topo.tri_ptr == (const void*)d_tri && g_topo.ntri == ntri && g_topo.nm == P.nm;
+        int st = cached ? GC_OK : ensure_topology(P.nm, 0, nullptr, ntri, d_tri);
+        if (st) return st;
+        k_par_area<<<blocks_for(P.nm, BS), BS, 0, s>>>(P.nm, g_topo.d_ts, g_topo.d_te, d_triarea,
+                                                       d_pararea);
+        GC_LAUNCH_CHECK("CalGeoPara pararea");
+    }
+    return GC_OK;
+}
+
+extern "C" int gc_SetMechPara(int ncell, int nedge, int6* d_edge, int* d_curpos,
+                              float* d_refedgelen, float* d_maxrefedgelen, float* d_bendmodpara,
+                              float* d_shearmodpara, float* d_hbending, float* d_powerindex,
+                              float* d_perlenWLC, float* d_hpow, float* d_reftriangle,
+                              tfloat3* d_trinordir, tfloat3* d_tricente, tfloat3* d_coor)
+{
+    (void)d_trinordir; (void)d_tricente;
+    if (ncell <= 0) return GC_OK;
+    GC_PARAMS_OR_RETURN(P);
+    if (nedge <= 0) return set_error(GC_ERR_INVALID, "SetMechPara: nedge <= 0 with cells");
+    if (!d_edge || !d_curpos || !d_refedgelen || !d_maxrefedgelen || !d_bendmodpara ||
+        !d_shearmodpara || !d_hbending || !d_powerindex || !d_perlenWLC || !d_hpow ||
+        !d_reftriangle || !d_coor)
+        return set_error(GC_ERR_INVALID, "SetMechPara: null pointer");
+    k_mech_para<<<blocks_for(nedge, BS), BS, 0, stream()>>>(P, nedge, d_edge, d_curpos, d_coor,
+        d_refedgelen, d_maxrefedgelen, d_bendmodpara, d_shearmodpara, d_hbending, d_powerindex,
+        d_perlenWLC, d_hpow, d_reftriangle);
+    GC_LAUNCH_CHECK("SetMechPara");
+    return GC_OK;
+}
+
+extern "C" int gc_CalCellDefForce(int ncell, unsigned nedge, unsigned ntri, unsigned ntotalpar,
+                                  unsigned ncellpar, float* d_bbox, int6* d_edge, tint3* d_tri,
+                                  tfloat3* d_coor, tfloat3* h_defforce, tfloat3* d_defforce,
+                                  tint3* d_nthrobound, float* d_maxrefedgelen, float* d_powerindex,
+                                  float* d_perlenWLC, float* d_hpow, tfloat3* d_trinordir,
+                                  tfloat3* d_tricenter, float* d_reftriangle, float* d_hbending,
+                                  float* d_triarea, float* d_reftriarea, float* d_hlarea_para,
+                                  float* d_hgarea_para, float* d_area_para, float* d_refarea_para,
+                                  float* d_hvol_para, float* d_vol_para, float* d_refvol_para,
+                                  tfloat3* d_totalforce, int* d_sortpart, int* d_curpos)
+{
+    (void)ntotalpar; (void)d_trinordir; (void)d_tricenter; (void)d_triarea; (void)d_sortpart;
+    if (ncell <= 0) return GC_OK;
+    GC_PARAMS_OR_RETURN(P);
+    if ((int)ncellpar != P.nm) return set_error(GC_ERR_INVALID, "CalCellDefForce: ncellpar != nCellPar");
+    if (!d_bbox || !d_edge || !d_tri || !d_coor || !d_defforce || !d_nthrobound ||
+        !d_maxrefedgelen || !d_powerindex || !d_perlenWLC || !d_hpow || !d_reftriangle ||
+        !d_hbending || !d_reftriarea || !d_hlarea_para || !d_hgarea_para || !d_area_para ||
+        !d_refarea_para || !d_hvol_para || !d_vol_para || !d_refvol_para || !d_totalforce ||
+        !d_curpos)
+        return set_error(GC_ERR_INVALID, "CalCellDefForce: null pointer");
+    int st = ensure_topology(P.nm, (int)nedge, d_edge, (int)ntri, d_tri);
+    if (st) return st;
+    DefArgs a;
+    a.edge = d_edge; a.tri = d_tri; a.coor = d_coor; a.nthro = d_nthrobound; a.curpos = d_curpos;
+    a.lmax = d_maxrefedgelen; a.mexp = d_powerindex; a.p = d_perlenWLC; a.kp = d_hpow;
+    a.th0 = d_reftriangle; a.kb = d_hbending; a.reftriarea = d_reftriarea; a.kd = d_hlarea_para;
+    a.ka = d_hgarea_para; a.Ac = d_area_para; a.A0c = d_refarea_para; a.kv = d_hvol_para;
+    a.Vc = d_vol_para; a.V0c = d_refvol_para;
+    a.es = g_topo.d_es; a.ee = g_topo.d_ee; a.ts = g_topo.d_ts; a.te = g_topo.d_te;
+    a.defforce = d_defforce; a.force = d_totalforce;
+    k_def_force<<<blocks_for(P.nm, BS), BS, 0, stream()>>>(P, P.nm, d_bbox, a);
+    GC_LAUNCH_CHECK("CalCellDefForce");
+    if (h_defforce) {
+        st = hip_check(hipMemcpyAsync(h_defforce, d_defforce, sizeof(tfloat3) * (size_t)P.nm,
+                                      hipMemcpyDeviceToHost, stream()), "DefForce D2H");
+        if (!st) st = hip_check(hipStreamSynchronize(stream()), "DefForce sync");
+    }
+    return st;
+}
+
+extern "C" int gc_CalCellAggForce(unsigned ncell, unsigned n, unsigned ncellpar,
+                                  tfloat3* d_totalcoor, tfloat3* d_aggforce, tint3* d_cc,
+                                  int* d_ncube, int* d_sortpart, int* d_celltab,
+                                  float* d_outerreigon, tint2* d_beginendcube,
+                                  tfloat3* d_totalforce, float* d_pararea)
+{
+    (void)d_pararea;
+    if (ncell == 0) return GC_OK;
+    GC_PARAMS_OR_RETURN(P);
+    if ((int)ncellpar != P.nm) return set_error(GC_ERR_INVALID, "CalCellAggForce: ncellpar != nCellPar");
+    if (!d_totalcoor || !d_aggforce || !d_cc || !d_ncube || !d_sortpart || !d_celltab ||
+        !d_outerreigon || !d_beginendcube || !d_totalforce)
+        return set_error(GC_ERR_INVALID, "CalCellAggForce: null pointer");
+    if (n == 0) return GC_OK;
+    k_agg_force<<<blocks_for(n, BS), BS, 0, stream()>>>((int)n, P, d_totalcoor, d_aggforce, d_cc,
+        d_ncube, d_sortpart, d_celltab, d_outerreigon, d_beginendcube, d_totalforce);
+    GC_LAUNCH_CHECK("CalCellAggForce");
+    return GC_OK;
+}

@@ -1,0 +1,64 @@
+"""The C-ABI library loads and exports every function include/gpucell/*.h
+declares (no GPU needed: nothing here launches a kernel)."""
+import ctypes
+import os
+import re
+
+from gpucell_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HDR = os.path.join(ROOT, "include", "gpucell", "gpucell.h")
+
+
+def declared_functions():
+    text = open(HDR).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gc_[A-Za-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_symbol_is_exported():
+    lib = abi.load()
+    names = declared_functions()
+    assert len(names) >= 35
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python prototypes cover exactly the declared set
+    assert sorted(abi.PROTOTYPES) == names
+
+
+def test_reference_entry_points_present():
+    names = set(declared_functions())
+    assert len(abi.REFERENCE_ENTRY_POINTS) == 20
+    assert set(abi.REFERENCE_ENTRY_POINTS) <= names
+
+
+def test_struct_layouts_match_header():
+    assert ctypes.sizeof(abi.Structconst) == 96
+    assert abi.Structconst.iSeed.offset == 24
+    assert abi.Structconst.CutRadius.offset == 32
+    assert abi.Structconst.Temp.offset == 88
+    assert ctypes.sizeof(abi.gc_ext_consts) == 48
+    assert ctypes.sizeof(abi.int6) == 24 and ctypes.sizeof(abi.tfloat4) == 16
+
+
+def test_host_only_calls():
+    lib = abi.load()
+    assert b"gfx950" in lib.gc_Version()
+    # grid sizing is host code: the reference's default domain -> 20 x 22 x 22
+    bbox = (ctypes.c_float * 6)(0, 20, -10, 10, -10, 10)
+    per = (ctypes.c_int * 3)(1, 0, 0)
+    nc = (ctypes.c_int * 3)()
+    outer = (ctypes.c_float * 3)()
+    inv = (ctypes.c_float * 3)()
+    K = ctypes.c_int()
+    assert lib.gc_ConfigDomain(bbox, 1.0, per, nc, outer, inv, ctypes.byref(K)) == 0
+    assert list(nc) == [20, 22, 22] and K.value == 9680
+    per2 = (ctypes.c_int * 3)(1, 1, 1)
+    bbox2 = (ctypes.c_float * 6)(0, 2, 0, 10, 0, 10)
+    assert lib.gc_ConfigDomain(bbox2, 1.0, per2, nc, outer, inv, ctypes.byref(K)) == -3
+    assert b"periodic" in lib.gc_GetLastError()
+    # a bad parameter block is rejected before any launch
+    c = abi.Structconst()
+    assert lib.gc_ConstsDataUp(ctypes.byref(c)) == -1
+    e = abi.default_ext()
+    assert list(e.periodic) == [1, 0, 0] and abs(e.wlc_ratio - 2.2) < 1e-6
