@@ -1,0 +1,158 @@
+"""SDPD step benchmark on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--case suspension_1m]
+
+A step is one Particle::UpdatePar (neighbour build, forces, integration) of
+the whole suspension, replayed from a HIP graph.  Inputs are generated on
+the host from a seed and resident in HBM before the timed region.  Prints
+one JSON line: whole-job particle-steps/s, the dominant kernel's HBM
+roofline (HIP events on the engine stream), the step-level roofline, and
+the FP64 OpenMP oracle timed on a bounded sample on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per particle-step of the whole step (five passes:
+# integrate+key 56, sort 18.3, gather+predict 124, density 16,
+# force+corrector 72) and per membrane vertex (440).
+STEP_BYTES_PER_PARTICLE = 286.0
+STEP_BYTES_PER_VERTEX = 440.0
+# Dominant kernel (SDPD pair force): reads position 12 B, velocity+density
+# 16 B, pressure 4 B, id 4 B; writes force 12 B, per particle.
+FORCE_BYTES_PER_PARTICLE = 48.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_case(name):
+    from gpucell_amd import casegen
+    if name == "suspension_1m":
+        return casegen.suspension_1m(), ("suspension_1m_100rbc: 230x60x60 duct (66x66 with 3 "
+                                         "frozen wall layers), 100 RBCs x 642 vertices at 20% "
+                                         "hematocrit, Poiseuille body force, aggregation off")
+    if name == "stenosis_2m":
+        return casegen.stenosis_2m(), ("stenosis_2m_300rbc: 460x60x60 duct with 50% cosine "
+                                       "stenosis, 300 RBCs at 30% hematocrit, Morse De=0.3kT")
+    if name == "box32":
+        return casegen.periodic_box(), "periodic 32^3 box, one RBC, reverse Poiseuille"
+    raise SystemExit(f"unknown case {name}")
+
+
+def cpu_baseline(case, seconds_target=15.0):
+    """FP64 OpenMP oracle on the same workload: as many whole steps as fit
+    in ~seconds_target (at least one), from the same initial state."""
+    from oracle import pyoracle as orc
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    o = orc.Oracle(case, threads=threads)
+    st = o.initial_state()
+    t0 = time.perf_counter()
+    o.step(st, 1)
+    t1 = time.perf_counter()
+    per = t1 - t0
+    n = max(0, int(seconds_target / max(per, 1e-6)) - 1)
+    n = min(n, 50)
+    t2 = time.perf_counter()
+    for _ in range(n):
+        o.step(st, 1)
+    t3 = time.perf_counter()
+    steps = n if n > 0 else 1
+    dt = (t3 - t2) if n > 0 else per
+    return {"value": case.N * steps / dt, "unit": "particle-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} full steps of the same {case.N}-particle case (FP64 oracle, "
+                      f"{threads} OpenMP threads, after 1 untimed step)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--case", default="suspension_1m")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--eager", action="store_true", help="launch kernels without a graph")
+    a = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(one suspension per GPU; see the sharding design)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from gpucell_amd.engine import Engine
+
+    log(f"[bench] generating case {a.case}")
+    t = time.perf_counter()
+    case, workload = make_case(a.case)
+    log(f"[bench] case ready: N={case.N} Nm={case.nCellPar} ({time.perf_counter() - t:.1f}s)")
+    eng = Engine(case)
+    log("[bench] engine created; warmup")
+    graph = not a.eager
+    eng.step(a.warmup, graph=graph)
+    eng.info()  # synchronises the engine stream
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    eng.step(a.steps, graph=graph)
+    eng.info()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / a.steps
+    S = eng.state()
+    import numpy as np
+    finite = bool(np.isfinite(S["pos"]).all() and np.isfinite(S["force"]).all())
+    log(f"[bench] {a.steps} steps: {ms:.4f} ms/step, finite={finite}")
+    prof = eng.profile(20)
+    log("[bench] per-phase us: " + json.dumps(prof))
+    dom = max(prof, key=prof.get)
+    force_us = prof.get("sdpd force", prof[dom])
+    N, Nm = case.N, case.nCellPar
+    achieved = FORCE_BYTES_PER_PARTICLE * N / (force_us * 1e-6) / 1e9
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", f"traffic_sdpd_force_{a.case}.json")
+    if os.path.exists(pmc):
+        traffic = json.load(open(pmc)).get("bytes_per_launch")
+    step_bytes = STEP_BYTES_PER_PARTICLE * N + STEP_BYTES_PER_VERTEX * Nm
+    step_gbs = step_bytes / (ms * 1e-3) / 1e9
+    out = {
+        "metric": "SDPD particle-steps/s on 1 GPU (% HBM roofline); surrogate vs plain speedup",
+        "value": N * a.steps / (t1 - t0),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded lattice fluid, walls and biconcave RBCs)",
+        "config": {"workload": workload, "N": N, "Nm": Nm, "h": float(case.consts.CutRadius),
+                   "dt": float(case.consts.TimeStep), "graph": graph, "finite": finite},
+        "roofline": {"bound": "hbm", "kernel": "sdpd force", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "avg_us": force_us},
+        "step_roofline": {"bytes_per_step": step_bytes, "achieved": step_gbs,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
+        "phases_us": prof,
+    }
+    if not a.no_cpu:
+        log("[bench] CPU baseline (FP64 oracle)")
+        out["cpu_baseline"] = cpu_baseline(case)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

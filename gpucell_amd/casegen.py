@@ -230,11 +230,22 @@ def _lattice(lo, hi, dx):
     return g
 
 
-def _place_cells(rng, n_cells, shape, inside, gap, lo, hi, max_tries=20000):
-    """Random sequential placement of rigid copies of `shape` (Nv,3)."""
+def _place_cells(rng, n_cells, shape, inside, gap, lo, hi, max_tries=200000, period=None):
+    """Random sequential placement of rigid copies of `shape` (Nv,3): a
+    candidate (uniform centre, uniform random rotation) is kept when it lies
+    inside the domain and no vertex comes within `gap` of a placed cell.
+    `period` (3 floats or None per axis) makes the distance test periodic;
+    coordinates stay unwrapped (as the reference's loader keeps them)."""
+    from scipy.spatial import cKDTree
+    big = 1e6
+    box = np.array([p if p else big for p in (period or (None, None, None))], dtype=np.float64)
+    shift = np.where(box == big, big / 2, 0.0)
+
+    def wrap(X):
+        return np.mod(X + shift, box)
+
     placed = []
-    rad = float(np.max(np.linalg.norm(shape, axis=1)))
-    centres = []
+    tree = None
     tries = 0
     while len(placed) < n_cells:
         tries += 1
@@ -245,52 +256,22 @@ def _place_cells(rng, n_cells, shape, inside, gap, lo, hi, max_tries=20000):
         X = shape @ R.T + c
         if not inside(X):
             continue
-        ok = True
-        for c2, X2 in zip(centres, placed):
-            if np.linalg.norm(c2 - c) > 2 * rad + gap:
+        if tree is not None:
+            d, _ = tree.query(wrap(X), k=1, distance_upper_bound=gap)
+            if np.isfinite(d).any():
                 continue
-            d2 = ((X[:, None, :] - X2[None, :, :]) ** 2).sum(-1)
-            if d2.min() < gap * gap:
-                ok = False
-                break
-        if ok:
-            placed.append(X)
-            centres.append(c)
+        placed.append(X)
+        tree = cKDTree(wrap(np.concatenate(placed)), boxsize=box)
     return placed
 
 
 def _remove_near(points: np.ndarray, verts: np.ndarray, dist: float) -> np.ndarray:
-    """Mask of points farther than `dist` from every vertex (grid hashing)."""
+    """Mask of points farther than `dist` from every vertex."""
     if verts.shape[0] == 0:
         return np.ones(points.shape[0], dtype=bool)
-    cell = dist
-    keys = np.floor(verts / cell).astype(np.int64)
-    table = {}
-    for i, k in enumerate(map(tuple, keys)):
-        table.setdefault(k, []).append(i)
-    pk = np.floor(points / cell).astype(np.int64)
-    keep = np.ones(points.shape[0], dtype=bool)
-    cand = np.unique(pk, axis=0)
-    near_keys = set()
-    for k in map(tuple, cand):
-        for dx in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dz in (-1, 0, 1):
-                    if (k[0] + dx, k[1] + dy, k[2] + dz) in table:
-                        near_keys.add(k)
-    idx = np.nonzero([tuple(k) in near_keys for k in pk])[0]
-    for i in idx:
-        k = pk[i]
-        vs = []
-        for dx in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dz in (-1, 0, 1):
-                    vs += table.get((k[0] + dx, k[1] + dy, k[2] + dz), [])
-        if vs:
-            d2 = ((verts[vs] - points[i]) ** 2).sum(-1).min()
-            if d2 < dist * dist:
-                keep[i] = False
-    return keep
+    from scipy.spatial import cKDTree
+    d, _ = cKDTree(verts).query(points, k=1, distance_upper_bound=dist)
+    return ~np.isfinite(d)
 
 
 def _assemble(name, ph, ext, bbox, bound1, bound2, fluid, cells, tri0, edge0, rng, jitter, dx):
@@ -405,15 +386,17 @@ def channel(length: int, width: int, n_rbc: int, hematocrit: float | None = None
             vol_unit = _mesh_volume(unit, F0)
             rbc_diameter = (hematocrit * fluid.shape[0] / (n_rbc * vol_unit)) ** (1.0 / 3.0)
     shape = biconcave(V0, rbc_diameter)
-    gap = 1.0
+    gap = 0.8
 
     def inside(X):
-        w = wall_half(X[:, 0])
-        return bool(np.all(np.maximum(np.abs(X[:, 1]), np.abs(X[:, 2])) < w - gap) and
-                    np.all((X[:, 0] >= 0) & (X[:, 0] < Lx)))
+        w = wall_half(np.mod(X[:, 0], Lx))
+        return bool(np.all(np.maximum(np.abs(X[:, 1]), np.abs(X[:, 2])) < w - gap))
 
     cells = _place_cells(rng, n_rbc, shape, inside, gap, np.array([0, -half, -half]),
-                         np.array([Lx, half, half])) if n_rbc else []
+                         np.array([Lx, half, half]), period=(Lx, None, None)) if n_rbc else []
+    # keep each cell whole but put its centre in [0, Lx): vertices may lie
+    # outside the box in x; the first step wraps them (as the reference)
+    cells = [X - Lx * np.floor(X.mean(0)[0] / Lx) * np.array([1.0, 0, 0]) for X in cells]
     E0 = edge_records(F0)
     case = _assemble(name, ph, ext, bbox, np.zeros((0, 3)), walls, fluid, cells, F0, E0, rng,
                      jitter, 1.0)
@@ -431,16 +414,17 @@ def mesh_volume(V, F):
 
 
 def suspension_1m(seed: int = 20261203, physics: Physics | None = None, noise: bool = True) -> Case:
-    """1M fluid+wall particles, 100 RBCs at 20 % hematocrit, 3 wall layers:
-    a 472 x 40 x 40 duct (46 x 46 with walls)."""
-    return channel(472, 40, 100, hematocrit=0.20, seed=seed, physics=physics, noise=noise,
+    """1M fluid+wall particles (1,001,880 lattice sites), 100 RBCs at 20 %
+    hematocrit, 3 wall layers: a 230 x 60 x 60 duct (66 x 66 with walls)."""
+    return channel(230, 60, 100, hematocrit=0.20, seed=seed, physics=physics, noise=noise,
                    name="suspension_1m_100rbc")
 
 
 def stenosis_2m(seed: int = 20261203, physics: Physics | None = None, noise: bool = True) -> Case:
     """2M particles, 300 RBCs at 30 % hematocrit, 50 % cosine stenosis,
-    Morse aggregation De = 0.3 kT, r0 = 0.3, beta = 1.5."""
+    Morse aggregation De = 0.3 kT, r0 = 0.3, beta = 1.5: a 460 x 60 x 60
+    duct (66 x 66 with walls)."""
     ph = physics or Physics()
     ph = dataclasses.replace(ph, De=0.3 * ph.kT, r0=0.3, beta=1.5, mcut=min(1.5, ph.h))
-    return channel(944, 40, 300, hematocrit=0.30, seed=seed, physics=ph, noise=noise,
+    return channel(460, 60, 300, hematocrit=0.30, seed=seed, physics=ph, noise=noise,
                    stenosis=0.5, name="stenosis_2m_300rbc")

@@ -1,15 +1,28 @@
 // gc_engine.hip -- the engine: library-owned state and the step of
-// Particle::UpdatePar (src/Particle.cu:1310-1329) as a fixed launch sequence
-// on one stream, capturable into a HIP graph.
+// Particle::UpdatePar (src/Particle.cu:1310-1329) as a fixed sequence of
+// fused kernels on one stream, captured into a HIP graph for replay.
 //
-// Differences from the one-call-per-entry-point sequence, all internal:
-//   * positions are accumulated in FP64 (pos64); the FP32 coordinates the
-//     neighbour kernels read are rounded from it once per step;
-//   * integrate + periodic wrap + cube key are one kernel;
-//   * the cell list is a counting sort keyed by (cube, id) with the cube
-//     offsets of the scan used directly as BeginEndCube (no host sync);
-//   * the sorted gather and the velocity predictor are one kernel;
-//   * the noise counter lives in device memory, so graph replays advance it.
+// Per step (reference launch numbers of UpdatePar in brackets):
+//   k2_integrate   [1,2,3]  r += dt v + dt^2/2 F/m in FP64, periodic wrap,
+//                           cube key, histogram rank (atomic)
+//   scan                    cube offsets (exclusive scan of the histogram)
+//   k2_scatter     [4]      (cube, id) pairs into cube segments
+//   k2_sortgather  [4,5,6,8] rank within the cube by id -> final slot;
+//                           gather of the state; Groot-Warren predictor
+//   k2_density     [7,9]    number density (moving) and no-slip wall
+//                           velocity (Bound2) in one neighbour pass
+//   k2_force       [10-13,17] EOS on the fly, SDPD pair force, Bound1
+//                           repulsion, body force; corrector for fluid
+//   k_geo_cell     [14]     membrane geometry, FP64 cell area/volume
+//   k2_agg         [16]     Morse between cells (when De != 0)
+//   k_def_force    [15,17]  membrane forces, corrector for membrane
+//   k2_wallvel     [18]     wall velocity from the corrected flow
+//
+// Layouts (slot order): posid float4 (x, y, z, id bits), velrho float4
+// (predicted v, rho) read by neighbour loops, vout float4 (v(t+dt), rho) =
+// the reference's Velrhop after the step, pos64 3 x double (accumulator),
+// coor/force/velhalf in the TypesDef float3 layout.  BeginEndCube is the
+// scan's offsets array: cube k holds slots [offs[k], offs[k+1]).
 #include "gc_kernels.h"
 
 #include <math.h>
@@ -22,24 +35,72 @@ namespace {
 
 constexpr int BS = GC_BS;
 
-__global__ __launch_bounds__(BS) void k_e_integrate(int n, GcParams P, int move,
-    double* __restrict__ pos64, const tfloat4* __restrict__ velrhop,
-    const tfloat3* __restrict__ force, const int* __restrict__ sortpart, tint3* __restrict__ nthro,
-    const float* __restrict__ bbox, const float* __restrict__ invw, const int* __restrict__ ncube,
-    int* __restrict__ key)
+struct Grid2 {
+    int nc[3];
+    float L[3];
+};
+
+__device__ __forceinline__ void key_to_cc(int k, const Grid2& g, int& cx, int& cy, int& cz)
+{
+    cx = k % g.nc[0];
+    int t = k / g.nc[0];
+    cy = t % g.nc[1];
+    cz = t / g.nc[1];
+}
+
+// Contiguous slot ranges of the 27 neighbour cubes (x-fastest keys: one
+// range per non-wrapping (y, z) row), with the image shift of each range.
+template <class Fn>
+__device__ __forceinline__ void visit2(const Grid2& g, const int* per, const int* __restrict__ offs,
+                                       int cx, int cy, int cz, Fn&& fn)
+{
+    for (int dz = -1; dz <= 1; ++dz) {
+        int z = cz + dz;
+        float sz = 0.0f;
+        if (z < 0) { if (!per[2]) continue; z += g.nc[2]; sz = -g.L[2]; }
+        else if (z >= g.nc[2]) { if (!per[2]) continue; z -= g.nc[2]; sz = g.L[2]; }
+        for (int dy = -1; dy <= 1; ++dy) {
+            int y = cy + dy;
+            float sy = 0.0f;
+            if (y < 0) { if (!per[1]) continue; y += g.nc[1]; sy = -g.L[1]; }
+            else if (y >= g.nc[1]) { if (!per[1]) continue; y -= g.nc[1]; sy = g.L[1]; }
+            int row = (z * g.nc[1] + y) * g.nc[0];
+            int xlo = cx - 1, xhi = cx + 1;
+            if (xlo < 0) {
+                if (per[0]) fn(offs[row + g.nc[0] - 1], offs[row + g.nc[0]], -g.L[0], sy, sz);
+                xlo = 0;
+            }
+            if (xhi >= g.nc[0]) {
+                if (per[0]) fn(offs[row], offs[row + 1], g.L[0], sy, sz);
+                xhi = g.nc[0] - 1;
+            }
+            fn(offs[row + xlo], offs[row + xhi + 1], 0.0f, sy, sz);
+        }
+    }
+}
+
+__device__ __forceinline__ float ext_force_x(const GcParams& P, float y)
+{
+    float f = P.fext;
+    return (P.force_mode == GC_FORCE_REVERSE_POISEUILLE && y >= P.force_split) ? -f : f;
+}
+
+// [1,2,3] integrate (FP64), wrap, key, histogram rank
+__global__ __launch_bounds__(BS) void k2_integrate(int n, GcParams P, int move,
+    double* __restrict__ pos64, const float4* __restrict__ vout, const tfloat3* __restrict__ force,
+    const int* __restrict__ sp, tint3* __restrict__ nthro, const float* __restrict__ bbox,
+    const float* __restrict__ invw, const int* __restrict__ ncube, int* __restrict__ key,
+    int* __restrict__ rank, int* __restrict__ cnt)
 {
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
-    int id = sortpart[p];
+    int id = sp[p];
     double x = pos64[3 * p], y = pos64[3 * p + 1], z = pos64[3 * p + 2];
     if (move && id >= P.nf) {
         double dt = P.dt, a = 0.5 * (double)P.dt * (double)P.dt * (double)P.inv_m;
-        tfloat4 v = velrhop[p];
+        float4 v = vout[p];
         tfloat3 f = force[p];
-        x += dt * v.x + a * f.x;
-        y += dt * v.y + a * f.y;
-        z += dt * v.z + a * f.z;
-        double u[3] = {x, y, z};
+        double u[3] = {x + (dt * v.x + a * f.x), y + (dt * v.y + a * f.y), z + (dt * v.z + a * f.z)};
         tint3 im = nthro[p];
         int img[3] = {im.x, im.y, im.z};
         bool wrapped = false;
@@ -59,48 +120,255 @@ __global__ __launch_bounds__(BS) void k_e_integrate(int n, GcParams P, int move,
     int cx = cube_coord((float)x, bbox[0], P.off[0], invw[0], nx);
     int cy = cube_coord((float)y, bbox[2], P.off[1], invw[1], ny);
     int cz = cube_coord((float)z, bbox[4], P.off[2], invw[2], nz);
-    key[p] = cx + nx * (cy + ny * cz);
+    int k = cx + nx * (cy + ny * cz);
+    key[p] = k;
+    rank[p] = atomicAdd(&cnt[k], 1);
 }
 
-// New slot s takes the particle whose old slot is curpos[id]; applies the
-// Groot-Warren predictor (PredictFluidVel) on the way.
-__global__ __launch_bounds__(BS) void k_e_gather(int n, GcParams P, int predict,
-    const int* __restrict__ sortpart_new, int* __restrict__ curpos, const int* __restrict__ key_new,
-    const int* __restrict__ ncube, const double* __restrict__ pos_old, double* __restrict__ pos_new,
-    const tfloat4* __restrict__ vr_old, tfloat4* __restrict__ vr_new,
-    const tint3* __restrict__ nt_old, tint3* __restrict__ nt_new, const tfloat3* __restrict__ force,
-    tfloat3* __restrict__ velhalf, tfloat3* __restrict__ coor, tint3* __restrict__ cc)
+// [4] (cube, id) into cube segments; clears the histogram for the next step
+// (every non-empty cube has exactly one rank-0 member).
+__global__ __launch_bounds__(BS) void k2_scatter(int n, const int* __restrict__ key,
+    const int* __restrict__ rank, const int* __restrict__ sp, const int* __restrict__ offs,
+    int* __restrict__ tk, int* __restrict__ tv, int* __restrict__ cnt)
 {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    int id = sortpart_new[s];
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int k = key[p], r = rank[p];
+    int q = offs[k] + r;
+    tk[q] = k;
+    tv[q] = sp[p];
+    if (r == 0) cnt[k] = 0;
+}
+
+// [4,5,6,8] final slot = cube start + number of smaller ids in the cube;
+// gather of the particle from its old slot curpos[id]; predictor.
+__global__ __launch_bounds__(BS) void k2_sortgather(int n, GcParams P, int predict,
+    const int* __restrict__ tk, const int* __restrict__ tv, const int* __restrict__ offs,
+    int* __restrict__ curpos, const double* __restrict__ pos_old, double* __restrict__ pos_new,
+    const float4* __restrict__ vout, const tfloat3* __restrict__ force,
+    const tint3* __restrict__ nt_old, tint3* __restrict__ nt_new, int* __restrict__ sp_new,
+    int* __restrict__ key_new, float4* __restrict__ posid, tfloat3* __restrict__ coor,
+    float4* __restrict__ velrho, tfloat3* __restrict__ velhalf)
+{
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    int k = tk[q], id = tv[q];
+    int b = offs[k], e = offs[k + 1];
+    int r = 0;
+    for (int t = b; t < e; ++t) r += tv[t] < id;
+    int s = b + r;
     int o = curpos[id];
     double x = pos_old[3 * o], y = pos_old[3 * o + 1], z = pos_old[3 * o + 2];
     pos_new[3 * s] = x; pos_new[3 * s + 1] = y; pos_new[3 * s + 2] = z;
-    coor[s] = tfloat3{(float)x, (float)y, (float)z};
-    int k = key_new[s], nx = ncube[0], ny = ncube[1];
-    cc[s] = tint3{k % nx, (k / nx) % ny, k / (nx * ny)};
+    float fx = (float)x, fy = (float)y, fz = (float)z;
+    posid[s] = make_float4(fx, fy, fz, __int_as_float(id));
+    coor[s] = tfloat3{fx, fy, fz};
     nt_new[s] = nt_old[o];
-    tfloat4 v = vr_old[o];
-    tfloat3 f = force[o];
+    sp_new[s] = id;
+    key_new[s] = k;
+    float4 v = vout[o];
     if (predict && id >= P.nf) {
+        tfloat3 f = force[o];
         float h = 0.5f * P.dt * P.inv_m, l = P.lam * P.dt * P.inv_m;
         velhalf[s] = tfloat3{v.x + h * f.x, v.y + h * f.y, v.z + h * f.z};
         v.x += l * f.x; v.y += l * f.y; v.z += l * f.z;
     } else {
         velhalf[s] = tfloat3{v.x, v.y, v.z};
     }
-    vr_new[s] = v;
+    velrho[s] = v;
     curpos[id] = s;
 }
 
-__global__ void k_e_begin_end(int K, const int* __restrict__ offs, tint2* __restrict__ be)
+// [7,9] density of moving particles; velocity of Bound2 walls from the
+// predicted velocities of their moving neighbours.
+__global__ __launch_bounds__(BS) void k2_density(int n, GcParams P, Grid2 g,
+    const float4* __restrict__ posid, float4* __restrict__ velrho, const int* __restrict__ offs,
+    const int* __restrict__ key)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < K) be[k] = tint2{offs[k], offs[k + 1]};
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float4 pi = posid[p];
+    int id = __float_as_int(pi.w);
+    if (id < P.nb2) return;
+    int cx, cy, cz;
+    key_to_cc(key[p], g, cx, cy, cz);
+    const float ih = 1.0f / P.h;
+    if (id >= P.nf) {
+        float d = 0.0f;  // self term added below
+        visit2(g, P.per, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
+            for (int j = jb; j < je; ++j) {
+                float4 pj = posid[j];
+                float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
+                float r2 = rx * rx + ry * ry + rz * rz;
+                if (r2 < P.h2 && __float_as_int(pj.w) >= P.nb2 && j != p) {
+                    float q = sqrtf(r2) * ih, om = 1.0f - q;
+                    d += (1.0f + 3.0f * q) * om * om * om;
+                }
+            }
+        });
+        velrho[p].w = P.m * P.wnorm * (d + 1.0f);
+    } else {
+        float sw = 0.f, svx = 0.f, svy = 0.f, svz = 0.f;
+        visit2(g, P.per, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
+            for (int j = jb; j < je; ++j) {
+                float4 pj = posid[j];
+                float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
+                float r2 = rx * rx + ry * ry + rz * rz;
+                if (r2 < P.h2 && __float_as_int(pj.w) >= P.nf) {
+                    float q = sqrtf(r2) * ih, om = 1.0f - q;
+                    float w = (1.0f + 3.0f * q) * om * om * om;
+                    float4 vj = velrho[j];
+                    sw += w; svx += w * vj.x; svy += w * vj.y; svz += w * vj.z;
+                }
+            }
+        });
+        float s = sw > 0.f ? -1.0f / sw : 0.0f;
+        float4 v = velrho[p];
+        v.x = svx * s; v.y = svy * s; v.z = svz * s;
+        velrho[p] = v;
+    }
 }
 
-__global__ void k_e_bump(unsigned long long* ctr) { *ctr += 1ull; }
+// [10-13,17] SDPD pair force with the pressure from the EOS on the fly,
+// Bound1 repulsion, body force; corrector for fluid (membrane vertices get
+// it after their membrane force).
+__global__ __launch_bounds__(BS) void k2_force(int n, GcParams P, Grid2 g, int driveon,
+    const float4* __restrict__ posid, const float4* __restrict__ velrho,
+    const int* __restrict__ offs, const int* __restrict__ key, const tfloat3* __restrict__ velhalf,
+    tfloat3* __restrict__ force, float4* __restrict__ vout, const unsigned long long* __restrict__ ctr)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float4 pi = posid[p];
+    int id = __float_as_int(pi.w);
+    if (id < P.nf) { force[p] = tfloat3{0.f, 0.f, 0.f}; return; }
+    unsigned long long c = *ctr;
+    P.ctr_lo = (uint32_t)c;
+    P.ctr_hi = (uint32_t)(c >> 32);
+    float4 vi = velrho[p];
+    float di = vi.w * P.inv_m;
+    float pi_d2 = (id < P.nc ? P.c1sq : P.c2sq) * vi.w / (di * di);
+    int cx, cy, cz;
+    key_to_cc(key[p], g, cx, cy, cz);
+    const float ih = 1.0f / P.h;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    visit2(g, P.per, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
+        for (int j = jb; j < je; ++j) {
+            float4 pj = posid[j];
+            float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
+            float r2 = rx * rx + ry * ry + rz * rz;
+            if (r2 >= P.h2 || r2 == 0.f) continue;
+            int idj = __float_as_int(pj.w);
+            float r = sqrtf(r2);
+            float om = 1.0f - r * ih;
+            float F = P.fnorm * om * om;
+            if (idj < P.nb2) {  // Bound1 wall: repulsion only
+                float kk = P.krep * F;
+                fx += kk * rx; fy += kk * ry; fz += kk * rz;
+                continue;
+            }
+            float4 vj = velrho[j];
+            float dj = vj.w * P.inv_m;
+            float cC = (pi_d2 + (idj < P.nc ? P.c1sq : P.c2sq) * vj.w / (dj * dj)) * F;
+            float ir = 1.0f / r;
+            float ex = rx * ir, ey = ry * ir, ez = rz * ir;
+            float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+            float ev = ex * vx + ey * vy + ez * vz;
+            float gg = F / (di * dj);
+            float ga = gg * P.visc_a, gb = gg * P.visc_b * ev;
+            fx += cC * rx - (ga * vx + gb * ex);
+            fy += cC * ry - (ga * vy + gb * ey);
+            fz += cC * rz - (ga * vz + gb * ez);
+            if (P.noise_on) {
+                float n0, n1, n2;
+                pair_normals(P, (uint32_t)id, (uint32_t)idj, n0, n1, n2);
+                float exi = ex * n0 + ey * n1 + ez * n2;
+                float amp = sqrtf(P.two_kT_over_dt * gg);
+                amp = id < idj ? amp : -amp;
+                float pa = amp * P.sq_a, pe = amp * (P.sq_ab - P.sq_a) * exi;
+                fx += pa * n0 + pe * ex;
+                fy += pa * n1 + pe * ey;
+                fz += pa * n2 + pe * ez;
+            }
+        }
+    });
+    if (driveon) fx += ext_force_x(P, pi.y);
+    force[p] = tfloat3{fx, fy, fz};
+    if (id < P.nc) {
+        float h = 0.5f * P.dt * P.inv_m;
+        tfloat3 vh = velhalf[p];
+        vout[p] = make_float4(vh.x + h * fx, vh.y + h * fy, vh.z + h * fz, vi.w);
+    }
+}
+
+// [16] Morse between membrane vertices of different cells
+__global__ __launch_bounds__(BS) void k2_agg(int n, GcParams P, Grid2 g,
+    const float4* __restrict__ posid, const int* __restrict__ offs, const int* __restrict__ key,
+    const int* __restrict__ celltab, tfloat3* __restrict__ force, tfloat3* __restrict__ aggforce)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float4 pi = posid[p];
+    int id = __float_as_int(pi.w);
+    if (id < P.nc) return;
+    int cell = celltab[id - P.nc];
+    int cx, cy, cz;
+    key_to_cc(key[p], g, cx, cy, cz);
+    const float tb = 2.0f * P.beta * P.De;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    visit2(g, P.per, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
+        for (int j = jb; j < je; ++j) {
+            float4 pj = posid[j];
+            int idj = __float_as_int(pj.w);
+            if (idj < P.nc) continue;
+            float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
+            float r2 = rx * rx + ry * ry + rz * rz;
+            if (r2 >= P.mcut2 || r2 == 0.f || celltab[idj - P.nc] == cell) continue;
+            float r = sqrtf(r2);
+            float ex = expf(P.beta * (P.r0 - r));
+            float fm = tb * (ex * ex - ex) / r;
+            fx += fm * rx; fy += fm * ry; fz += fm * rz;
+        }
+    });
+    aggforce[id - P.nc] = tfloat3{fx, fy, fz};
+    tfloat3 F = force[p];
+    F.x += fx; F.y += fy; F.z += fz;
+    force[p] = F;
+}
+
+// [18] wall velocity from the corrected velocities; walls' entry of vout
+__global__ __launch_bounds__(BS) void k2_wallvel(int n, GcParams P, Grid2 g,
+    const float4* __restrict__ posid, float4* __restrict__ vout, const int* __restrict__ offs,
+    const int* __restrict__ key)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    float4 pi = posid[p];
+    int id = __float_as_int(pi.w);
+    if (id >= P.nf) return;
+    if (id < P.nb2) { vout[p] = make_float4(0.f, 0.f, 0.f, P.rho0); return; }
+    int cx, cy, cz;
+    key_to_cc(key[p], g, cx, cy, cz);
+    const float ih = 1.0f / P.h;
+    float sw = 0.f, svx = 0.f, svy = 0.f, svz = 0.f;
+    visit2(g, P.per, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
+        for (int j = jb; j < je; ++j) {
+            float4 pj = posid[j];
+            float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
+            float r2 = rx * rx + ry * ry + rz * rz;
+            if (r2 < P.h2 && __float_as_int(pj.w) >= P.nf) {
+                float q = sqrtf(r2) * ih, om = 1.0f - q;
+                float w = (1.0f + 3.0f * q) * om * om * om;
+                float4 vj = vout[j];
+                sw += w; svx += w * vj.x; svy += w * vj.y; svz += w * vj.z;
+            }
+        }
+    });
+    float s = sw > 0.f ? -1.0f / sw : 0.0f;
+    vout[p] = make_float4(svx * s, svy * s, svz * s, P.rho0);
+}
+
+__global__ void k2_bump(unsigned long long* ctr) { *ctr += 1ull; }
 
 template <class T>
 int dalloc(T** p, size_t count)
@@ -131,14 +399,11 @@ struct gc_engine {
     int cur = 0;
     std::vector<void*> allocs;
 
-    // grid
     float *d_bbox, *d_outer, *d_invw; int* d_ncube;
-    // slot state (double-buffered where the gather reads and writes)
-    double* d_pos64[2]; tfloat4* d_vr[2]; tint3* d_nt[2]; int* d_sp[2];
-    tfloat3 *d_coor, *d_velhalf, *d_force; float* d_press; tint3* d_cc;
-    int *d_key, *d_cubepart, *d_curpos;
-    tint2* d_be;
-    int *d_cnt, *d_offs, *d_rank, *d_tk, *d_tv, *d_scanws;
+    double* d_pos64[2]; tint3* d_nt[2]; int* d_sp[2];
+    float4 *d_posid, *d_velrho, *d_vout;
+    tfloat3 *d_coor, *d_velhalf, *d_force;
+    int *d_key0, *d_key, *d_rank, *d_cnt, *d_offs, *d_tk, *d_tv, *d_curpos, *d_scanws;
     unsigned long long* d_ctr;
     // membrane
     tint3* d_tri; int6* d_edge; int* d_celltab; tint2* d_betri;
@@ -148,7 +413,6 @@ struct gc_engine {
     float *d_refarea_para, *d_refvol_para, *d_area_para, *d_vol_para, *d_area, *d_vol, *d_pararea;
     tfloat3 *d_defforce, *d_aggforce;
     int *d_es, *d_ee, *d_ts, *d_te;
-    // graph
     hipGraphExec_t gexec = nullptr;
     int g_driveon = -1;
 
@@ -169,24 +433,32 @@ GcParams engine_params(const gc_engine* e)
     return P;
 }
 
-// Cell list: keys from the (optionally advanced) FP64 positions, counting
-// sort by (cube, id), cube offsets -> BeginEndCube, sorted gather.
+Grid2 engine_grid(const gc_engine* e)
+{
+    Grid2 g;
+    for (int d = 0; d < 3; ++d) { g.nc[d] = e->ncube[d]; g.L[d] = e->outer[d]; }
+    return g;
+}
+
+// Neighbour build: keys from the (optionally advanced) FP64 positions,
+// counting sort by (cube, id), gather (+ predictor).
 int launch_build(gc_engine* e, const GcParams& P, int move, int predict)
 {
     const int N = e->N, c = e->cur, nx = 1 - c;
     hipStream_t s = e->s;
-    k_e_integrate<<<blocks_for(N, BS), BS, 0, s>>>(N, P, move, e->d_pos64[c], e->d_vr[c],
-        e->d_force, e->d_sp[c], e->d_nt[c], e->d_bbox, e->d_invw, e->d_ncube, e->d_key);
+    const int gb = blocks_for(N, BS);
+    k2_integrate<<<gb, BS, 0, s>>>(N, P, move, e->d_pos64[c], e->d_vout, e->d_force, e->d_sp[c],
+        e->d_nt[c], e->d_bbox, e->d_invw, e->d_ncube, e->d_key0, e->d_rank, e->d_cnt);
     GC_LAUNCH_CHECK("engine integrate");
-    int st = counting_sort(N, e->K, e->d_key, e->d_sp[c], e->d_cubepart, e->d_sp[nx], e->d_cnt,
-                           e->d_offs, e->d_rank, e->d_tk, e->d_tv, s, e->d_scanws);
+    int st = exclusive_scan(e->d_cnt, e->d_offs, e->K, s, e->d_scanws);
     if (st) return st;
-    k_e_begin_end<<<blocks_for(e->K, 256), 256, 0, s>>>(e->K, e->d_offs, e->d_be);
-    GC_LAUNCH_CHECK("engine begin/end");
-    k_e_gather<<<blocks_for(N, BS), BS, 0, s>>>(N, P, predict, e->d_sp[nx], e->d_curpos,
-        e->d_cubepart, e->d_ncube, e->d_pos64[c], e->d_pos64[nx], e->d_vr[c], e->d_vr[nx],
-        e->d_nt[c], e->d_nt[nx], e->d_force, e->d_velhalf, e->d_coor, e->d_cc);
-    GC_LAUNCH_CHECK("engine gather");
+    k2_scatter<<<gb, BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[c], e->d_offs, e->d_tk, e->d_tv,
+                                 e->d_cnt);
+    GC_LAUNCH_CHECK("engine scatter");
+    k2_sortgather<<<gb, BS, 0, s>>>(N, P, predict, e->d_tk, e->d_tv, e->d_offs, e->d_curpos,
+        e->d_pos64[c], e->d_pos64[nx], e->d_vout, e->d_force, e->d_nt[c], e->d_nt[nx], e->d_sp[nx],
+        e->d_key, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf);
+    GC_LAUNCH_CHECK("engine sort/gather");
     e->cur = nx;
     return GC_OK;
 }
@@ -199,58 +471,44 @@ struct Prof {
 
 void mark(Prof* pr, hipStream_t s, const char* name)
 {
-    if (!pr) return;
+    if (!pr || pr->n >= 32) return;
     (void)hipEventRecord(pr->ev[pr->n], s);
     pr->name[pr->n] = name;
     pr->n++;
 }
 
-// One Particle::UpdatePar.
 int launch_step(gc_engine* e, int driveon, Prof* pr)
 {
     GcParams P = engine_params(e);
+    Grid2 g = engine_grid(e);
     const int N = e->N;
     hipStream_t s = e->s;
+    const int gb = blocks_for(N, BS);
     mark(pr, s, "start");
     int st = launch_build(e, P, 1, 1);
     if (st) return st;
     mark(pr, s, "neighbour build+predict");
     const int c = e->cur;
-    k_density<<<blocks_for(N, BS), BS, 0, s>>>(N, P, e->d_coor, e->d_vr[c], e->d_cc, e->d_ncube,
-                                               e->d_outer, e->d_sp[c], e->d_be);
+    k2_density<<<gb, BS, 0, s>>>(N, P, g, e->d_posid, e->d_velrho, e->d_offs, e->d_key);
     GC_LAUNCH_CHECK("engine density");
-    mark(pr, s, "density");
-    if (P.nf > P.nb2) {
-        k_bound2_vel<<<blocks_for(N, BS), BS, 0, s>>>(N, P, e->d_coor, e->d_vr[c], e->d_cc,
-                                                      e->d_ncube, e->d_outer, e->d_sp[c], e->d_be);
-        GC_LAUNCH_CHECK("engine bound2 vel");
-    }
-    mark(pr, s, "bound2 velocity");
-    k_eos<<<blocks_for(N, BS), BS, 0, s>>>(N, P, e->d_sp[c], e->d_vr[c], e->d_press);
-    GC_LAUNCH_CHECK("engine eos");
-    mark(pr, s, "eos");
-    k_sdpd_force<<<blocks_for(N, BS), BS, 0, s>>>(N, P, e->d_coor, e->d_vr[c], e->d_cc, e->d_ncube,
-                                                  e->d_sp[c], e->d_outer, e->d_be, e->d_force,
-                                                  e->d_press, e->d_ctr);
-    GC_LAUNCH_CHECK("engine sdpd force");
-    k_e_bump<<<1, 1, 0, s>>>(e->d_ctr);
+    mark(pr, s, "density+wall velocity");
+    k2_force<<<gb, BS, 0, s>>>(N, P, g, driveon, e->d_posid, e->d_velrho, e->d_offs, e->d_key,
+                               e->d_velhalf, e->d_force, e->d_vout, e->d_ctr);
+    GC_LAUNCH_CHECK("engine force");
+    k2_bump<<<1, 1, 0, s>>>(e->d_ctr);
     mark(pr, s, "sdpd force");
-    if (P.nb2 > 0) {
-        k_rep_force<<<blocks_for(N, BS), BS, 0, s>>>(N, P, e->d_coor, e->d_cc, e->d_ncube,
-                                                     e->d_sp[c], e->d_outer, e->d_be, e->d_force);
-        GC_LAUNCH_CHECK("engine rep force");
-    }
-    if (driveon) {
-        k_ext_force<<<blocks_for(N, BS), BS, 0, s>>>(N, P, e->d_sp[c], e->d_coor, e->d_force);
-        GC_LAUNCH_CHECK("engine ext force");
-    }
-    mark(pr, s, "wall+body force");
     if (e->ncell > 0) {
         k_geo_cell<<<e->ncell, BS, 0, s>>>(P, e->step, e->d_bbox, e->d_curpos, e->d_tri, e->d_betri,
             e->d_nt[c], e->d_coor, e->d_trinor, e->d_triarea, e->d_reftriarea, e->d_tricen,
             e->d_refarea_para, e->d_refvol_para, e->d_area_para, e->d_vol_para, e->d_area, e->d_vol);
         GC_LAUNCH_CHECK("engine geo");
         mark(pr, s, "membrane geometry");
+        if (P.De != 0.0f) {
+            k2_agg<<<gb, BS, 0, s>>>(N, P, g, e->d_posid, e->d_offs, e->d_key, e->d_celltab,
+                                     e->d_force, e->d_aggforce);
+            GC_LAUNCH_CHECK("engine morse");
+        }
+        mark(pr, s, "morse");
         DefArgs a;
         a.edge = e->d_edge; a.tri = e->d_tri; a.coor = e->d_coor; a.nthro = e->d_nt[c];
         a.curpos = e->d_curpos; a.lmax = e->d_lmax; a.mexp = e->d_mexp; a.p = e->d_p;
@@ -259,25 +517,16 @@ int launch_step(gc_engine* e, int driveon, Prof* pr)
         a.kv = e->d_kv; a.Vc = e->d_vol_para; a.V0c = e->d_refvol_para;
         a.es = e->d_es; a.ee = e->d_ee; a.ts = e->d_ts; a.te = e->d_te;
         a.defforce = e->d_defforce; a.force = e->d_force;
+        a.velhalf = e->d_velhalf; a.velrho = e->d_velrho; a.vout = e->d_vout;
         k_def_force<<<blocks_for(e->nm, BS), BS, 0, s>>>(P, e->nm, e->d_bbox, a);
-        GC_LAUNCH_CHECK("engine def force");
-        mark(pr, s, "membrane force");
-        if (P.De != 0.0f) {
-            k_agg_force<<<blocks_for(N, BS), BS, 0, s>>>(N, P, e->d_coor, e->d_aggforce, e->d_cc,
-                e->d_ncube, e->d_sp[c], e->d_celltab, e->d_outer, e->d_be, e->d_force);
-            GC_LAUNCH_CHECK("engine agg force");
-        }
-        mark(pr, s, "morse");
+        GC_LAUNCH_CHECK("engine membrane force");
+        mark(pr, s, "membrane force+corrector");
     }
-    k_correct<<<blocks_for(N, BS), BS, 0, s>>>(N, P, e->d_sp[c], e->d_force, e->d_velhalf, e->d_vr[c]);
-    GC_LAUNCH_CHECK("engine correct");
-    mark(pr, s, "corrector");
-    if (P.nf > P.nb2) {
-        k_bound2_vel<<<blocks_for(N, BS), BS, 0, s>>>(N, P, e->d_coor, e->d_vr[c], e->d_cc,
-                                                      e->d_ncube, e->d_outer, e->d_sp[c], e->d_be);
-        GC_LAUNCH_CHECK("engine bound2 vel 2");
+    if (P.nf > 0) {
+        k2_wallvel<<<gb, BS, 0, s>>>(N, P, g, e->d_posid, e->d_vout, e->d_offs, e->d_key);
+        GC_LAUNCH_CHECK("engine wall velocity");
     }
-    mark(pr, s, "bound2 velocity 2");
+    mark(pr, s, "wall velocity");
     e->step += 1;
     return GC_OK;
 }
@@ -290,6 +539,27 @@ void destroy(gc_engine* e)
     for (void* p : e->allocs) (void)hipFree(p);
     if (e->s) (void)hipStreamDestroy(e->s);
     delete e;
+}
+
+// After a build without predictor (create / SetState) the arrays the gather
+// does not carry (vout, force) are still in the previous slot order: put
+// them, and velhalf, into the new order on the host.
+int reorder_carried(gc_engine* e, const std::vector<float4>& vout_id,
+                    const std::vector<tfloat3>& force_id, const std::vector<tfloat3>& vh_id)
+{
+    const int N = e->N;
+    std::vector<int> sp(N);
+    int st = hip_check(hipStreamSynchronize(e->s), "reorder sync");
+    if (st) return st;
+    if ((st = hip_check(hipMemcpy(sp.data(), e->d_sp[e->cur], sizeof(int) * N, hipMemcpyDeviceToHost), "reorder sp")))
+        return st;
+    std::vector<float4> vo(N);
+    std::vector<tfloat3> f(N), vh(N);
+    for (int s = 0; s < N; ++s) { vo[s] = vout_id[sp[s]]; f[s] = force_id[sp[s]]; vh[s] = vh_id[sp[s]]; }
+    if ((st = upload(e->d_vout, vo.data(), N, e->s)) || (st = upload(e->d_force, f.data(), N, e->s)) ||
+        (st = upload(e->d_velhalf, vh.data(), N, e->s)))
+        return st;
+    return hip_check(hipStreamSynchronize(e->s), "reorder sync2");
 }
 
 }  // namespace
@@ -322,7 +592,6 @@ extern "C" gc_engine* gc_EngineCreate(const gc_case_desc* d, int* status)
     e->consts.nCellStart = e->consts.nFluidStart + (unsigned)d->nFluidPar;
     e->consts.nCellPar = (unsigned)(d->nCell > 0 ? d->nCellPar : 0);
     e->consts.nTotalPar = e->consts.nCellStart + e->consts.nCellPar;
-    // validate the parameter block the same way the entry points do
     if (!(e->consts.CutRadius > 0) || !(e->consts.Mass > 0) || !(e->consts.TimeStep > 0) ||
         !(e->consts.DenInit > 0))
         return fail(set_error(GC_ERR_INVALID, "EngineCreate: CutRadius, Mass, TimeStep, DenInit must be > 0"));
@@ -350,48 +619,49 @@ extern "C" gc_engine* gc_EngineCreate(const gc_case_desc* d, int* status)
         (st = e->alloc(&e->d_invw, 3)) || (st = e->alloc(&e->d_ncube, 3)))
         return fail(st);
     for (int b = 0; b < 2; ++b)
-        if ((st = e->alloc(&e->d_pos64[b], 3 * N)) || (st = e->alloc(&e->d_vr[b], N)) ||
-            (st = e->alloc(&e->d_nt[b], N)) || (st = e->alloc(&e->d_sp[b], N)))
+        if ((st = e->alloc(&e->d_pos64[b], 3 * N)) || (st = e->alloc(&e->d_nt[b], N)) ||
+            (st = e->alloc(&e->d_sp[b], N)))
             return fail(st);
-    if ((st = e->alloc(&e->d_coor, N)) || (st = e->alloc(&e->d_velhalf, N)) ||
-        (st = e->alloc(&e->d_force, N)) || (st = e->alloc(&e->d_press, N)) ||
-        (st = e->alloc(&e->d_cc, N)) || (st = e->alloc(&e->d_key, N)) ||
-        (st = e->alloc(&e->d_cubepart, N)) || (st = e->alloc(&e->d_curpos, N)) ||
-        (st = e->alloc(&e->d_be, K)) || (st = e->alloc(&e->d_cnt, K + 1)) ||
-        (st = e->alloc(&e->d_offs, K + 1)) || (st = e->alloc(&e->d_rank, N)) ||
-        (st = e->alloc(&e->d_tk, N)) || (st = e->alloc(&e->d_tv, N)) ||
+    if ((st = e->alloc(&e->d_posid, N)) || (st = e->alloc(&e->d_velrho, N)) ||
+        (st = e->alloc(&e->d_vout, N)) || (st = e->alloc(&e->d_coor, N)) ||
+        (st = e->alloc(&e->d_velhalf, N)) || (st = e->alloc(&e->d_force, N)) ||
+        (st = e->alloc(&e->d_key0, N)) || (st = e->alloc(&e->d_key, N)) ||
+        (st = e->alloc(&e->d_rank, N)) || (st = e->alloc(&e->d_cnt, K + 1)) ||
+        (st = e->alloc(&e->d_offs, K + 1)) || (st = e->alloc(&e->d_tk, N)) ||
+        (st = e->alloc(&e->d_tv, N)) || (st = e->alloc(&e->d_curpos, N)) ||
         (st = e->alloc(&e->d_scanws, scan_ws_ints(e->K))) || (st = e->alloc(&e->d_ctr, 1)))
         return fail(st);
 
     hipStream_t s = e->s;
-    // initial state in id order (Particle::SetParticle, Particle.cu:485-546)
+    std::vector<float4> vout_id(N);
+    std::vector<tfloat3> vh_id(N), f_id(N, tfloat3{0.f, 0.f, 0.f});
     {
+        // Particle::SetParticle (Particle.cu:485-546): rho = DenInit, F = 0
         std::vector<double> pos(3 * N);
-        std::vector<tfloat4> vr(N);
-        std::vector<tfloat3> vh(N), f(N, tfloat3{0.f, 0.f, 0.f});
         std::vector<tint3> nt(N, tint3{0, 0, 0});
-        std::vector<int> idn(N);
+        std::vector<int> idn(N), zero(K + 1, 0);
         for (size_t i = 0; i < N; ++i) {
             pos[3 * i] = d->coor[i].x; pos[3 * i + 1] = d->coor[i].y; pos[3 * i + 2] = d->coor[i].z;
             tfloat3 v = d->vel ? d->vel[i] : tfloat3{0.f, 0.f, 0.f};
-            vr[i] = tfloat4{v.x, v.y, v.z, e->consts.DenInit};
-            vh[i] = v;
+            if ((int)i < (int)e->consts.nBound2Start) v = tfloat3{0.f, 0.f, 0.f};
+            vout_id[i] = make_float4(v.x, v.y, v.z, e->consts.DenInit);
+            vh_id[i] = v;
             idn[i] = (int)i;
         }
         unsigned long long ctr = e->ext.noise_counter;
         if ((st = upload(e->d_bbox, e->bbox, 6, s)) || (st = upload(e->d_outer, e->outer, 3, s)) ||
             (st = upload(e->d_invw, e->invw, 3, s)) || (st = upload(e->d_ncube, e->ncube, 3, s)) ||
-            (st = upload(e->d_pos64[0], pos.data(), 3 * N, s)) || (st = upload(e->d_vr[0], vr.data(), N, s)) ||
-            (st = upload(e->d_velhalf, vh.data(), N, s)) || (st = upload(e->d_force, f.data(), N, s)) ||
+            (st = upload(e->d_pos64[0], pos.data(), 3 * N, s)) ||
+            (st = upload(e->d_vout, vout_id.data(), N, s)) || (st = upload(e->d_force, f_id.data(), N, s)) ||
             (st = upload(e->d_nt[0], nt.data(), N, s)) || (st = upload(e->d_sp[0], idn.data(), N, s)) ||
-            (st = upload(e->d_curpos, idn.data(), N, s)) || (st = upload(e->d_ctr, &ctr, 1, s)))
+            (st = upload(e->d_curpos, idn.data(), N, s)) || (st = upload(e->d_cnt, zero.data(), K + 1, s)) ||
+            (st = upload(e->d_ctr, &ctr, 1, s)))
             return fail(st);
-        if ((st = hip_check(hipStreamSynchronize(s), "engine upload sync"))) return fail(st);
     }
     GcParams P = engine_params(e);
     // first neighbour build (Particle::ConCubeList at init, Particle.cu:570)
     if ((st = launch_build(e, P, 0, 0))) return fail(st);
-    k_eos<<<blocks_for(e->N, BS), BS, 0, s>>>(e->N, P, e->d_sp[e->cur], e->d_vr[e->cur], e->d_press);
+    if ((st = reorder_carried(e, vout_id, f_id, vh_id))) return fail(st);
 
     if (e->ncell > 0) {
         const size_t T = (size_t)e->ntri, E = (size_t)e->nedge, C = (size_t)e->ncell, M = (size_t)e->nm;
@@ -416,6 +686,9 @@ extern "C" gc_engine* gc_EngineCreate(const gc_case_desc* d, int* status)
             if (d->beginendtri[c].x < 0 || d->beginendtri[c].y >= e->ntri ||
                 d->beginendtri[c].x > d->beginendtri[c].y)
                 return fail(set_error(GC_ERR_TOPOLOGY, "EngineCreate: bad BeginEndTri"));
+        for (int v = 0; v < e->nm; ++v)
+            if (d->celltab[v] < 0 || d->celltab[v] >= e->ncell)
+                return fail(set_error(GC_ERR_TOPOLOGY, "EngineCreate: CellTab out of range"));
         if ((st = e->alloc(&e->d_es, es.size())) || (st = e->alloc(&e->d_ee, ee.size())) ||
             (st = e->alloc(&e->d_ts, ts.size())) || (st = e->alloc(&e->d_te, te.size())))
             return fail(st);
@@ -461,13 +734,14 @@ extern "C" int gc_EngineStepGraph(gc_engine* e, int nsteps, int driveon)
 {
     if (!e || nsteps < 0) return set_error(GC_ERR_INVALID, "EngineStepGraph: bad arguments");
     if (nsteps == 0) return GC_OK;
-    // A step alternates the double buffers, so one graph covers two steps.
-    if (!e->gexec || e->g_driveon != driveon || (e->cur != 0)) {
-        if (e->cur != 0) {
-            int st = launch_step(e, driveon, nullptr);
-            if (st) return st;
-            if (--nsteps == 0) return GC_OK;
-        }
+    // A step alternates the double buffers, so one graph covers two steps
+    // starting from buffer 0.
+    if (e->cur != 0) {
+        int st = launch_step(e, driveon, nullptr);
+        if (st) return st;
+        if (--nsteps == 0) return GC_OK;
+    }
+    if (!e->gexec || e->g_driveon != driveon) {
         if (e->gexec) { (void)hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
         hipGraph_t g;
         int st = hip_check(hipStreamBeginCapture(e->s, hipStreamCaptureModeThreadLocal), "begin capture");
@@ -500,8 +774,10 @@ extern "C" int gc_EngineInfo(const gc_engine* e, gc_engine_info* info)
     memset(info, 0, sizeof(*info));
     info->step = e->step;
     unsigned long long c = 0;
-    (void)hipStreamSynchronize(e->s);
-    (void)hipMemcpy(&c, e->d_ctr, sizeof(c), hipMemcpyDeviceToHost);
+    int st = hip_check(hipStreamSynchronize(e->s), "EngineInfo sync");
+    if (st) return st;
+    if ((st = hip_check(hipMemcpy(&c, e->d_ctr, sizeof(c), hipMemcpyDeviceToHost), "EngineInfo D2H")))
+        return st;
     info->noise_counter = c;
     info->nTotalPar = e->N; info->nCellPar = e->nm; info->nCell = e->ncell; info->sizecube = e->K;
     for (int d = 0; d < 3; ++d) {
@@ -513,36 +789,35 @@ extern "C" int gc_EngineInfo(const gc_engine* e, gc_engine_info* info)
 extern "C" int gc_EngineGetState(gc_engine* e, gc_host_state* o)
 {
     if (!e || !o) return set_error(GC_ERR_INVALID, "EngineGetState: null");
-    hipStream_t s = e->s;
     const int N = e->N, c = e->cur;
-    int st = hip_check(hipStreamSynchronize(s), "GetState sync");
+    int st = hip_check(hipStreamSynchronize(e->s), "GetState sync");
     if (st) return st;
     std::vector<int> sp(N);
     std::vector<double> pos(3 * (size_t)N);
-    std::vector<tfloat4> vr(N);
+    std::vector<float4> vo(N);
     std::vector<tfloat3> vh(N), f(N), co(N);
-    std::vector<float> pr(N);
     std::vector<tint3> nt(N);
     auto dl = [&](void* h, const void* dv, size_t b) {
         return hip_check(hipMemcpy(h, dv, b, hipMemcpyDeviceToHost), "GetState D2H");
     };
     if ((st = dl(sp.data(), e->d_sp[c], sizeof(int) * N)) ||
         (st = dl(pos.data(), e->d_pos64[c], sizeof(double) * 3 * N)) ||
-        (st = dl(vr.data(), e->d_vr[c], sizeof(tfloat4) * N)) ||
+        (st = dl(vo.data(), e->d_vout, sizeof(float4) * N)) ||
         (st = dl(vh.data(), e->d_velhalf, sizeof(tfloat3) * N)) ||
         (st = dl(f.data(), e->d_force, sizeof(tfloat3) * N)) ||
         (st = dl(co.data(), e->d_coor, sizeof(tfloat3) * N)) ||
-        (st = dl(pr.data(), e->d_press, sizeof(float) * N)) ||
         (st = dl(nt.data(), e->d_nt[c], sizeof(tint3) * N)))
         return st;
+    const float c1sq = e->consts.SoundSpeed1 * e->consts.SoundSpeed1;
+    const float c2sq = e->consts.SoundSpeed2 * e->consts.SoundSpeed2;
     for (int slot = 0; slot < N; ++slot) {
         int id = sp[slot];
         if (o->pos) for (int d = 0; d < 3; ++d) o->pos[3 * (size_t)id + d] = pos[3 * (size_t)slot + d];
         if (o->coor) o->coor[id] = co[slot];
-        if (o->velrhop) o->velrhop[id] = vr[slot];
+        if (o->velrhop) o->velrhop[id] = tfloat4{vo[slot].x, vo[slot].y, vo[slot].z, vo[slot].w};
         if (o->velhalf) o->velhalf[id] = vh[slot];
         if (o->force) o->force[id] = f[slot];
-        if (o->press) o->press[id] = pr[slot];
+        if (o->press) o->press[id] = (id < (int)e->consts.nCellStart ? c1sq : c2sq) * vo[slot].w;
         if (o->nthrobound) o->nthrobound[id] = nt[slot];
     }
     if (e->ncell > 0) {
@@ -562,54 +837,36 @@ extern "C" int gc_EngineSetState(gc_engine* e, const gc_host_state* in, long ste
     int st = hip_check(hipStreamSynchronize(e->s), "SetState sync");
     if (st) return st;
     if (e->gexec) { (void)hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
-    // current state in id order, then overwrite with the given fields
     std::vector<double> pos(3 * (size_t)N);
     std::vector<tfloat4> vr(N);
     std::vector<tfloat3> vh(N), f(N);
-    std::vector<float> pr(N);
     std::vector<tint3> nt(N);
     gc_host_state cur;
     memset(&cur, 0, sizeof(cur));
     cur.pos = pos.data(); cur.velrhop = vr.data(); cur.velhalf = vh.data(); cur.force = f.data();
-    cur.press = pr.data(); cur.nthrobound = nt.data();
+    cur.nthrobound = nt.data();
     if ((st = gc_EngineGetState(e, &cur))) return st;
     if (in->pos) memcpy(pos.data(), in->pos, sizeof(double) * 3 * N);
     if (in->velrhop) memcpy(vr.data(), in->velrhop, sizeof(tfloat4) * N);
     if (in->velhalf) memcpy(vh.data(), in->velhalf, sizeof(tfloat3) * N);
     if (in->force) memcpy(f.data(), in->force, sizeof(tfloat3) * N);
-    if (in->press) memcpy(pr.data(), in->press, sizeof(float) * N);
     if (in->nthrobound) memcpy(nt.data(), in->nthrobound, sizeof(tint3) * N);
     std::vector<int> idn(N);
-    for (int i = 0; i < N; ++i) idn[i] = i;
+    std::vector<float4> vo(N);
+    for (int i = 0; i < N; ++i) { idn[i] = i; vo[i] = make_float4(vr[i].x, vr[i].y, vr[i].z, vr[i].w); }
     e->cur = 0;
     hipStream_t s = e->s;
     unsigned long long ctr = noise_counter;
-    if ((st = upload(e->d_pos64[0], pos.data(), 3 * (size_t)N, s)) || (st = upload(e->d_vr[0], vr.data(), N, s)) ||
-        (st = upload(e->d_velhalf, vh.data(), N, s)) || (st = upload(e->d_force, f.data(), N, s)) ||
-        (st = upload(e->d_press, pr.data(), N, s)) || (st = upload(e->d_nt[0], nt.data(), N, s)) ||
+    if ((st = upload(e->d_pos64[0], pos.data(), 3 * (size_t)N, s)) || (st = upload(e->d_vout, vo.data(), N, s)) ||
+        (st = upload(e->d_force, f.data(), N, s)) || (st = upload(e->d_nt[0], nt.data(), N, s)) ||
         (st = upload(e->d_sp[0], idn.data(), N, s)) || (st = upload(e->d_curpos, idn.data(), N, s)) ||
         (st = upload(e->d_ctr, &ctr, 1, s)))
         return st;
     GcParams P = engine_params(e);
     if ((st = launch_build(e, P, 0, 0))) return st;
-    // the gather without predictor copies velrhop into velhalf and does not
-    // move force and pressure: put those three into the new slot order
-    {
-        std::vector<int> sp(N);
-        if ((st = hip_check(hipStreamSynchronize(s), "SetState sync2"))) return st;
-        if ((st = hip_check(hipMemcpy(sp.data(), e->d_sp[e->cur], sizeof(int) * N, hipMemcpyDeviceToHost), "SetState sp")))
-            return st;
-        std::vector<tfloat3> vhs(N), fs(N);
-        std::vector<float> prs(N);
-        for (int slot = 0; slot < N; ++slot) {
-            vhs[slot] = vh[sp[slot]]; fs[slot] = f[sp[slot]]; prs[slot] = pr[sp[slot]];
-        }
-        if ((st = upload(e->d_velhalf, vhs.data(), N, s)) || (st = upload(e->d_press, prs.data(), N, s)) ||
-            (st = upload(e->d_force, fs.data(), N, s)))
-            return st;
-    }
+    if ((st = reorder_carried(e, vo, f, vh))) return st;
     e->step = step;
-    return hip_check(hipStreamSynchronize(s), "SetState sync3");
+    return GC_OK;
 }
 
 extern "C" int gc_EngineGetCells(gc_engine* e, gc_host_cells* o)
@@ -617,16 +874,24 @@ extern "C" int gc_EngineGetCells(gc_engine* e, gc_host_cells* o)
     if (!e || !o) return set_error(GC_ERR_INVALID, "EngineGetCells: null");
     int st = hip_check(hipStreamSynchronize(e->s), "GetCells sync");
     if (st) return st;
-    const int N = e->N, c = e->cur;
+    const int N = e->N, c = e->cur, K = e->K;
     auto dl = [&](void* h, const void* dv, size_t b) {
         return h ? hip_check(hipMemcpy(h, dv, b, hipMemcpyDeviceToHost), "GetCells D2H") : GC_OK;
     };
+    std::vector<int> key(N), offs(K + 1);
     if ((st = dl(o->sortpart, e->d_sp[c], sizeof(int) * N)) ||
-        (st = dl(o->cubepart, e->d_cubepart, sizeof(int) * N)) ||
-        (st = dl(o->cc, e->d_cc, sizeof(tint3) * N)) ||
         (st = dl(o->curpos, e->d_curpos, sizeof(int) * N)) ||
-        (st = dl(o->beginendcube, e->d_be, sizeof(tint2) * e->K)))
+        (st = dl(key.data(), e->d_key, sizeof(int) * N)) ||
+        (st = dl(offs.data(), e->d_offs, sizeof(int) * (K + 1))))
         return st;
+    if (o->cubepart) memcpy(o->cubepart, key.data(), sizeof(int) * N);
+    if (o->cc)
+        for (int s = 0; s < N; ++s) {
+            int k = key[s];
+            o->cc[s] = tint3{k % e->ncube[0], (k / e->ncube[0]) % e->ncube[1], k / (e->ncube[0] * e->ncube[1])};
+        }
+    if (o->beginendcube)
+        for (int k = 0; k < K; ++k) o->beginendcube[k] = tint2{offs[k], offs[k + 1]};
     return GC_OK;
 }
 

@@ -43,6 +43,9 @@ struct DefArgs {
     const float* A0c; const float* kv; const float* Vc; const float* V0c;
     const int* es; const int* ee; const int* ts; const int* te;
     tfloat3* defforce; tfloat3* force;
+    // engine only (NULL in the entry point): Groot-Warren corrector after
+    // the membrane force, vout = (velhalf + dt/2 F/m, rho of velrho)
+    const tfloat3* velhalf = nullptr; const float4* velrho = nullptr; float4* vout = nullptr;
 };
 
 __global__ void k_geo_cell(GcParams P, long stepcount, const float* __restrict__ bbox,

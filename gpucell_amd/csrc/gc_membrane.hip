@@ -344,6 +344,11 @@ __global__ __launch_bounds__(BS) void k_def_force(GcParams P, int nm, const floa
     tfloat3 F = a.force[sv];
     F.x += fx; F.y += fy; F.z += fz;
     a.force[sv] = F;
+    if (a.vout) {  // engine: this is the vertex's last force term -> corrector
+        float h = 0.5f * P.dt * P.inv_m;
+        tfloat3 vh = a.velhalf[sv];
+        a.vout[sv] = make_float4(vh.x + h * F.x, vh.y + h * F.y, vh.z + h * F.z, a.velrho[sv].w);
+    }
 }
 
 // ---- CalCellAggForce ------------------------------------------------------

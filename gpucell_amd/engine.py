@@ -1,0 +1,95 @@
+"""Python handle on the C++ engine (gc_Engine* in libgpucell_hip.so).
+
+Mirrors the reference's host driver (class Particle, src/Particle.cuh:8-167):
+create = InitPar/SetPar, step = DrivePar + UpdatePar, state = the arrays
+ParticlesDataReturn copies back (un-sorted to id order).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+
+
+class Engine:
+    def __init__(self, case):
+        self.lib = abi.load()
+        self.case = case
+        desc, self._keep = case.desc()
+        st = C.c_int(0)
+        self.h = self.lib.gc_EngineCreate(C.byref(desc), C.byref(st))
+        if not self.h:
+            raise abi.GcError(f"gc_EngineCreate failed ({st.value}): "
+                              f"{self.lib.gc_GetLastError().decode()}")
+        self.N = case.N
+        self.nm = case.nCellPar
+        self.ncell = case.nCell
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gc_EngineDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int = 1, driveon: int = 1, graph: bool = False):
+        f = self.lib.gc_EngineStepGraph if graph else self.lib.gc_EngineStep
+        abi.check(f(self.h, int(n), int(driveon)), "EngineStep")
+
+    def info(self) -> abi.gc_engine_info:
+        i = abi.gc_engine_info()
+        abi.check(self.lib.gc_EngineInfo(self.h, C.byref(i)), "EngineInfo")
+        return i
+
+    def state(self) -> dict:
+        N, M, Cn = self.N, self.nm, self.ncell
+        out = {
+            "pos": np.zeros((N, 3), np.float64), "coor": np.zeros((N, 3), np.float32),
+            "velrhop": np.zeros((N, 4), np.float32), "velhalf": np.zeros((N, 3), np.float32),
+            "force": np.zeros((N, 3), np.float32), "press": np.zeros(N, np.float32),
+            "nthrobound": np.zeros((N, 3), np.int32),
+        }
+        if Cn:
+            out.update(area=np.zeros(Cn, np.float32), vol=np.zeros(Cn, np.float32),
+                       defforce=np.zeros((M, 3), np.float32), aggforce=np.zeros((M, 3), np.float32))
+        hs = abi.gc_host_state()
+        for k, v in out.items():
+            setattr(hs, k, v.ctypes.data)
+        abi.check(self.lib.gc_EngineGetState(self.h, C.byref(hs)), "EngineGetState")
+        return out
+
+    def set_state(self, st: dict, step: int, counter: int):
+        hs = abi.gc_host_state()
+        keep = []
+        for k, dt in (("pos", np.float64), ("velrhop", np.float32), ("velhalf", np.float32),
+                      ("force", np.float32), ("press", np.float32), ("nthrobound", np.int32)):
+            if k in st and st[k] is not None:
+                a = np.ascontiguousarray(st[k], dtype=dt)
+                keep.append(a)
+                setattr(hs, k, a.ctypes.data)
+        abi.check(self.lib.gc_EngineSetState(self.h, C.byref(hs), int(step), int(counter)),
+                  "EngineSetState")
+
+    def cells(self) -> dict:
+        i = self.info()
+        N, K = self.N, i.sizecube
+        out = {"sortpart": np.zeros(N, np.int32), "cubepart": np.zeros(N, np.int32),
+               "cc": np.zeros((N, 3), np.int32), "curpos": np.zeros(N, np.int32),
+               "beginendcube": np.zeros((K, 2), np.int32)}
+        hc = abi.gc_host_cells()
+        for k, v in out.items():
+            setattr(hc, k, v.ctypes.data)
+        abi.check(self.lib.gc_EngineGetCells(self.h, C.byref(hc)), "EngineGetCells")
+        return out
+
+    def profile(self, nsteps: int = 10, driveon: int = 1) -> dict:
+        p = abi.gc_engine_profile()
+        abi.check(self.lib.gc_EngineProfile(self.h, int(nsteps), int(driveon), C.byref(p)),
+                  "EngineProfile")
+        return {p.names[k].value.decode(): float(p.us[k]) for k in range(p.nkernels)}

@@ -1,0 +1,130 @@
+"""Parity of the engine's step (gc_EngineStep / gc_EngineStepGraph) with the
+FP64 oracle:
+
+* the cell list the engine builds is bit-exact against the oracle's
+  (cube, id) counting sort of the same FP32 positions;
+* one step from an exported state: FP64 positions to 1e-9, densities to
+  2e-6 relative, forces to the per-particle bound 1e-5, velocities
+  through the corrector;
+* same case twice -> bitwise-equal state; graph replay == eager launches."""
+import numpy as np
+import pytest
+
+from gpucell_amd import casegen
+from gpucell_amd.engine import Engine
+from oracle import pyoracle as orc
+from tests import gpu_util as gu
+from tests import util
+
+pytestmark = pytest.mark.gpu
+
+
+def channel_case(noise=True, De=0.0, seed=5):
+    ph = util.replace_physics(kT=0.01, De=De, r0=0.3, beta=1.5, mcut=1.5, fext=0.05)
+    return casegen.channel(24, 12, 2, rbc_diameter=5.0, subdiv=2, seed=seed, physics=ph,
+                           noise=noise)
+
+
+def box_case():
+    return casegen.periodic_box(n=16, rbc_diameter=7.0, subdiv=2, seed=11)
+
+
+CASES = {"channel": lambda: channel_case(), "channel_morse": lambda: channel_case(De=0.02),
+         "box": box_case}
+
+
+@pytest.mark.parametrize("name", ["channel", "box"])
+def test_engine_cells_bit_exact(gpu, name):
+    case = CASES[name]()
+    e = Engine(case)
+    e.step(3)
+    st, cl, info = e.state(), e.cells(), e.info()
+    nc = np.array(info.ncube)
+    inv = np.array(info.invwid, np.float32)
+    cc_o, key_o, _ = orc.preconlist(st["coor"], np.float32(case.consts.CutRadius),
+                                    list(case.ext.periodic), case.bbox, inv, nc,
+                                    np.arange(case.N, dtype=np.int32))
+    ks, sps = orc.sort(key_o, np.arange(case.N, dtype=np.int32))
+    assert np.array_equal(cl["sortpart"], sps)
+    assert np.array_equal(cl["cubepart"], ks)
+    assert np.array_equal(cl["cc"], cc_o[sps])
+    assert np.array_equal(cl["beginendcube"], orc.begin_end(ks, info.sizecube))
+    assert np.array_equal(cl["curpos"][sps], np.arange(case.N))
+
+
+def _oracle_state(o, case, S, info):
+    st = {
+        "pos": S["pos"].copy(), "img": S["nthrobound"].copy(),
+        "vel": S["velrhop"][:, :3].astype(np.float64), "rho": S["velrhop"][:, 3].astype(np.float64),
+        "velhalf": S["velhalf"].astype(np.float64), "force": S["force"].astype(np.float64),
+        "press": S["press"].astype(np.float64), "abs_sum": np.zeros(case.N),
+        "step": int(info.step), "counter": int(info.noise_counter)}
+    if case.nCell:
+        st.update(defforce=np.zeros((case.nCellPar, 3)), aggforce=np.zeros((case.nCellPar, 3)),
+                  cell_area=np.zeros(case.nCell), cell_vol=np.zeros(case.nCell))
+    return st
+
+
+@pytest.mark.parametrize("name", ["channel", "channel_morse", "box"])
+def test_engine_step_parity(gpu, name):
+    case = CASES[name]()
+    e = Engine(case)
+    o = orc.Oracle(case)
+    o.initial_state()               # reference geometry and mechanics
+    e.step(4)
+    S, info = e.state(), e.info()
+    e.step(1)
+    S1 = e.state()
+    # positions: FP64 accumulator on both sides
+    st = _oracle_state(o, case, S, info)
+    o.step(st, driveon=1)
+    assert np.abs(S1["pos"] - st["pos"]).max() < 1e-9
+    assert np.array_equal(S1["nthrobound"], st["img"])
+    # forces at the FP32 positions the kernels read
+    mov = np.arange(case.N) >= case.consts.nFluidStart
+    dt, lam, m = float(case.consts.TimeStep), float(case.consts.VVsigma), float(case.consts.Mass)
+    vt = S["velrhop"][:, :3].astype(np.float64).copy()
+    vt[mov] += lam * dt * S["force"][mov].astype(np.float64) / m
+    pos32 = S1["coor"].astype(np.float64)
+    o.bound2_vel(pos32, vt)
+    rho = np.full(case.N, float(case.consts.DenInit))
+    F_o, ab, rho, extra = gu.oracle_forces(o, case, pos32, vt, rho, S1["nthrobound"],
+                                           info.noise_counter, 1)
+    assert np.allclose(S1["velrhop"][:, 3], rho, rtol=2e-6)
+    emax, l2 = util.b11_error(S1["force"].astype(np.float64), F_o, ab, case.consts.ConstExtForce)
+    print(f"{name}: step force max per-particle error {emax:.3e}, relative L2 {l2:.3e}")
+    assert emax <= 1e-5
+    if case.nCell:
+        assert np.allclose(S1["area"], extra["geo"]["cell_area"], rtol=2e-6)
+        assert np.allclose(S1["vol"], extra["geo"]["cell_vol"], rtol=2e-6)
+    # corrector: v = velhalf + dt/2 F/m with this step's force
+    vh = S1["velhalf"].astype(np.float64)
+    v_exp = vh + 0.5 * dt * S1["force"].astype(np.float64) / m
+    assert np.allclose(S1["velrhop"][mov, :3], v_exp[mov], rtol=1e-6, atol=1e-7)
+
+
+def test_engine_deterministic_and_graph_equals_eager(gpu):
+    case = channel_case()
+    a, b = Engine(case), Engine(case)
+    a.step(7)
+    b.step(7, graph=True)
+    sa, sb = a.state(), b.state()
+    for k in ("pos", "velrhop", "velhalf", "force", "press", "nthrobound", "area", "vol"):
+        assert np.array_equal(sa[k], sb[k]), k
+    assert a.info().noise_counter == b.info().noise_counter == 7
+    c = Engine(case)
+    c.step(7)
+    sc = c.state()
+    assert np.array_equal(sa["pos"], sc["pos"]) and np.array_equal(sa["force"], sc["force"])
+
+
+def test_engine_momentum_conservation_without_forcing(gpu):
+    # fully periodic box, no walls, no body force: total momentum is kept
+    case = casegen.periodic_box(n=12, n_rbc=0, seed=2, reverse_poiseuille=False)
+    e = Engine(case)
+    p0 = e.state()["velrhop"][:, :3].astype(np.float64).sum(0)
+    e.step(20, driveon=0)
+    S = e.state()
+    p1 = S["velrhop"][:, :3].astype(np.float64).sum(0)
+    scale = np.abs(S["force"]).sum() * float(case.consts.TimeStep)
+    assert np.abs(p1 - p0).max() < 1e-4 * max(scale, 1e-12) + 1e-5

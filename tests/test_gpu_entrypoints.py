@@ -1,0 +1,311 @@
+"""Parity of the reference entry points (C-ABI, one call per cupar/cudiv
+function) against the FP64 oracle on the same inputs.
+
+* cell list (PreConList, Sort, CalBeginEndCube, SortArraysWithCells):
+  bit-exact;
+* fluid kernels: densities to 2e-6 relative; forces to the per-particle
+  error |F - F*| / (sum_j |f_ij*| + |F_ext|) <= 1e-5 (FP32 vs FP64);
+* membrane kernels: geometry and moduli to 1e-5 relative, forces to the
+  same per-particle bound with the membrane terms included."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from gpucell_amd import abi, casegen
+from oracle import pyoracle as orc
+from tests import gpu_util as gu
+from tests import util
+
+pytestmark = pytest.mark.gpu
+
+
+def small_channel(noise=True, De=0.0, with_bound1=True, seed=3):
+    ph = util.replace_physics(kT=0.01, De=De, r0=0.3, beta=1.5, mcut=1.5, mu0=1.0, kc=0.1,
+                              ka=20.0, kd=5.0, kv=50.0, fext=0.02)
+    c = casegen.channel(24, 12, 2, rbc_diameter=5.0, subdiv=2, seed=seed, physics=ph,
+                        noise=noise)
+    if with_bound1:
+        # the outermost wall layer becomes Bound1 (repulsion only)
+        walls = c.coor[:c.nBound2]
+        outer = np.maximum(np.abs(walls[:, 1]), np.abs(walls[:, 2])) > 6.0 + 2.0
+        b1, b2 = walls[outer], walls[~outer]
+        rest = c.coor[c.nBound2:]
+        vel = np.concatenate([np.zeros((b1.shape[0] + b2.shape[0], 3), np.float32),
+                              c.vel[c.nBound2:]])
+        c.coor = np.concatenate([b1, b2, rest]).astype(np.float32)
+        c.vel = vel
+        c.nBound1, c.nBound2 = int(b1.shape[0]), int(b2.shape[0])
+        c.consts = casegen.make_consts(ph, c.nBound1, c.nBound2, c.nFluid, c.nCellPar)
+    # membranes start wrapped into the box for the per-entry tests
+    return c
+
+
+def wrap_into_box(case):
+    """Wrap periodic axes into [lo, hi) and return the image counters."""
+    pos = case.coor.astype(np.float64)
+    img = np.zeros(pos.shape, np.int32)
+    for d in range(3):
+        if not case.ext.periodic[d]:
+            continue
+        lo, hi = float(case.bbox[2 * d]), float(case.bbox[2 * d + 1])
+        L = np.float32(hi - lo)
+        k = np.floor((pos[:, d] - lo) / L).astype(np.int32)
+        pos[:, d] -= k * L
+        img[:, d] = k
+    case.coor = pos.astype(np.float32)
+    return img
+
+
+def setup(case, img=None):
+    gu.upload_consts(case)
+    per = list(case.ext.periodic)
+    nc, outer, inv, K = orc.config_domain(case.bbox, case.consts.CutRadius, per)
+    g = gu.GpuCase(case, nc, outer, inv, K)
+    if img is not None:
+        g.nthro = gu.dev(img, np.int32)
+    return g, (nc, outer, inv, K)
+
+
+def test_con_cube_list_bit_exact(gpu):
+    case = small_channel()
+    img = wrap_into_box(case)
+    g, (nc, outer, inv, K) = setup(case, img)
+    cut = np.float32(case.consts.CutRadius)
+    per = list(case.ext.periodic)
+    for rnd in range(2):
+        coor_slot = gu.host(g.coor)
+        sp_before = gu.host(g.sortpart)
+        cc_o, key_o, old_o = orc.preconlist(coor_slot, cut, per, case.bbox, inv, nc, sp_before)
+        g.con_cube_list()
+        key_s, sp_s = orc.sort(key_o, sp_before)
+        be_o = orc.begin_end(key_s, K)
+        assert np.array_equal(gu.host(g.cubepart), key_s)
+        assert np.array_equal(gu.host(g.sortpart), sp_s)
+        assert np.array_equal(gu.host(g.be), be_o)
+        assert np.array_equal(gu.host(g.oldpos), old_o)
+        # gather: new[s] = old[oldpos[sortpart[s]]]; curpos inverse of sortpart
+        src = old_o[sp_s]
+        assert np.array_equal(gu.host(g.coor), coor_slot[src])
+        assert np.array_equal(gu.host(g.cc), cc_o[src])
+        cur = gu.host(g.curpos)
+        assert np.array_equal(cur[sp_s], np.arange(case.N))
+        # move particles a little and rebuild from a non-identity order
+        rng = np.random.default_rng(rnd)
+        moved = gu.host(g.coor) + rng.uniform(-0.4, 0.4, size=(case.N, 3)).astype(np.float32)
+        moved[:, 0] = np.mod(moved[:, 0], np.float32(case.bbox[1]))
+        g.coor.copy_(gu.dev(moved, np.float32))
+
+
+def test_sort_rejects_overfull_cube(gpu):
+    import torch
+    n = 9000
+    keys = torch.zeros(n, dtype=torch.int32, device="cuda")
+    vals = torch.arange(n, dtype=torch.int32, device="cuda")
+    lib = abi.load()
+    st = lib.gc_Sort(n, abi.ptr(keys), abi.ptr(vals))
+    assert st == -3 and b"over-full" in lib.gc_GetLastError()
+    # a legal sort of the same size with ties sorted by value
+    keys = gu.dev(np.arange(n)[::-1] % 7, np.int32)
+    vals = gu.dev(np.random.default_rng(0).permutation(n), np.int32)
+    k0, v0 = gu.host(keys), gu.host(vals)
+    gu.call("gc_Sort", n, keys, vals)
+    k1, v1 = orc.sort(k0, v0)
+    assert np.array_equal(gu.host(keys), k1) and np.array_equal(gu.host(vals), v1)
+
+
+def _orc_inputs(g, case):
+    sp = gu.host(g.sortpart)
+    pos = gu.id_order(gu.host(g.coor), sp).astype(np.float64)
+    vr = gu.id_order(gu.host(g.velrhop), sp).astype(np.float64)
+    img = gu.id_order(gu.host(g.nthro), sp)
+    return sp, pos, vr, img
+
+
+@pytest.mark.parametrize("noise", [False, True])
+def test_fluid_entry_points(gpu, noise):
+    case = small_channel(noise=noise)
+    img = wrap_into_box(case)
+    g, grid = setup(case, img)
+    o = orc.Oracle(case)
+    g.con_cube_list()
+    args = (g.coor, g.velrhop, g.cc, g.ncube, g.outer, g.sortpart, g.be)
+    # density
+    gu.call("gc_CalFluidDen", case.N, *args)
+    sp, pos, vr, _ = _orc_inputs(g, case)
+    rho = np.full(case.N, float(case.consts.DenInit))
+    o.density(pos, rho)
+    assert np.allclose(vr[:, 3], rho, rtol=2e-6, atol=0)
+    # predictor with a nonzero old force
+    rng = np.random.default_rng(1)
+    f0 = rng.normal(0, 1.0, size=(case.N, 3)).astype(np.float32)
+    g.force.copy_(gu.dev(f0[sp], np.float32))   # slot order
+    gu.call("gc_PredictFluidVel", case.N, g.sortpart, g.velrhop, g.velhalf, g.force)
+    v_o = case.vel.astype(np.float64).copy()
+    vh_o = v_o.copy()
+    o.predict(v_o, vh_o, f0.astype(np.float64))
+    _, _, vr2, _ = _orc_inputs(g, case)
+    mov = np.arange(case.N) >= case.consts.nFluidStart
+    assert np.allclose(vr2[mov, :3], v_o[mov], rtol=1e-6, atol=1e-7)
+    # wall velocity from the predicted fluid velocities
+    gu.call("gc_CalBound2Vel", case.N, *args)
+    _, pos, vr3, _ = _orc_inputs(g, case)
+    vel_o = vr2[:, :3].copy()
+    o.bound2_vel(pos, vel_o)
+    assert np.allclose(vr3[:, :3], vel_o, rtol=1e-5, atol=1e-7)
+    # EOS + SDPD + repulsion + body force (reverse-Poiseuille pattern needs positions)
+    gu.call("gc_EOS", case.N, g.sortpart, g.velrhop, g.press)
+    press_o = np.zeros(case.N)
+    o.eos(vr3[:, 3], press_o)
+    assert np.allclose(gu.id_order(gu.host(g.press), sp), press_o, rtol=1e-6)
+    ext = abi.gc_ext_consts()
+    abi.load().gc_GetExtConsts(C.byref(ext))
+    counter = int(ext.noise_counter)
+    gu.call("gc_CalFluidSDPDForce", case.N, None, g.coor, g.velrhop, g.cc, g.ncube, g.sortpart,
+            g.outer, g.be, g.force, g.press)
+    gu.call("gc_CalFluidRepForce", case.N, g.coor, g.cc, g.ncube, g.sortpart, g.outer, g.be,
+            g.force)
+    gu.call("gc_SetFluidExtForce", 1, case.N, g.sortpart, g.force)
+    F_o = np.zeros((case.N, 3))
+    ab = np.zeros(case.N)
+    o.sdpd_force(pos, vr3[:, :3].copy(), vr3[:, 3].copy(), press_o, counter, F_o, ab)
+    o.rep_force(pos, F_o, ab)
+    o.ext_force(pos, 1, F_o, ab)
+    F = gu.id_order(gu.host(g.force), sp).astype(np.float64)
+    emax, l2 = util.b11_error(F, F_o, ab, case.consts.ConstExtForce)
+    print(f"fluid entry points noise={noise}: max per-particle error {emax:.3e}, L2 {l2:.3e}")
+    assert emax <= 1e-5
+    # the noise counter advanced by one evaluation
+    abi.load().gc_GetExtConsts(C.byref(ext))
+    assert ext.noise_counter == counter + 1
+    # corrector
+    gu.call("gc_CorrectFliudVel", case.N, g.sortpart, g.force, g.velhalf, g.velrhop, g.coor,
+            g.cc, g.ncube, g.outer, g.be)
+    vh = gu.id_order(gu.host(g.velhalf), sp).astype(np.float64)
+    v_corr = vh.copy()
+    o.correct(v_corr, vh, F)
+    _, _, vr4, _ = _orc_inputs(g, case)
+    assert np.allclose(vr4[mov, :3], v_corr[mov], rtol=1e-6, atol=1e-7)
+
+
+def test_update_and_periodic_wrap(gpu):
+    case = small_channel(noise=False)
+    img = wrap_into_box(case)
+    g, grid = setup(case, img)
+    g.con_cube_list()
+    sp = gu.host(g.sortpart)
+    rng = np.random.default_rng(4)
+    v = rng.normal(0, 30.0, size=(case.N, 3)).astype(np.float32)   # large: many wraps
+    f = rng.normal(0, 5.0, size=(case.N, 3)).astype(np.float32)
+    vr = gu.host(g.velrhop)
+    vr[:, :3] = v[sp]
+    g.velrhop.copy_(gu.dev(vr, np.float32))
+    g.force.copy_(gu.dev(f[sp], np.float32))
+    pos_before = gu.id_order(gu.host(g.coor), sp).astype(np.float64)
+    gu.call("gc_UpdateFluidCoor", case.N, g.sortpart, g.force, g.coor, g.velrhop)
+    gu.call("gc_AppFluidPeriodicBC", case.N, g.sortpart, g.coor, g.nthro, g.bbox)
+    pos_o = pos_before.copy()
+    o = orc.Oracle(case)
+    o.update_coor(pos_o, v.astype(np.float64), f.astype(np.float64))
+    img_o = img.copy()
+    o.periodic_bc(pos_o, img_o)
+    pos_g = gu.id_order(gu.host(g.coor), sp).astype(np.float64)
+    img_g = gu.id_order(gu.host(g.nthro), sp)
+    L = float(np.float32(case.bbox[1] - case.bbox[0]))
+    unwr_g = pos_g[:, 0] + img_g[:, 0] * L
+    unwr_o = pos_o[:, 0] + img_o[:, 0] * L
+    assert np.allclose(unwr_g, unwr_o, atol=2e-5)
+    assert np.all((pos_g[:, 0] >= case.bbox[0]) & (pos_g[:, 0] <= case.bbox[1]))
+    # image counts agree except for particles within rounding of a face
+    near = np.minimum(np.abs(pos_o[:, 0] - case.bbox[0]), np.abs(pos_o[:, 0] - case.bbox[1])) < 1e-5
+    assert np.array_equal(img_g[~near], img_o[~near])
+    assert np.count_nonzero(img_o[:, 0] != img[:, 0]) > 10
+    # walls do not move
+    w = np.arange(case.N) < case.consts.nFluidStart
+    assert np.array_equal(pos_g[w], pos_before[w])
+
+
+def test_membrane_entry_points(gpu):
+    import torch
+    case = small_channel(noise=False, De=0.05)
+    img = wrap_into_box(case)
+    g, grid = setup(case, img)
+    g.con_cube_list()
+    o = orc.Oracle(case)
+    T, E, Cn, M = case.tri.shape[0], case.edge.shape[0], case.nCell, case.nCellPar
+    z = lambda *s: torch.zeros(s, dtype=torch.float32, device="cuda")
+    d = {k: z(T) for k in ("triarea", "reftriarea", "refarea_para", "refvol_para",
+                           "area_para", "vol_para", "hl", "hg", "hv")}
+    d["trinor"], d["tricen"] = z(T, 3), z(T, 3)
+    d["area"], d["vol"], d["pararea"] = z(Cn), z(Cn), z(M)
+    d["hl"].copy_(gu.dev(case.hlarea)); d["hg"].copy_(gu.dev(case.hgarea)); d["hv"].copy_(gu.dev(case.hvol))
+    tri, edge = gu.dev(case.tri, np.int32), gu.dev(case.edge, np.int32)
+    bet, celltab = gu.dev(case.beginendtri, np.int32), gu.dev(case.celltab, np.int32)
+
+    def geo(step):
+        gu.call("gc_CalGeoPara", step, T, Cn, g.bbox, g.curpos, tri, bet, g.nthro, g.coor,
+                d["trinor"], d["triarea"], d["reftriarea"], d["tricen"], d["refarea_para"],
+                d["refvol_para"], d["area_para"], d["vol_para"], d["area"], d["vol"], d["pararea"])
+
+    geo(-1)
+    sp, pos, vr, imgs = _orc_inputs(g, case)
+    go = o.geo(-1, pos, imgs)
+    assert np.allclose(gu.host(d["area"]), go["cell_area"], rtol=2e-6)
+    assert np.allclose(gu.host(d["vol"]), go["cell_vol"], rtol=2e-6)
+    assert np.allclose(gu.host(d["triarea"]), go["tri_area"], rtol=1e-5)
+    assert np.allclose(gu.host(d["trinor"]), go["tri_nor"], atol=2e-6)
+    assert np.allclose(gu.host(d["tricen"]), go["tri_cen"], rtol=1e-6, atol=1e-5)
+    assert np.allclose(gu.host(d["pararea"]), go["par_area"], rtol=1e-5)
+    assert np.array_equal(gu.host(d["reftriarea"]), gu.host(d["triarea"]))
+    # mechanical parameters
+    m = {k: z(E) for k in ("l0", "lmax", "kb", "mexp", "p", "kp", "th0")}
+    mu0, kc = gu.dev(case.shearmod), gu.dev(case.bendmod)
+    gu.call("gc_SetMechPara", Cn, E, edge, g.curpos, m["l0"], m["lmax"], kc, mu0, m["kb"],
+            m["mexp"], m["p"], m["kp"], m["th0"], d["trinor"], d["tricen"], g.coor)
+    mo = o.mech_para(pos, imgs)
+    for k, ko in (("l0", "l0"), ("lmax", "lmax"), ("kb", "kb"), ("mexp", "mexp"), ("p", "p"),
+                  ("kp", "kp")):
+        assert np.allclose(gu.host(m[k]), mo[ko], rtol=1e-5), k
+    assert np.allclose(gu.host(m["th0"]), mo["theta0"], atol=2e-6)
+    # deform the membrane, then geometry + forces
+    rng = np.random.default_rng(9)
+    c2 = gu.host(g.coor)
+    memb = sp >= case.consts.nCellStart
+    c2[memb] += rng.normal(0, 0.05, size=(memb.sum(), 3)).astype(np.float32)
+    g.coor.copy_(gu.dev(c2, np.float32))
+    geo(0)
+    sp, pos, vr, imgs = _orc_inputs(g, case)
+    go = o.geo(0, pos, imgs)
+    assert np.allclose(gu.host(d["area"]), go["cell_area"], rtol=2e-6)
+    g.force.zero_()
+    defF, hdef = z(M, 3), np.zeros((M, 3), np.float32)
+    gu.call("gc_CalCellDefForce", Cn, E, T, case.N, M, g.bbox, edge, tri, g.coor,
+            hdef, defF, g.nthro, m["lmax"], m["mexp"], m["p"], m["kp"], d["trinor"],
+            d["tricen"], m["th0"], m["kb"], d["triarea"], d["reftriarea"], d["hl"], d["hg"],
+            d["area_para"], d["refarea_para"], d["hv"], d["vol_para"], d["refvol_para"], g.force,
+            g.sortpart, g.curpos)
+    ab = np.zeros(case.N)
+    Fo = np.zeros((case.N, 3))
+    df_o = o.def_force(pos, imgs, Fo, ab)
+    e_def = np.linalg.norm(gu.host(defF) - df_o, axis=1) / np.maximum(ab[case.consts.nCellStart:], 1e-30)
+    print(f"membrane force max per-vertex error {e_def.max():.3e}")
+    assert e_def.max() <= 1e-5
+    assert np.array_equal(hdef, gu.host(defF))
+    # Morse between the two cells
+    g.force.zero_()
+    aggF = z(M, 3)
+    gu.call("gc_CalCellAggForce", Cn, case.N, M, g.coor, aggF, g.cc, g.ncube, g.sortpart, celltab,
+            g.outer, g.be, g.force, d["pararea"])
+    ab2 = np.zeros(case.N)
+    af_o = o.agg_force(pos, None, ab2)
+    assert np.allclose(gu.host(aggF), af_o, rtol=1e-5, atol=1e-6 * max(1e-30, np.abs(af_o).max()))
+    F = gu.id_order(gu.host(g.force), sp)
+    assert np.allclose(F[case.consts.nCellStart:], af_o, rtol=1e-5,
+                       atol=1e-6 * max(1e-30, np.abs(af_o).max()))
+
+
+def test_null_and_unconfigured_arguments(gpu):
+    lib = abi.load()
+    assert lib.gc_CalFluidDen(10, None, None, None, None, None, None, None) == -1
+    assert lib.gc_Sort(-1, None, None) == -1
+    assert lib.gc_CalGeoPara(0, 0, 0, *([None] * 17)) == 0  # nCell == 0: nothing to do
