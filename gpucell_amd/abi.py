@@ -139,6 +139,12 @@ PROTOTYPES = {
     "gc_EngineSetState": (I, [P, C.POINTER(gc_host_state), L, C.c_uint64]),
     "gc_EngineGetCells": (I, [P, C.POINTER(gc_host_cells)]),
     "gc_EngineProfile": (I, [P, I, I, C.POINTER(gc_engine_profile)]),
+    "gc_EngineGetTriangles": (I, [P, P, P]),
+    "gc_EngineRanges": (I, [P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "gc_ReadCase": (I, [C.c_char_p, C.POINTER(gc_case_desc)]),
+    "gc_FreeCase": (None, [C.POINTER(gc_case_desc)]),
+    "gc_WriteCase": (I, [C.c_char_p, C.POINTER(gc_case_desc)]),
+    "gc_EngineWriteTecplot": (I, [P, C.c_char_p, L, L, C.c_float, L, P, P, P]),
 }
 
 # Entry points that replace the reference's cupar/cudiv functions.

@@ -22,7 +22,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgpucell_hip.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 
-SOURCES = ["gc_abi.hip", "gc_sdpd.hip", "gc_membrane.hip", "gc_engine.hip"]
+SOURCES = ["gc_abi.hip", "gc_sdpd.hip", "gc_membrane.hip", "gc_engine.hip", "gc_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -47,7 +47,7 @@ def build_lib(verbose: bool = False) -> str:
     objs = []
 
     def comp(src):
-        obj = os.path.join(BUILD, src.replace(".hip", ".o"))
+        obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
         return obj
 

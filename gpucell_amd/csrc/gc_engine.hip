@@ -928,3 +928,26 @@ extern "C" int gc_EngineProfile(gc_engine* e, int nsteps, int driveon, gc_engine
     out->nkernels = m;
     return GC_OK;
 }
+
+extern "C" int gc_EngineGetTriangles(gc_engine* e, tfloat3* trinordir, tfloat3* tricenter)
+{
+    if (!e) return set_error(GC_ERR_INVALID, "EngineGetTriangles: null");
+    if (e->ncell == 0) return GC_OK;
+    int st = hip_check(hipStreamSynchronize(e->s), "GetTriangles sync");
+    if (st) return st;
+    if (trinordir && (st = hip_check(hipMemcpy(trinordir, e->d_trinor, sizeof(tfloat3) * e->ntri,
+                                               hipMemcpyDeviceToHost), "GetTriangles nor")))
+        return st;
+    if (tricenter && (st = hip_check(hipMemcpy(tricenter, e->d_tricen, sizeof(tfloat3) * e->ntri,
+                                               hipMemcpyDeviceToHost), "GetTriangles cen")))
+        return st;
+    return GC_OK;
+}
+
+extern "C" int gc_EngineRanges(const gc_engine* e, int* nbound1, int* nbound12)
+{
+    if (!e || !nbound1 || !nbound12) return set_error(GC_ERR_INVALID, "EngineRanges: null");
+    *nbound1 = (int)e->consts.nBound2Start;
+    *nbound12 = (int)e->consts.nFluidStart;
+    return GC_OK;
+}

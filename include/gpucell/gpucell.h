@@ -295,6 +295,30 @@ typedef struct {
 } gc_host_cells;
 int gc_EngineGetCells(gc_engine* e, gc_host_cells* out);
 
+/* Triangle normals and centres of the last step (nAllTri each). */
+int gc_EngineGetTriangles(gc_engine* e, tfloat3* trinordir, tfloat3* tricenter);
+/* nBound2Start and nFluidStart of the engine's parameter block. */
+int gc_EngineRanges(const gc_engine* e, int* nbound1, int* nbound12);
+
+/* ---- case decks and Tecplot output (host only) ------------------------ */
+
+/* Reads the reference's four input decks from `dir` (SDPDParCoor.txt,
+ * SDPDPhyPara.txt, CELLParStruct.txt, CELLPhyPara.txt; formats of
+ * Particle::LoadFBPar/LoadSDPDPara/LoadCellStruct/LoadCellPara,
+ * src/Particle.cu:45-427), with the compile-time constants of
+ * src/main.cuh:6-16.  Arrays are allocated; release with gc_FreeCase. */
+int gc_ReadCase(const char* dir, gc_case_desc* out);
+void gc_FreeCase(gc_case_desc* d);
+/* Writes a case as the four decks (fields narrower than 20 columns). */
+int gc_WriteCase(const char* dir, const gc_case_desc* d);
+/* One frame of Particle::OutputPar (src/Particle.cu:681-1300) under
+ * dir/Result/: FluidCoor, FluidField, CellCoor, CellForce (+TF_Cell<i>),
+ * CellGeo (Geo_Cell<i>), CellNorDir; stepcount == 0 starts the per-cell
+ * time series, later frames append. */
+int gc_EngineWriteTecplot(gc_engine* e, const char* dir, long stepdata, long stepcount,
+                          float timestep, long stepoutput, const tint3* tri,
+                          const tint2* beginendtri, const int* celltab);
+
 /* Per-kernel averages of the last gc_EngineProfile call, in microseconds. */
 typedef struct {
     int nkernels;

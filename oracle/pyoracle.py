@@ -65,7 +65,11 @@ def load():
 
 
 def _p(a):
-    return None if a is None else a.ctypes.data
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("oracle arrays must be C-contiguous (got a strided view)")
+    return a.ctypes.data
 
 
 class OrcState(C.Structure):

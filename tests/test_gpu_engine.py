@@ -128,3 +128,51 @@ def test_engine_momentum_conservation_without_forcing(gpu):
     p1 = S["velrhop"][:, :3].astype(np.float64).sum(0)
     scale = np.abs(S["force"]).sum() * float(case.consts.TimeStep)
     assert np.abs(p1 - p0).max() < 1e-4 * max(scale, 1e-12) + 1e-5
+
+
+def test_engine_tecplot_frames(gpu, tmp_path):
+    """Particle::OutputPar's six outputs (Particle.cu:681-1300) from the
+    engine: zone structure, slot order, values equal to the exported state."""
+    import ctypes as C
+
+    from gpucell_amd import abi
+    case = channel_case()
+    e = Engine(case)
+    tri = np.ascontiguousarray(case.tri, np.int32)
+    bet = np.ascontiguousarray(case.beginendtri, np.int32)
+    ctab = np.ascontiguousarray(case.celltab, np.int32)
+    lib = abi.load()
+    for frame in range(2):
+        e.step(3)
+        abi.check(lib.gc_EngineWriteTecplot(e.h, str(tmp_path).encode(), frame, 3 * frame,
+                                            float(case.consts.TimeStep), 3, abi.ptr(tri),
+                                            abi.ptr(bet), abi.ptr(ctab)), "WriteTecplot")
+    S, cl = e.state(), e.cells()
+    R = tmp_path / "Result"
+    lines = (R / "FluidCoor" / "1.plt").read_text().splitlines()
+    assert lines[0].startswith('Title="Particle Coordinate"')
+    zones = [i for i, l in enumerate(lines) if l.startswith("Zone")]
+    assert len(zones) == 3  # Bound2, fluid, cells (no Bound1)
+    body = [l for l in lines[2:] if not l.startswith("Zone")]
+    assert len(body) == case.N
+    sp_file = np.array([int(l.split()[3]) for l in body])
+    # within each zone, slot order of the engine's sorted arrays
+    order = cl["sortpart"]
+    expect = np.concatenate([order[(order >= a) & (order < b)] for a, b in
+                             ((case.nBound1, case.nBound1 + case.nBound2),
+                              (case.nBound1 + case.nBound2, case.N - case.nCellPar),
+                              (case.N - case.nCellPar, case.N))])
+    assert np.array_equal(sp_file, expect)
+    xyz = np.array([[float(v) for v in l.split()[:3]] for l in body])
+    assert np.allclose(xyz, S["coor"][sp_file], rtol=1e-5, atol=1e-5)
+    field = (R / "FluidField" / "1.plt").read_text().splitlines()
+    assert len(field) == 2 + case.N - case.nCellPar
+    cc = (R / "CellCoor" / "1.plt").read_text().splitlines()
+    assert sum(l.startswith("Zone N=") for l in cc) == case.nCell
+    assert len(cc) == 2 + case.nCell + case.nCellPar + case.tri.shape[0]
+    tf = (R / "CellForce" / "TF_Cell0.plt").read_text().splitlines()
+    geo = (R / "CellGeo" / "Geo_Cell1.plt").read_text().splitlines()
+    assert len(tf) == 2 and len(geo) == 2        # frame 0 created, frame 1 appended
+    assert float(geo[1].split()[2]) == pytest.approx(float(S["area"][1]), rel=1e-4)
+    nd = (R / "CellNorDir" / "1.plt").read_text().splitlines()
+    assert len(nd) == 2 + case.nCell + case.tri.shape[0]

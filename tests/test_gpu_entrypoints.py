@@ -156,7 +156,7 @@ def test_fluid_entry_points(gpu, noise):
     # EOS + SDPD + repulsion + body force (reverse-Poiseuille pattern needs positions)
     gu.call("gc_EOS", case.N, g.sortpart, g.velrhop, g.press)
     press_o = np.zeros(case.N)
-    o.eos(vr3[:, 3], press_o)
+    o.eos(np.ascontiguousarray(vr3[:, 3]), press_o)
     assert np.allclose(gu.id_order(gu.host(g.press), sp), press_o, rtol=1e-6)
     ext = abi.gc_ext_consts()
     abi.load().gc_GetExtConsts(C.byref(ext))
