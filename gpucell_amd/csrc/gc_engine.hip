@@ -20,7 +20,7 @@
 //
 // Layouts (slot order): posid float4 (x, y, z, id bits), velrho float4
 // (predicted v, rho) read by neighbour loops, vout float4 (v(t+dt), rho) =
-// the reference's Velrhop after the step, pos64 3 x double (accumulator),
+// the reference's Velrhop after the step, pos64 = x[N], y[N], z[N] doubles (accumulator),
 // coor/force/velhalf in the TypesDef float3 layout.  BeginEndCube is the
 // scan's offsets array: cube k holds slots [offs[k], offs[k+1]).
 #include "gc_kernels.h"
@@ -34,6 +34,8 @@ using namespace gc;
 namespace {
 
 constexpr int BS = GC_BS;
+// Hits buffered per lane by the force kernel (LDS: FORCE_HMAX * BS ints).
+constexpr int FORCE_HMAX = 32;
 
 struct Grid2 {
     int nc[3];
@@ -95,7 +97,7 @@ __global__ __launch_bounds__(BS) void k2_integrate(int n, GcParams P, int move,
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= n) return;
     int id = sp[p];
-    double x = pos64[3 * p], y = pos64[3 * p + 1], z = pos64[3 * p + 2];
+    double x = pos64[p], y = pos64[n + p], z = pos64[2 * n + p];
     if (move && id >= P.nf) {
         double dt = P.dt, a = 0.5 * (double)P.dt * (double)P.dt * (double)P.inv_m;
         float4 v = vout[p];
@@ -114,7 +116,7 @@ __global__ __launch_bounds__(BS) void k2_integrate(int n, GcParams P, int move,
         }
         if (wrapped) nthro[p] = tint3{img[0], img[1], img[2]};
         x = u[0]; y = u[1]; z = u[2];
-        pos64[3 * p] = x; pos64[3 * p + 1] = y; pos64[3 * p + 2] = z;
+        pos64[p] = x; pos64[n + p] = y; pos64[2 * n + p] = z;
     }
     int nx = ncube[0], ny = ncube[1], nz = ncube[2];
     int cx = cube_coord((float)x, bbox[0], P.off[0], invw[0], nx);
@@ -158,8 +160,8 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcParams P, int predi
     for (int t = b; t < e; ++t) r += tv[t] < id;
     int s = b + r;
     int o = curpos[id];
-    double x = pos_old[3 * o], y = pos_old[3 * o + 1], z = pos_old[3 * o + 2];
-    pos_new[3 * s] = x; pos_new[3 * s + 1] = y; pos_new[3 * s + 2] = z;
+    double x = pos_old[o], y = pos_old[n + o], z = pos_old[2 * n + o];
+    pos_new[s] = x; pos_new[n + s] = y; pos_new[2 * n + s] = z;
     float fx = (float)x, fy = (float)y, fz = (float)z;
     posid[s] = make_float4(fx, fy, fz, __int_as_float(id));
     coor[s] = tfloat3{fx, fy, fz};
@@ -252,46 +254,72 @@ __global__ __launch_bounds__(BS) void k2_force(int n, GcParams P, Grid2 g, int d
     key_to_cc(key[p], g, cx, cy, cz);
     const float ih = 1.0f / P.h;
     float fx = 0.f, fy = 0.f, fz = 0.f;
+    // Pass 1 scans the candidates and keeps the hits (slot << 6 | image
+    // code) in this lane's column of an LDS list; pass 2 evaluates them with
+    // the wave's lanes in step, so the expensive pair body (EOS, viscous
+    // terms, Philox + Box-Muller) runs with nearly all lanes busy instead of
+    // the ~15 % that are inside the cut-off at any one candidate.
+    __shared__ int hits[FORCE_HMAX * BS];
+    int* mine = hits + threadIdx.x;
+    int nh = 0;
+    auto pair = [&](int code) {
+        int j = code >> 6;
+        float sx = (code & 3) == 1 ? -g.L[0] : ((code & 3) == 2 ? g.L[0] : 0.0f);
+        float sy = ((code >> 2) & 3) == 1 ? -g.L[1] : (((code >> 2) & 3) == 2 ? g.L[1] : 0.0f);
+        float sz = ((code >> 4) & 3) == 1 ? -g.L[2] : (((code >> 4) & 3) == 2 ? g.L[2] : 0.0f);
+        float4 pj = posid[j];
+        float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
+        float r2 = rx * rx + ry * ry + rz * rz;
+        int idj = __float_as_int(pj.w);
+        float r = sqrtf(r2);
+        float om = 1.0f - r * ih;
+        float F = P.fnorm * om * om;
+        if (idj < P.nb2) {  // Bound1 wall: repulsion only
+            float kk = P.krep * F;
+            fx += kk * rx; fy += kk * ry; fz += kk * rz;
+            return;
+        }
+        float4 vj = velrho[j];
+        float dj = vj.w * P.inv_m;
+        float cC = (pi_d2 + (idj < P.nc ? P.c1sq : P.c2sq) * vj.w / (dj * dj)) * F;
+        float ir = 1.0f / r;
+        float ex = rx * ir, ey = ry * ir, ez = rz * ir;
+        float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
+        float ev = ex * vx + ey * vy + ez * vz;
+        float gg = F / (di * dj);
+        float ga = gg * P.visc_a, gb = gg * P.visc_b * ev;
+        fx += cC * rx - (ga * vx + gb * ex);
+        fy += cC * ry - (ga * vy + gb * ey);
+        fz += cC * rz - (ga * vz + gb * ez);
+        if (P.noise_on) {
+            float n0, n1, n2;
+            pair_normals(P, (uint32_t)id, (uint32_t)idj, n0, n1, n2);
+            float exi = ex * n0 + ey * n1 + ez * n2;
+            float amp = sqrtf(P.two_kT_over_dt * gg);
+            amp = id < idj ? amp : -amp;
+            float pa = amp * P.sq_a, pe = amp * (P.sq_ab - P.sq_a) * exi;
+            fx += pa * n0 + pe * ex;
+            fy += pa * n1 + pe * ey;
+            fz += pa * n2 + pe * ez;
+        }
+    };
     visit2(g, P.per, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
+        int code = (sx < 0.f ? 1 : (sx > 0.f ? 2 : 0)) | ((sy < 0.f ? 1 : (sy > 0.f ? 2 : 0)) << 2) |
+                   ((sz < 0.f ? 1 : (sz > 0.f ? 2 : 0)) << 4);
         for (int j = jb; j < je; ++j) {
             float4 pj = posid[j];
             float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
             float r2 = rx * rx + ry * ry + rz * rz;
-            if (r2 >= P.h2 || r2 == 0.f) continue;
-            int idj = __float_as_int(pj.w);
-            float r = sqrtf(r2);
-            float om = 1.0f - r * ih;
-            float F = P.fnorm * om * om;
-            if (idj < P.nb2) {  // Bound1 wall: repulsion only
-                float kk = P.krep * F;
-                fx += kk * rx; fy += kk * ry; fz += kk * rz;
-                continue;
-            }
-            float4 vj = velrho[j];
-            float dj = vj.w * P.inv_m;
-            float cC = (pi_d2 + (idj < P.nc ? P.c1sq : P.c2sq) * vj.w / (dj * dj)) * F;
-            float ir = 1.0f / r;
-            float ex = rx * ir, ey = ry * ir, ez = rz * ir;
-            float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
-            float ev = ex * vx + ey * vy + ez * vz;
-            float gg = F / (di * dj);
-            float ga = gg * P.visc_a, gb = gg * P.visc_b * ev;
-            fx += cC * rx - (ga * vx + gb * ex);
-            fy += cC * ry - (ga * vy + gb * ey);
-            fz += cC * rz - (ga * vz + gb * ez);
-            if (P.noise_on) {
-                float n0, n1, n2;
-                pair_normals(P, (uint32_t)id, (uint32_t)idj, n0, n1, n2);
-                float exi = ex * n0 + ey * n1 + ez * n2;
-                float amp = sqrtf(P.two_kT_over_dt * gg);
-                amp = id < idj ? amp : -amp;
-                float pa = amp * P.sq_a, pe = amp * (P.sq_ab - P.sq_a) * exi;
-                fx += pa * n0 + pe * ex;
-                fy += pa * n1 + pe * ey;
-                fz += pa * n2 + pe * ez;
+            if (r2 < P.h2 && r2 != 0.f) {
+                mine[nh * BS] = (j << 6) | code;
+                if (++nh == FORCE_HMAX) {  // list full (dense spot): drain it here
+                    for (int k = 0; k < nh; ++k) pair(mine[k * BS]);
+                    nh = 0;
+                }
             }
         }
     });
+    for (int k = 0; k < nh; ++k) pair(mine[k * BS]);
     if (driveon) fx += ext_force_x(P, pi.y);
     force[p] = tfloat3{fx, fy, fz};
     if (id < P.nc) {
@@ -415,6 +443,7 @@ struct gc_engine {
     int *d_es, *d_ee, *d_ts, *d_te;
     hipGraphExec_t gexec = nullptr;
     int g_driveon = -1;
+    bool walls_dirty = false;
 
     template <class T> int alloc(T** p, size_t n)
     {
@@ -518,16 +547,30 @@ int launch_step(gc_engine* e, int driveon, Prof* pr)
         a.es = e->d_es; a.ee = e->d_ee; a.ts = e->d_ts; a.te = e->d_te;
         a.defforce = e->d_defforce; a.force = e->d_force;
         a.velhalf = e->d_velhalf; a.velrho = e->d_velrho; a.vout = e->d_vout;
-        k_def_force<<<blocks_for(e->nm, BS), BS, 0, s>>>(P, e->nm, e->d_bbox, a);
+        // 64-thread groups: a few hundred RBCs give only ~Nm/256 groups of 256
+        k_def_force<<<blocks_for(e->nm, 64), 64, 0, s>>>(P, e->nm, e->d_bbox, a);
         GC_LAUNCH_CHECK("engine membrane force");
         mark(pr, s, "membrane force+corrector");
     }
-    if (P.nf > 0) {
-        k2_wallvel<<<gb, BS, 0, s>>>(N, P, g, e->d_posid, e->d_vout, e->d_offs, e->d_key);
-        GC_LAUNCH_CHECK("engine wall velocity");
-    }
-    mark(pr, s, "wall velocity");
+    // [18] CalBound2Vel after the corrector only feeds the output: the next
+    // step overwrites the walls' velocity (k2_density) before anything reads
+    // it.  It is computed when the state leaves the engine (finish_walls).
+    e->walls_dirty = P.nf > 0;
     e->step += 1;
+    return GC_OK;
+}
+
+// Wall velocities from the corrected flow (UpdatePar's final CalBound2Vel,
+// Particle.cu:1327) for the exported state.
+int finish_walls(gc_engine* e)
+{
+    if (!e->walls_dirty) return GC_OK;
+    GcParams P = engine_params(e);
+    Grid2 g = engine_grid(e);
+    k2_wallvel<<<blocks_for(e->N, BS), BS, 0, e->s>>>(e->N, P, g, e->d_posid, e->d_vout, e->d_offs,
+                                                        e->d_key);
+    GC_LAUNCH_CHECK("engine wall velocity");
+    e->walls_dirty = false;
     return GC_OK;
 }
 
@@ -615,6 +658,8 @@ extern "C" gc_engine* gc_EngineCreate(const gc_case_desc* d, int* status)
         return fail(st);
 
     const size_t N = (size_t)e->N, K = (size_t)e->K;
+    if (e->N >= (1 << 25))  // the force kernel packs slot << 6 into an int
+        return fail(set_error(GC_ERR_INVALID, "EngineCreate: more than 2^25 particles"));
     if ((st = e->alloc(&e->d_bbox, 6)) || (st = e->alloc(&e->d_outer, 3)) ||
         (st = e->alloc(&e->d_invw, 3)) || (st = e->alloc(&e->d_ncube, 3)))
         return fail(st);
@@ -641,7 +686,7 @@ extern "C" gc_engine* gc_EngineCreate(const gc_case_desc* d, int* status)
         std::vector<tint3> nt(N, tint3{0, 0, 0});
         std::vector<int> idn(N), zero(K + 1, 0);
         for (size_t i = 0; i < N; ++i) {
-            pos[3 * i] = d->coor[i].x; pos[3 * i + 1] = d->coor[i].y; pos[3 * i + 2] = d->coor[i].z;
+            pos[i] = d->coor[i].x; pos[N + i] = d->coor[i].y; pos[2 * N + i] = d->coor[i].z;
             tfloat3 v = d->vel ? d->vel[i] : tfloat3{0.f, 0.f, 0.f};
             if ((int)i < (int)e->consts.nBound2Start) v = tfloat3{0.f, 0.f, 0.f};
             vout_id[i] = make_float4(v.x, v.y, v.z, e->consts.DenInit);
@@ -790,7 +835,9 @@ extern "C" int gc_EngineGetState(gc_engine* e, gc_host_state* o)
 {
     if (!e || !o) return set_error(GC_ERR_INVALID, "EngineGetState: null");
     const int N = e->N, c = e->cur;
-    int st = hip_check(hipStreamSynchronize(e->s), "GetState sync");
+    int st = finish_walls(e);
+    if (st) return st;
+    st = hip_check(hipStreamSynchronize(e->s), "GetState sync");
     if (st) return st;
     std::vector<int> sp(N);
     std::vector<double> pos(3 * (size_t)N);
@@ -812,7 +859,7 @@ extern "C" int gc_EngineGetState(gc_engine* e, gc_host_state* o)
     const float c2sq = e->consts.SoundSpeed2 * e->consts.SoundSpeed2;
     for (int slot = 0; slot < N; ++slot) {
         int id = sp[slot];
-        if (o->pos) for (int d = 0; d < 3; ++d) o->pos[3 * (size_t)id + d] = pos[3 * (size_t)slot + d];
+        if (o->pos) for (int d = 0; d < 3; ++d) o->pos[3 * (size_t)id + d] = pos[(size_t)d * N + slot];
         if (o->coor) o->coor[id] = co[slot];
         if (o->velrhop) o->velrhop[id] = tfloat4{vo[slot].x, vo[slot].y, vo[slot].z, vo[slot].w};
         if (o->velhalf) o->velhalf[id] = vh[slot];
@@ -847,6 +894,9 @@ extern "C" int gc_EngineSetState(gc_engine* e, const gc_host_state* in, long ste
     cur.nthrobound = nt.data();
     if ((st = gc_EngineGetState(e, &cur))) return st;
     if (in->pos) memcpy(pos.data(), in->pos, sizeof(double) * 3 * N);
+    std::vector<double> pos_soa(3 * (size_t)N);
+    for (int i = 0; i < N; ++i)
+        for (int d = 0; d < 3; ++d) pos_soa[(size_t)d * N + i] = pos[3 * (size_t)i + d];
     if (in->velrhop) memcpy(vr.data(), in->velrhop, sizeof(tfloat4) * N);
     if (in->velhalf) memcpy(vh.data(), in->velhalf, sizeof(tfloat3) * N);
     if (in->force) memcpy(f.data(), in->force, sizeof(tfloat3) * N);
@@ -857,7 +907,7 @@ extern "C" int gc_EngineSetState(gc_engine* e, const gc_host_state* in, long ste
     e->cur = 0;
     hipStream_t s = e->s;
     unsigned long long ctr = noise_counter;
-    if ((st = upload(e->d_pos64[0], pos.data(), 3 * (size_t)N, s)) || (st = upload(e->d_vout, vo.data(), N, s)) ||
+    if ((st = upload(e->d_pos64[0], pos_soa.data(), 3 * (size_t)N, s)) || (st = upload(e->d_vout, vo.data(), N, s)) ||
         (st = upload(e->d_force, f.data(), N, s)) || (st = upload(e->d_nt[0], nt.data(), N, s)) ||
         (st = upload(e->d_sp[0], idn.data(), N, s)) || (st = upload(e->d_curpos, idn.data(), N, s)) ||
         (st = upload(e->d_ctr, &ctr, 1, s)))

@@ -82,20 +82,22 @@ __device__ __forceinline__ float u01f(uint32_t x)
 
 // Three standard normals of the pair {ida, idb}: Philox counter
 // (min id, max id, counter lo, counter hi), key (seed lo, seed hi),
-// Box-Muller on (u0, u1) and (u2, u3).
+// Box-Muller on (u0, u1) and (u2, u3).  The transcendentals are the
+// hardware instructions: v_log_f32 (log2) and v_sin_f32 / v_cos_f32, whose
+// argument is in revolutions, so cos(2 pi u) is one instruction.
 __device__ __forceinline__ void pair_normals(const GcParams& P, uint32_t ida, uint32_t idb,
                                              float& n0, float& n1, float& n2)
 {
     uint32_t lo = min(ida, idb), hi = max(ida, idb);
     uint32_t o[4];
     philox4x32_10(lo, hi, P.ctr_lo, P.ctr_hi, P.seed_lo, P.seed_hi, o);
-    float r0 = sqrtf(-2.0f * logf(u01f(o[0])));
-    float r1 = sqrtf(-2.0f * logf(u01f(o[2])));
-    float s0, c0, s1, c1;
-    sincosf(6.283185307179586f * u01f(o[1]), &s0, &c0);
-    sincosf(6.283185307179586f * u01f(o[3]), &s1, &c1);
-    (void)s1;
-    n0 = r0 * c0; n1 = r0 * s0; n2 = r1 * c1;
+    const float m2ln2 = -1.3862943611198906f;  // -2 ln 2
+    float r0 = sqrtf(m2ln2 * __builtin_amdgcn_logf(u01f(o[0])));
+    float r1 = sqrtf(m2ln2 * __builtin_amdgcn_logf(u01f(o[2])));
+    float u1 = u01f(o[1]), u3 = u01f(o[3]);
+    n0 = r0 * __builtin_amdgcn_cosf(u1);
+    n1 = r0 * __builtin_amdgcn_sinf(u1);
+    n2 = r1 * __builtin_amdgcn_cosf(u3);
 }
 
 // Cube coordinate of one axis, FP32 with no contraction so the host oracle

@@ -195,7 +195,7 @@ def test_update_and_periodic_wrap(gpu):
     g.con_cube_list()
     sp = gu.host(g.sortpart)
     rng = np.random.default_rng(4)
-    v = rng.normal(0, 30.0, size=(case.N, 3)).astype(np.float32)   # large: many wraps
+    v = rng.normal(0, 200.0, size=(case.N, 3)).astype(np.float32)  # ~1 per step: many wraps
     f = rng.normal(0, 5.0, size=(case.N, 3)).astype(np.float32)
     vr = gu.host(g.velrhop)
     vr[:, :3] = v[sp]
@@ -260,8 +260,14 @@ def test_membrane_entry_points(gpu):
     # mechanical parameters
     m = {k: z(E) for k in ("l0", "lmax", "kb", "mexp", "p", "kp", "th0")}
     mu0, kc = gu.dev(case.shearmod), gu.dev(case.bendmod)
+    # SetMechPara has no image counters (ParticleGPU.cuh:28-29): the reference
+    # calls it at init, on unwrapped positions (Particle.cu:479)
+    Lx = np.float32(case.bbox[1] - case.bbox[0])
+    unwrapped = gu.host(g.coor).copy()
+    unwrapped[:, 0] += gu.host(g.nthro)[:, 0] * Lx
+    coor_unwrapped = gu.dev(unwrapped, np.float32)
     gu.call("gc_SetMechPara", Cn, E, edge, g.curpos, m["l0"], m["lmax"], kc, mu0, m["kb"],
-            m["mexp"], m["p"], m["kp"], m["th0"], d["trinor"], d["tricen"], g.coor)
+            m["mexp"], m["p"], m["kp"], m["th0"], d["trinor"], d["tricen"], coor_unwrapped)
     mo = o.mech_para(pos, imgs)
     for k, ko in (("l0", "l0"), ("lmax", "lmax"), ("kb", "kb"), ("mexp", "mexp"), ("p", "p"),
                   ("kp", "kp")):
