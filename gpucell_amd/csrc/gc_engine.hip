@@ -94,7 +94,7 @@ __global__ __launch_bounds__(BS) void k2_integrate(int n, GcParams P, int move,
     const float* __restrict__ invw, const int* __restrict__ ncube, int* __restrict__ key,
     int* __restrict__ rank, int* __restrict__ cnt)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (p >= n) return;
     int id = sp[p];
     double x = pos64[p], y = pos64[n + p], z = pos64[2 * n + p];
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(BS) void k2_scatter(int n, const int* __restrict__ 
     const int* __restrict__ rank, const int* __restrict__ sp, const int* __restrict__ offs,
     int* __restrict__ tk, int* __restrict__ tv, int* __restrict__ cnt)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (p >= n) return;
     int k = key[p], r = rank[p];
     int q = offs[k] + r;
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcParams P, int predi
     int* __restrict__ key_new, float4* __restrict__ posid, tfloat3* __restrict__ coor,
     float4* __restrict__ velrho, tfloat3* __restrict__ velhalf)
 {
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    int q = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (q >= n) return;
     int k = tk[q], id = tv[q];
     int b = offs[k], e = offs[k + 1];
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(BS) void k2_density(int n, GcParams P, Grid2 g,
     const float4* __restrict__ posid, float4* __restrict__ velrho, const int* __restrict__ offs,
     const int* __restrict__ key)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (p >= n) return;
     float4 pi = posid[p];
     int id = __float_as_int(pi.w);
@@ -239,7 +239,7 @@ __global__ __launch_bounds__(BS) void k2_force(int n, GcParams P, Grid2 g, int d
     const int* __restrict__ offs, const int* __restrict__ key, const tfloat3* __restrict__ velhalf,
     tfloat3* __restrict__ force, float4* __restrict__ vout, const unsigned long long* __restrict__ ctr)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (p >= n) return;
     float4 pi = posid[p];
     int id = __float_as_int(pi.w);
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(BS) void k2_agg(int n, GcParams P, Grid2 g,
     const float4* __restrict__ posid, const int* __restrict__ offs, const int* __restrict__ key,
     const int* __restrict__ celltab, tfloat3* __restrict__ force, tfloat3* __restrict__ aggforce)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (p >= n) return;
     float4 pi = posid[p];
     int id = __float_as_int(pi.w);
@@ -369,7 +369,7 @@ __global__ __launch_bounds__(BS) void k2_wallvel(int n, GcParams P, Grid2 g,
     const float4* __restrict__ posid, float4* __restrict__ vout, const int* __restrict__ offs,
     const int* __restrict__ key)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (p >= n) return;
     float4 pi = posid[p];
     int id = __float_as_int(pi.w);
@@ -397,6 +397,25 @@ __global__ __launch_bounds__(BS) void k2_wallvel(int n, GcParams P, Grid2 g,
 }
 
 __global__ void k2_bump(unsigned long long* ctr) { *ctr += 1ull; }
+
+// Unwrapped membrane vertex positions in id order (FP64, x[nm] y[nm] z[nm]):
+// the membrane force kernel then reads a compact 24 B x Nm array instead of
+// gathering coor / nthrobound / curpos from the whole slot-ordered state.
+__global__ __launch_bounds__(BS) void k2_mgather(int nm, GcParams P, const int* __restrict__ curpos,
+    const tfloat3* __restrict__ coor, const tint3* __restrict__ nthro, const float* __restrict__ bbox,
+    double* __restrict__ mpos)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nm) return;
+    int s = curpos[P.nc + v];
+    tfloat3 x = coor[s];
+    tint3 n = nthro[s];
+    double L0 = (double)(bbox[1] - bbox[0]), L1 = (double)(bbox[3] - bbox[2]),
+           L2 = (double)(bbox[5] - bbox[4]);
+    mpos[v] = (double)x.x + (P.per[0] ? (double)n.x * L0 : 0.0);
+    mpos[nm + v] = (double)x.y + (P.per[1] ? (double)n.y * L1 : 0.0);
+    mpos[2 * nm + v] = (double)x.z + (P.per[2] ? (double)n.z * L2 : 0.0);
+}
 
 template <class T>
 int dalloc(T** p, size_t count)
@@ -440,6 +459,7 @@ struct gc_engine {
     tfloat3 *d_trinor, *d_tricen; float *d_triarea, *d_reftriarea;
     float *d_refarea_para, *d_refvol_para, *d_area_para, *d_vol_para, *d_area, *d_vol, *d_pararea;
     tfloat3 *d_defforce, *d_aggforce;
+    double* d_mpos;
     int *d_es, *d_ee, *d_ts, *d_te;
     hipGraphExec_t gexec = nullptr;
     int g_driveon = -1;
@@ -476,15 +496,15 @@ int launch_build(gc_engine* e, const GcParams& P, int move, int predict)
     const int N = e->N, c = e->cur, nx = 1 - c;
     hipStream_t s = e->s;
     const int gb = blocks_for(N, BS);
-    k2_integrate<<<gb, BS, 0, s>>>(N, P, move, e->d_pos64[c], e->d_vout, e->d_force, e->d_sp[c],
+    k2_integrate<<<xcd_grid(gb), BS, 0, s>>>(N, P, move, e->d_pos64[c], e->d_vout, e->d_force, e->d_sp[c],
         e->d_nt[c], e->d_bbox, e->d_invw, e->d_ncube, e->d_key0, e->d_rank, e->d_cnt);
     GC_LAUNCH_CHECK("engine integrate");
     int st = exclusive_scan(e->d_cnt, e->d_offs, e->K, s, e->d_scanws);
     if (st) return st;
-    k2_scatter<<<gb, BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[c], e->d_offs, e->d_tk, e->d_tv,
+    k2_scatter<<<xcd_grid(gb), BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[c], e->d_offs, e->d_tk, e->d_tv,
                                  e->d_cnt);
     GC_LAUNCH_CHECK("engine scatter");
-    k2_sortgather<<<gb, BS, 0, s>>>(N, P, predict, e->d_tk, e->d_tv, e->d_offs, e->d_curpos,
+    k2_sortgather<<<xcd_grid(gb), BS, 0, s>>>(N, P, predict, e->d_tk, e->d_tv, e->d_offs, e->d_curpos,
         e->d_pos64[c], e->d_pos64[nx], e->d_vout, e->d_force, e->d_nt[c], e->d_nt[nx], e->d_sp[nx],
         e->d_key, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf);
     GC_LAUNCH_CHECK("engine sort/gather");
@@ -518,10 +538,10 @@ int launch_step(gc_engine* e, int driveon, Prof* pr)
     if (st) return st;
     mark(pr, s, "neighbour build+predict");
     const int c = e->cur;
-    k2_density<<<gb, BS, 0, s>>>(N, P, g, e->d_posid, e->d_velrho, e->d_offs, e->d_key);
+    k2_density<<<xcd_grid(gb), BS, 0, s>>>(N, P, g, e->d_posid, e->d_velrho, e->d_offs, e->d_key);
     GC_LAUNCH_CHECK("engine density");
     mark(pr, s, "density+wall velocity");
-    k2_force<<<gb, BS, 0, s>>>(N, P, g, driveon, e->d_posid, e->d_velrho, e->d_offs, e->d_key,
+    k2_force<<<xcd_grid(gb), BS, 0, s>>>(N, P, g, driveon, e->d_posid, e->d_velrho, e->d_offs, e->d_key,
                                e->d_velhalf, e->d_force, e->d_vout, e->d_ctr);
     GC_LAUNCH_CHECK("engine force");
     k2_bump<<<1, 1, 0, s>>>(e->d_ctr);
@@ -533,7 +553,7 @@ int launch_step(gc_engine* e, int driveon, Prof* pr)
         GC_LAUNCH_CHECK("engine geo");
         mark(pr, s, "membrane geometry");
         if (P.De != 0.0f) {
-            k2_agg<<<gb, BS, 0, s>>>(N, P, g, e->d_posid, e->d_offs, e->d_key, e->d_celltab,
+            k2_agg<<<xcd_grid(gb), BS, 0, s>>>(N, P, g, e->d_posid, e->d_offs, e->d_key, e->d_celltab,
                                      e->d_force, e->d_aggforce);
             GC_LAUNCH_CHECK("engine morse");
         }
@@ -547,8 +567,12 @@ int launch_step(gc_engine* e, int driveon, Prof* pr)
         a.es = e->d_es; a.ee = e->d_ee; a.ts = e->d_ts; a.te = e->d_te;
         a.defforce = e->d_defforce; a.force = e->d_force;
         a.velhalf = e->d_velhalf; a.velrho = e->d_velrho; a.vout = e->d_vout;
+        a.mpos = e->d_mpos; a.nm = e->nm;
+        k2_mgather<<<blocks_for(e->nm, BS), BS, 0, s>>>(e->nm, P, e->d_curpos, e->d_coor, e->d_nt[c],
+                                                        e->d_bbox, e->d_mpos);
+        GC_LAUNCH_CHECK("engine membrane gather");
         // 64-thread groups: a few hundred RBCs give only ~Nm/256 groups of 256
-        k_def_force<<<blocks_for(e->nm, 64), 64, 0, s>>>(P, e->nm, e->d_bbox, a);
+        k_def_force<<<xcd_grid(blocks_for(e->nm, 64)), 64, 0, s>>>(P, e->nm, e->d_bbox, a);
         GC_LAUNCH_CHECK("engine membrane force");
         mark(pr, s, "membrane force+corrector");
     }
@@ -567,7 +591,7 @@ int finish_walls(gc_engine* e)
     if (!e->walls_dirty) return GC_OK;
     GcParams P = engine_params(e);
     Grid2 g = engine_grid(e);
-    k2_wallvel<<<blocks_for(e->N, BS), BS, 0, e->s>>>(e->N, P, g, e->d_posid, e->d_vout, e->d_offs,
+    k2_wallvel<<<xcd_grid(blocks_for(e->N, BS)), BS, 0, e->s>>>(e->N, P, g, e->d_posid, e->d_vout, e->d_offs,
                                                         e->d_key);
     GC_LAUNCH_CHECK("engine wall velocity");
     e->walls_dirty = false;
@@ -722,7 +746,8 @@ extern "C" gc_engine* gc_EngineCreate(const gc_case_desc* d, int* status)
             (st = e->alloc(&e->d_refvol_para, T)) || (st = e->alloc(&e->d_area_para, T)) ||
             (st = e->alloc(&e->d_vol_para, T)) || (st = e->alloc(&e->d_area, C)) ||
             (st = e->alloc(&e->d_vol, C)) || (st = e->alloc(&e->d_pararea, M)) ||
-            (st = e->alloc(&e->d_defforce, M)) || (st = e->alloc(&e->d_aggforce, M)))
+            (st = e->alloc(&e->d_defforce, M)) || (st = e->alloc(&e->d_aggforce, M)) ||
+            (st = e->alloc(&e->d_mpos, 3 * M)))
             return fail(st);
         std::vector<int> es, ee, ts, te;
         if ((st = build_incidence(e->nm, e->nedge, d->edge, e->ntri, d->tri, es, ee, ts, te)))

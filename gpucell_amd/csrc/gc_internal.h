@@ -13,6 +13,19 @@
 // Work-group size of the per-particle kernels: 4 wave64 per group.
 #define GC_BS 256
 
+// MI355X has 8 XCDs, each with its own L2; the dispatcher deals work-groups
+// to XCDs round-robin.  xcd_block() renumbers them so that XCD x runs one
+// contiguous run of logical blocks: neighbouring slots, and the neighbour
+// cubes they read, then share an L2 instead of being fetched by all eight.
+// Launch xcd_grid(nb) groups; groups whose logical id is >= nb return.
+#define GC_NXCD 8
+inline int xcd_grid(int nb) { return ((nb + GC_NXCD - 1) / GC_NXCD) * GC_NXCD; }
+__device__ __forceinline__ int xcd_block(int nb)
+{
+    int per = (nb + GC_NXCD - 1) / GC_NXCD;
+    return (int)(blockIdx.x % GC_NXCD) * per + (int)(blockIdx.x / GC_NXCD);
+}
+
 // ---------------------------------------------------------------------------
 // Parameter block: Structconst + gc_ext_consts + derived constants, passed
 // by value as a kernel argument (kernarg segment: scalar loads).

@@ -46,6 +46,9 @@ struct DefArgs {
     // engine only (NULL in the entry point): Groot-Warren corrector after
     // the membrane force, vout = (velhalf + dt/2 F/m, rho of velrho)
     const tfloat3* velhalf = nullptr; const float4* velrho = nullptr; float4* vout = nullptr;
+    // engine only: unwrapped vertex positions in id order, x[nm], y[nm],
+    // z[nm] (FP64); replaces the coor/nthro/curpos gathers of the vertices
+    const double* mpos = nullptr; int nm = 0;
 };
 
 __global__ void k_geo_cell(GcParams P, long stepcount, const float* __restrict__ bbox,

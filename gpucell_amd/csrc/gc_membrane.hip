@@ -266,9 +266,18 @@ __global__ __launch_bounds__(BS) void k_mech_para(GcParams P, int nedge, const i
 }
 
 // ---- CalCellDefForce --------------------------------------------------------
+// Position of vertex v1 (1-based) relative to this vertex: an FP64
+// difference of unwrapped positions, rounded once (both paths give the same
+// bits: x + n L is exact in FP64 for an FP32 x).
 __device__ __forceinline__ float3 vpos(const DefArgs& a, const GcParams& P, const PerBox& b, int v1,
-                                       const tfloat3& xo, const tint3& no)
+                                       const tfloat3& xo, const tint3& no, double ox, double oy,
+                                       double oz)
 {
+    if (a.mpos) {
+        int u = v1 - 1;
+        return make_float3((float)(a.mpos[u] - ox), (float)(a.mpos[a.nm + u] - oy),
+                           (float)(a.mpos[2 * a.nm + u] - oz));
+    }
     int s = a.curpos[P.nc + v1 - 1];
     return urel(a.coor[s], a.nthro[s], xo, no, b);
 }
@@ -276,19 +285,21 @@ __device__ __forceinline__ float3 vpos(const DefArgs& a, const GcParams& P, cons
 __global__ __launch_bounds__(BS) void k_def_force(GcParams P, int nm, const float* __restrict__ bbox,
                                                   DefArgs a)
 {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    int v = xcd_block((nm + (int)blockDim.x - 1) / (int)blockDim.x) * blockDim.x + threadIdx.x;
     if (v >= nm) return;
     PerBox b = load_box(P, bbox);
     int sv = a.curpos[P.nc + v];
     tfloat3 xo = a.coor[sv];
     tint3 no = a.nthro[sv];
+    double ox = 0.0, oy = 0.0, oz = 0.0;
+    if (a.mpos) { ox = a.mpos[v]; oy = a.mpos[nm + v]; oz = a.mpos[2 * nm + v]; }
     float fx = 0.f, fy = 0.f, fz = 0.f;
     // edges: WLC-POW spring (roles A, B) and bending (all four roles)
     for (int q = a.es[v]; q < a.es[v + 1]; ++q) {
         int code = a.ee[q], e = code >> 2, role = code & 3;
         int6 E = a.edge[e];
-        float3 pA = vpos(a, P, b, E.x, xo, no), pB = vpos(a, P, b, E.y, xo, no);
-        float3 pC = vpos(a, P, b, E.z, xo, no), pD = vpos(a, P, b, E.k, xo, no);
+        float3 pA = vpos(a, P, b, E.x, xo, no, ox, oy, oz), pB = vpos(a, P, b, E.y, xo, no, ox, oy, oz);
+        float3 pC = vpos(a, P, b, E.z, xo, no, ox, oy, oz), pD = vpos(a, P, b, E.k, xo, no, ox, oy, oz);
         if (role < 2) {
             float3 d = f3sub(pA, pB);
             float l = sqrtf(f3dot(d, d));
@@ -324,8 +335,8 @@ __global__ __launch_bounds__(BS) void k_def_force(GcParams P, int nm, const floa
         int code = a.te[q], t = code >> 2, k = code & 3;
         tint3 T = a.tri[t];
         int id[3] = {T.x, T.y, T.z};
-        float3 P0 = vpos(a, P, b, id[0], xo, no), P1 = vpos(a, P, b, id[1], xo, no);
-        float3 P2 = vpos(a, P, b, id[2], xo, no);
+        float3 P0 = vpos(a, P, b, id[0], xo, no, ox, oy, oz), P1 = vpos(a, P, b, id[1], xo, no, ox, oy, oz);
+        float3 P2 = vpos(a, P, b, id[2], xo, no, ox, oy, oz);
         float3 n = f3cross(f3sub(P1, P0), f3sub(P2, P0));
         float nn = sqrtf(f3dot(n, n));
         float At = 0.5f * nn;
@@ -486,7 +497,7 @@ extern "C" int gc_CalCellDefForce(int ncell, unsigned nedge, unsigned ntri, unsi
     a.Vc = d_vol_para; a.V0c = d_refvol_para;
     a.es = g_topo.d_es; a.ee = g_topo.d_ee; a.ts = g_topo.d_ts; a.te = g_topo.d_te;
     a.defforce = d_defforce; a.force = d_totalforce;
-    k_def_force<<<blocks_for(P.nm, BS), BS, 0, stream()>>>(P, P.nm, d_bbox, a);
+    k_def_force<<<xcd_grid(blocks_for(P.nm, BS)), BS, 0, stream()>>>(P, P.nm, d_bbox, a);
     GC_LAUNCH_CHECK("CalCellDefForce");
     if (h_defforce) {
         st = hip_check(hipMemcpyAsync(h_defforce, d_defforce, sizeof(tfloat3) * (size_t)P.nm,
