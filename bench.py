@@ -26,9 +26,11 @@ HBM_PEAK_GBS = 8000.0
 # force+corrector 72) and per membrane vertex (440).
 STEP_BYTES_PER_PARTICLE = 286.0
 STEP_BYTES_PER_VERTEX = 440.0
-# Dominant kernel (SDPD pair force): reads position 12 B, velocity+density
-# 16 B, pressure 4 B, id 4 B; writes force 12 B, per particle.
-FORCE_BYTES_PER_PARTICLE = 48.0
+# Dominant kernel (SDPD pair force with EOS, walls, body force and
+# corrector fused): reads posid 16 B + predicted velocity/density 16 B +
+# half-step velocity 12 B, writes force 12 B + velocity/density 16 B, per
+# particle (neighbour data come from cache).
+FORCE_BYTES_PER_PARTICLE = 72.0
 
 
 def log(msg):
