@@ -145,6 +145,8 @@ PROTOTYPES = {
     "gc_FreeCase": (None, [C.POINTER(gc_case_desc)]),
     "gc_WriteCase": (I, [C.c_char_p, C.POINTER(gc_case_desc)]),
     "gc_EngineWriteTecplot": (I, [P, C.c_char_p, L, L, C.c_float, L, P, P, P]),
+    "gc_EngineSave": (I, [P, C.c_char_p]),
+    "gc_EngineLoad": (I, [P, C.c_char_p]),
 }
 
 # Entry points that replace the reference's cupar/cudiv functions.

@@ -566,3 +566,90 @@ extern "C" int gc_EngineWriteTecplot(gc_engine* e, const char* dir_c, long stepd
     }
     return GC_OK;
 }
+
+// -------------------------------------------------------------------------
+// Checkpoint / restart.  The reference has none (its outputs carry 5
+// significant digits).  A checkpoint holds the engine's dynamic state in id
+// order at full precision: FP64 positions, Velrhop, VelHalf, TotalForce,
+// nThroBound, the step and the noise counter.  Restoring it into an engine
+// created from the same case resumes bit for bit: the slot order is the
+// (cube, id) order of the positions, which the restore rebuilds.
+namespace {
+const char kMagic[8] = {'G', 'C', 'E', 'L', 'L', 'C', 'K', '1'};
+struct CkHeader {
+    char magic[8];
+    int64_t n, nm, ncell, step;
+    uint64_t counter;
+};
+}  // namespace
+
+extern "C" int gc_EngineSave(gc_engine* e, const char* path)
+{
+    if (!e || !path) return gc::set_error(GC_ERR_INVALID, "EngineSave: null argument");
+    gc_engine_info info;
+    int st = gc_EngineInfo(e, &info);
+    if (st) return st;
+    const size_t N = (size_t)info.nTotalPar;
+    std::vector<double> pos(3 * N);
+    std::vector<tfloat4> vr(N);
+    std::vector<tfloat3> vh(N), f(N);
+    std::vector<tint3> nt(N);
+    gc_host_state hs;
+    std::memset(&hs, 0, sizeof hs);
+    hs.pos = pos.data(); hs.velrhop = vr.data(); hs.velhalf = vh.data(); hs.force = f.data();
+    hs.nthrobound = nt.data();
+    if ((st = gc_EngineGetState(e, &hs))) return st;
+    CkHeader h;
+    std::memcpy(h.magic, kMagic, 8);
+    h.n = (int64_t)N; h.nm = info.nCellPar; h.ncell = info.nCell; h.step = info.step;
+    h.counter = info.noise_counter;
+    std::string tmp = std::string(path) + ".tmp";
+    FILE* fp = std::fopen(tmp.c_str(), "wb");
+    if (!fp) return gc::set_error(GC_ERR_INVALID, std::string("cannot write ") + tmp);
+    bool ok = std::fwrite(&h, sizeof h, 1, fp) == 1 &&
+              std::fwrite(pos.data(), sizeof(double), 3 * N, fp) == 3 * N &&
+              std::fwrite(vr.data(), sizeof(tfloat4), N, fp) == N &&
+              std::fwrite(vh.data(), sizeof(tfloat3), N, fp) == N &&
+              std::fwrite(f.data(), sizeof(tfloat3), N, fp) == N &&
+              std::fwrite(nt.data(), sizeof(tint3), N, fp) == N;
+    ok = (std::fclose(fp) == 0) && ok;
+    if (!ok || std::rename(tmp.c_str(), path) != 0)
+        return gc::set_error(GC_ERR_INVALID, std::string("write failed: ") + path);
+    return GC_OK;
+}
+
+extern "C" int gc_EngineLoad(gc_engine* e, const char* path)
+{
+    if (!e || !path) return gc::set_error(GC_ERR_INVALID, "EngineLoad: null argument");
+    gc_engine_info info;
+    int st = gc_EngineInfo(e, &info);
+    if (st) return st;
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return gc::set_error(GC_ERR_INVALID, std::string("cannot open ") + path);
+    CkHeader h;
+    if (std::fread(&h, sizeof h, 1, fp) != 1 || std::memcmp(h.magic, kMagic, 8) != 0) {
+        std::fclose(fp);
+        return gc::set_error(GC_ERR_INVALID, std::string("not a checkpoint: ") + path);
+    }
+    if (h.n != info.nTotalPar || h.nm != info.nCellPar || h.ncell != info.nCell) {
+        std::fclose(fp);
+        return gc::set_error(GC_ERR_INVALID, "checkpoint was written by a different case");
+    }
+    const size_t N = (size_t)h.n;
+    std::vector<double> pos(3 * N);
+    std::vector<tfloat4> vr(N);
+    std::vector<tfloat3> vh(N), f(N);
+    std::vector<tint3> nt(N);
+    bool ok = std::fread(pos.data(), sizeof(double), 3 * N, fp) == 3 * N &&
+              std::fread(vr.data(), sizeof(tfloat4), N, fp) == N &&
+              std::fread(vh.data(), sizeof(tfloat3), N, fp) == N &&
+              std::fread(f.data(), sizeof(tfloat3), N, fp) == N &&
+              std::fread(nt.data(), sizeof(tint3), N, fp) == N;
+    std::fclose(fp);
+    if (!ok) return gc::set_error(GC_ERR_INVALID, std::string("truncated checkpoint: ") + path);
+    gc_host_state hs;
+    std::memset(&hs, 0, sizeof hs);
+    hs.pos = pos.data(); hs.velrhop = vr.data(); hs.velhalf = vh.data(); hs.force = f.data();
+    hs.nthrobound = nt.data();
+    return gc_EngineSetState(e, &hs, (long)h.step, h.counter);
+}

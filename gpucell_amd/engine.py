@@ -76,6 +76,25 @@ class Engine:
         abi.check(self.lib.gc_EngineSetState(self.h, C.byref(hs), int(step), int(counter)),
                   "EngineSetState")
 
+    def save(self, path: str):
+        """Checkpoint (bitwise-resumable) to `path`."""
+        abi.check(self.lib.gc_EngineSave(self.h, str(path).encode()), "EngineSave")
+
+    def load(self, path: str):
+        abi.check(self.lib.gc_EngineLoad(self.h, str(path).encode()), "EngineLoad")
+
+    def write_tecplot(self, directory: str, stepdata: int, stepcount: int, stepoutput: int):
+        """One frame of the reference's Tecplot output under directory/Result."""
+        c = self.case
+        self._topo = (np.ascontiguousarray(c.tri, np.int32),
+                      np.ascontiguousarray(c.beginendtri, np.int32),
+                      np.ascontiguousarray(c.celltab, np.int32))
+        t, b, ct = self._topo
+        abi.check(self.lib.gc_EngineWriteTecplot(
+            self.h, str(directory).encode(), int(stepdata), int(stepcount),
+            float(c.consts.TimeStep), int(stepoutput), abi.ptr(t) if t.size else None,
+            abi.ptr(b) if b.size else None, abi.ptr(ct) if ct.size else None), "EngineWriteTecplot")
+
     def cells(self) -> dict:
         i = self.info()
         N, K = self.N, i.sizecube

@@ -319,6 +319,14 @@ int gc_EngineWriteTecplot(gc_engine* e, const char* dir, long stepdata, long ste
                           float timestep, long stepoutput, const tint3* tri,
                           const tint2* beginendtri, const int* celltab);
 
+/* Checkpoint / restart (no reference counterpart: the reference's outputs
+ * keep 5 digits).  Save writes the dynamic state (FP64 positions, Velrhop,
+ * VelHalf, TotalForce, nThroBound, step, noise counter) in id order; Load
+ * restores it into an engine created from the same case, after which the
+ * run continues bit for bit. */
+int gc_EngineSave(gc_engine* e, const char* path);
+int gc_EngineLoad(gc_engine* e, const char* path);
+
 /* Per-kernel averages of the last gc_EngineProfile call, in microseconds. */
 typedef struct {
     int nkernels;

@@ -176,3 +176,24 @@ def test_engine_tecplot_frames(gpu, tmp_path):
     assert float(geo[1].split()[2]) == pytest.approx(float(S["area"][1]), rel=1e-4)
     nd = (R / "CellNorDir" / "1.plt").read_text().splitlines()
     assert len(nd) == 2 + case.nCell + case.tri.shape[0]
+
+
+def test_checkpoint_resume_is_bitwise(gpu, tmp_path):
+    case = channel_case(De=0.02)
+    a = Engine(case)
+    a.step(6, graph=True)
+    ck = tmp_path / "state.ck"
+    a.save(ck)
+    a.step(5, graph=True)
+    sa = a.state()
+    b = Engine(case)
+    b.load(ck)
+    assert b.info().step == 6 and b.info().noise_counter == 6
+    b.step(5, graph=True)
+    sb = b.state()
+    for k in ("pos", "velrhop", "velhalf", "force", "nthrobound", "area", "vol", "defforce"):
+        assert np.array_equal(sa[k], sb[k]), k
+    # a checkpoint of another case is refused
+    other = Engine(box_case())
+    with pytest.raises(Exception, match="different case"):
+        other.load(ck)
