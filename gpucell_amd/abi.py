@@ -147,6 +147,9 @@ PROTOTYPES = {
     "gc_EngineWriteTecplot": (I, [P, C.c_char_p, L, L, C.c_float, L, P, P, P]),
     "gc_EngineSave": (I, [P, C.c_char_p]),
     "gc_EngineLoad": (I, [P, C.c_char_p]),
+    "gc_EnsembleCreate": (P, [C.POINTER(gc_case_desc), I, C.POINTER(C.c_int)]),
+    "gc_EnsembleGetState": (I, [P, I, C.POINTER(gc_host_state)]),
+    "gc_EnsembleCount": (I, [P]),
 }
 
 # Entry points that replace the reference's cupar/cudiv functions.
@@ -159,6 +162,10 @@ REFERENCE_ENTRY_POINTS = [
 ]
 
 _lib = None
+
+# status codes of gpucell.h
+GC_OK, GC_ERR_INVALID, GC_ERR_NOCONSTS, GC_ERR_GRID = 0, -1, -2, -3
+GC_ERR_NOMEM, GC_ERR_TOPOLOGY, GC_ERR_HIP, GC_ERR_NONFINITE = -4, -5, -6, -7
 
 
 class GcError(RuntimeError):

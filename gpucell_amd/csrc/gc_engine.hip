@@ -1,6 +1,9 @@
 // gc_engine.hip -- the engine: library-owned state and the step of
 // Particle::UpdatePar (src/Particle.cu:1310-1329) as a fixed sequence of
-// fused kernels on one stream, captured into a HIP graph for replay.
+// fused kernels on one stream, captured into a HIP graph for replay.  One
+// engine holds one suspension (gc_EngineCreate) or a batch of independent
+// suspensions of any sizes stepped together by the same kernels
+// (gc_EnsembleCreate).
 //
 // Per step (reference launch numbers of UpdatePar in brackets):
 //   k2_integrate   [1,2,3]  r += dt v + dt^2/2 F/m in FP64, periodic wrap,
@@ -15,14 +18,22 @@
 //                           repulsion, body force; corrector for fluid
 //   k_geo_cell     [14]     membrane geometry, FP64 cell area/volume
 //   k2_agg         [16]     Morse between cells (when De != 0)
-//   k_def_force    [15,17]  membrane forces, corrector for membrane
-//   k2_wallvel     [18]     wall velocity from the corrected flow
+//   k2_mgather, k_def_force [15,17] membrane forces, corrector for membrane
+//   k2_wallvel     [18]     wall velocity from the corrected flow (on export)
 //
 // Layouts (slot order): posid float4 (x, y, z, id bits), velrho float4
 // (predicted v, rho) read by neighbour loops, vout float4 (v(t+dt), rho) =
-// the reference's Velrhop after the step, pos64 = x[N], y[N], z[N] doubles (accumulator),
-// coor/force/velhalf in the TypesDef float3 layout.  BeginEndCube is the
-// scan's offsets array: cube k holds slots [offs[k], offs[k+1]).
+// the reference's Velrhop after the step, pos64 = x[N], y[N], z[N] doubles
+// (accumulator), coor/force/velhalf in the TypesDef float3 layout.
+// BeginEndCube is the scan's offsets array: cube k holds [offs[k], offs[k+1]).
+//
+// Batches: replicas are concatenated with the id layout [Bound1 of all |
+// Bound2 of all | fluid of all | membrane of all] and disjoint ranges of cube
+// keys; a kernel finds a particle's replica from its id (rep_of) and reads
+// that replica's GcRep (parameters, grid, box).  Slots of one replica are
+// contiguous and ordered by (cube, id) as when it runs alone, and the pair
+// noise uses the replica's own ids and seed, so every replica evolves bit
+// for bit as its solo run.
 #include "gc_kernels.h"
 
 #include <math.h>
@@ -37,44 +48,53 @@ constexpr int BS = GC_BS;
 // Hits buffered per lane by the force kernel (LDS: FORCE_HMAX * BS ints).
 constexpr int FORCE_HMAX = 32;
 
-struct Grid2 {
-    int nc[3];
-    float L[3];
-};
-
-__device__ __forceinline__ void key_to_cc(int k, const Grid2& g, int& cx, int& cy, int& cz)
+__device__ __forceinline__ void key_to_cc(int k, const GcRep& R, int& cx, int& cy, int& cz)
 {
-    cx = k % g.nc[0];
-    int t = k / g.nc[0];
-    cy = t % g.nc[1];
-    cz = t / g.nc[1];
+    k -= R.cube_off;
+    cx = k % R.nc[0];
+    int t = k / R.nc[0];
+    cy = t % R.nc[1];
+    cz = t / R.nc[1];
+}
+
+// The replica of global id `id`: the kernel argument for one suspension,
+// the table entry for a batch (a reference either way: a by-value copy of
+// the whole record costs registers or scratch).
+template <bool ENS>
+__device__ __forceinline__ const GcRep& rep_of_id(const GcRep& R0, const GcRep* __restrict__ reps,
+                                                  const int* __restrict__ rep_of, int id)
+{
+    if constexpr (ENS) return reps[rep_of[id]];
+    else return R0;
 }
 
 // Contiguous slot ranges of the 27 neighbour cubes (x-fastest keys: one
 // range per non-wrapping (y, z) row), with the image shift of each range.
 template <class Fn>
-__device__ __forceinline__ void visit2(const Grid2& g, const int* per, const int* __restrict__ offs,
-                                       int cx, int cy, int cz, Fn&& fn)
+__device__ __forceinline__ void visit2(const GcRep& R, const int* __restrict__ offs_all, int cx,
+                                       int cy, int cz, Fn&& fn)
 {
+    const int* offs = offs_all + R.cube_off;
+    const int* per = R.P.per;
     for (int dz = -1; dz <= 1; ++dz) {
         int z = cz + dz;
         float sz = 0.0f;
-        if (z < 0) { if (!per[2]) continue; z += g.nc[2]; sz = -g.L[2]; }
-        else if (z >= g.nc[2]) { if (!per[2]) continue; z -= g.nc[2]; sz = g.L[2]; }
+        if (z < 0) { if (!per[2]) continue; z += R.nc[2]; sz = -R.L[2]; }
+        else if (z >= R.nc[2]) { if (!per[2]) continue; z -= R.nc[2]; sz = R.L[2]; }
         for (int dy = -1; dy <= 1; ++dy) {
             int y = cy + dy;
             float sy = 0.0f;
-            if (y < 0) { if (!per[1]) continue; y += g.nc[1]; sy = -g.L[1]; }
-            else if (y >= g.nc[1]) { if (!per[1]) continue; y -= g.nc[1]; sy = g.L[1]; }
-            int row = (z * g.nc[1] + y) * g.nc[0];
+            if (y < 0) { if (!per[1]) continue; y += R.nc[1]; sy = -R.L[1]; }
+            else if (y >= R.nc[1]) { if (!per[1]) continue; y -= R.nc[1]; sy = R.L[1]; }
+            int row = (z * R.nc[1] + y) * R.nc[0];
             int xlo = cx - 1, xhi = cx + 1;
             if (xlo < 0) {
-                if (per[0]) fn(offs[row + g.nc[0] - 1], offs[row + g.nc[0]], -g.L[0], sy, sz);
+                if (per[0]) fn(offs[row + R.nc[0] - 1], offs[row + R.nc[0]], -R.L[0], sy, sz);
                 xlo = 0;
             }
-            if (xhi >= g.nc[0]) {
-                if (per[0]) fn(offs[row], offs[row + 1], g.L[0], sy, sz);
-                xhi = g.nc[0] - 1;
+            if (xhi >= R.nc[0]) {
+                if (per[0]) fn(offs[row], offs[row + 1], R.L[0], sy, sz);
+                xhi = R.nc[0] - 1;
             }
             fn(offs[row + xlo], offs[row + xhi + 1], 0.0f, sy, sz);
         }
@@ -88,15 +108,17 @@ __device__ __forceinline__ float ext_force_x(const GcParams& P, float y)
 }
 
 // [1,2,3] integrate (FP64), wrap, key, histogram rank
-__global__ __launch_bounds__(BS) void k2_integrate(int n, GcParams P, int move,
-    double* __restrict__ pos64, const float4* __restrict__ vout, const tfloat3* __restrict__ force,
-    const int* __restrict__ sp, tint3* __restrict__ nthro, const float* __restrict__ bbox,
-    const float* __restrict__ invw, const int* __restrict__ ncube, int* __restrict__ key,
-    int* __restrict__ rank, int* __restrict__ cnt)
+template <bool ENS>
+__global__ __launch_bounds__(BS) void k2_integrate(int n, GcRep R0, const GcRep* __restrict__ reps,
+    const int* __restrict__ rep_of, int move, double* __restrict__ pos64,
+    const float4* __restrict__ vout, const tfloat3* __restrict__ force, const int* __restrict__ sp,
+    tint3* __restrict__ nthro, int* __restrict__ key, int* __restrict__ rank, int* __restrict__ cnt)
 {
     int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (p >= n) return;
     int id = sp[p];
+    const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
+    const GcParams& P = R.P;
     double x = pos64[p], y = pos64[n + p], z = pos64[2 * n + p];
     if (move && id >= P.nf) {
         double dt = P.dt, a = 0.5 * (double)P.dt * (double)P.dt * (double)P.inv_m;
@@ -109,8 +131,8 @@ __global__ __launch_bounds__(BS) void k2_integrate(int n, GcParams P, int move,
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             if (!P.per[d]) continue;
-            double lo = bbox[2 * d], hi = bbox[2 * d + 1];
-            double L = (double)(bbox[2 * d + 1] - bbox[2 * d]);
+            double lo = R.bbox[2 * d], hi = R.bbox[2 * d + 1];
+            double L = (double)(R.bbox[2 * d + 1] - R.bbox[2 * d]);
             if (u[d] < lo) { u[d] += L; img[d] -= 1; wrapped = true; }
             else if (u[d] >= hi) { u[d] -= L; img[d] += 1; wrapped = true; }
         }
@@ -118,11 +140,10 @@ __global__ __launch_bounds__(BS) void k2_integrate(int n, GcParams P, int move,
         x = u[0]; y = u[1]; z = u[2];
         pos64[p] = x; pos64[n + p] = y; pos64[2 * n + p] = z;
     }
-    int nx = ncube[0], ny = ncube[1], nz = ncube[2];
-    int cx = cube_coord((float)x, bbox[0], P.off[0], invw[0], nx);
-    int cy = cube_coord((float)y, bbox[2], P.off[1], invw[1], ny);
-    int cz = cube_coord((float)z, bbox[4], P.off[2], invw[2], nz);
-    int k = cx + nx * (cy + ny * cz);
+    int cx = cube_coord((float)x, R.bbox[0], P.off[0], R.invw[0], R.nc[0]);
+    int cy = cube_coord((float)y, R.bbox[2], P.off[1], R.invw[1], R.nc[1]);
+    int cz = cube_coord((float)z, R.bbox[4], P.off[2], R.invw[2], R.nc[2]);
+    int k = R.cube_off + cx + R.nc[0] * (cy + R.nc[1] * cz);
     key[p] = k;
     rank[p] = atomicAdd(&cnt[k], 1);
 }
@@ -144,9 +165,11 @@ __global__ __launch_bounds__(BS) void k2_scatter(int n, const int* __restrict__ 
 
 // [4,5,6,8] final slot = cube start + number of smaller ids in the cube;
 // gather of the particle from its old slot curpos[id]; predictor.
-__global__ __launch_bounds__(BS) void k2_sortgather(int n, GcParams P, int predict,
-    const int* __restrict__ tk, const int* __restrict__ tv, const int* __restrict__ offs,
-    int* __restrict__ curpos, const double* __restrict__ pos_old, double* __restrict__ pos_new,
+template <bool ENS>
+__global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep* __restrict__ reps,
+    const int* __restrict__ rep_of, int predict, const int* __restrict__ tk,
+    const int* __restrict__ tv, const int* __restrict__ offs, int* __restrict__ curpos,
+    const double* __restrict__ pos_old, double* __restrict__ pos_new,
     const float4* __restrict__ vout, const tfloat3* __restrict__ force,
     const tint3* __restrict__ nt_old, tint3* __restrict__ nt_new, int* __restrict__ sp_new,
     int* __restrict__ key_new, float4* __restrict__ posid, tfloat3* __restrict__ coor,
@@ -169,7 +192,9 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcParams P, int predi
     sp_new[s] = id;
     key_new[s] = k;
     float4 v = vout[o];
-    if (predict && id >= P.nf) {
+    if (predict && id >= R0.P.nf) {
+        const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
+        const GcParams& P = R.P;
         tfloat3 f = force[o];
         float h = 0.5f * P.dt * P.inv_m, l = P.lam * P.dt * P.inv_m;
         velhalf[s] = tfloat3{v.x + h * f.x, v.y + h * f.y, v.z + h * f.z};
@@ -183,21 +208,24 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcParams P, int predi
 
 // [7,9] density of moving particles; velocity of Bound2 walls from the
 // predicted velocities of their moving neighbours.
-__global__ __launch_bounds__(BS) void k2_density(int n, GcParams P, Grid2 g,
-    const float4* __restrict__ posid, float4* __restrict__ velrho, const int* __restrict__ offs,
-    const int* __restrict__ key)
+template <bool ENS>
+__global__ __launch_bounds__(BS) void k2_density(int n, GcRep R0, const GcRep* __restrict__ reps,
+    const int* __restrict__ rep_of, const float4* __restrict__ posid, float4* __restrict__ velrho,
+    const int* __restrict__ offs, const int* __restrict__ key)
 {
     int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (p >= n) return;
     float4 pi = posid[p];
     int id = __float_as_int(pi.w);
-    if (id < P.nb2) return;
+    if (id < R0.P.nb2) return;
+    const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
+    const GcParams& P = R.P;
     int cx, cy, cz;
-    key_to_cc(key[p], g, cx, cy, cz);
+    key_to_cc(key[p], R, cx, cy, cz);
     const float ih = 1.0f / P.h;
     if (id >= P.nf) {
         float d = 0.0f;  // self term added below
-        visit2(g, P.per, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
+        visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
             for (int j = jb; j < je; ++j) {
                 float4 pj = posid[j];
                 float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
@@ -211,7 +239,7 @@ __global__ __launch_bounds__(BS) void k2_density(int n, GcParams P, Grid2 g,
         velrho[p].w = P.m * P.wnorm * (d + 1.0f);
     } else {
         float sw = 0.f, svx = 0.f, svy = 0.f, svz = 0.f;
-        visit2(g, P.per, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
+        visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
             for (int j = jb; j < je; ++j) {
                 float4 pj = posid[j];
                 float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
@@ -234,24 +262,29 @@ __global__ __launch_bounds__(BS) void k2_density(int n, GcParams P, Grid2 g,
 // [10-13,17] SDPD pair force with the pressure from the EOS on the fly,
 // Bound1 repulsion, body force; corrector for fluid (membrane vertices get
 // it after their membrane force).
-__global__ __launch_bounds__(BS) void k2_force(int n, GcParams P, Grid2 g, int driveon,
-    const float4* __restrict__ posid, const float4* __restrict__ velrho,
-    const int* __restrict__ offs, const int* __restrict__ key, const tfloat3* __restrict__ velhalf,
-    tfloat3* __restrict__ force, float4* __restrict__ vout, const unsigned long long* __restrict__ ctr)
+template <bool ENS>
+__global__ __launch_bounds__(BS) void k2_force(int n, GcRep R0, const GcRep* __restrict__ reps,
+    const int* __restrict__ rep_of, int driveon, const float4* __restrict__ posid,
+    const float4* __restrict__ velrho, const int* __restrict__ offs, const int* __restrict__ key,
+    const tfloat3* __restrict__ velhalf, tfloat3* __restrict__ force, float4* __restrict__ vout,
+    const unsigned long long* __restrict__ ctr)
 {
     int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (p >= n) return;
     float4 pi = posid[p];
     int id = __float_as_int(pi.w);
-    if (id < P.nf) { force[p] = tfloat3{0.f, 0.f, 0.f}; return; }
+    if (id < R0.P.nf) { force[p] = tfloat3{0.f, 0.f, 0.f}; return; }
+    const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
+    GcParams P = R.P;
     unsigned long long c = *ctr;
     P.ctr_lo = (uint32_t)c;
     P.ctr_hi = (uint32_t)(c >> 32);
+    const int lid = ENS ? local_id(R, id) : id;
     float4 vi = velrho[p];
     float di = vi.w * P.inv_m;
     float pi_d2 = (id < P.nc ? P.c1sq : P.c2sq) * vi.w / (di * di);
     int cx, cy, cz;
-    key_to_cc(key[p], g, cx, cy, cz);
+    key_to_cc(key[p], R, cx, cy, cz);
     const float ih = 1.0f / P.h;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     // Pass 1 scans the candidates and keeps the hits (slot << 6 | image
@@ -264,9 +297,9 @@ __global__ __launch_bounds__(BS) void k2_force(int n, GcParams P, Grid2 g, int d
     int nh = 0;
     auto pair = [&](int code) {
         int j = code >> 6;
-        float sx = (code & 3) == 1 ? -g.L[0] : ((code & 3) == 2 ? g.L[0] : 0.0f);
-        float sy = ((code >> 2) & 3) == 1 ? -g.L[1] : (((code >> 2) & 3) == 2 ? g.L[1] : 0.0f);
-        float sz = ((code >> 4) & 3) == 1 ? -g.L[2] : (((code >> 4) & 3) == 2 ? g.L[2] : 0.0f);
+        float sx = (code & 3) == 1 ? -R.L[0] : ((code & 3) == 2 ? R.L[0] : 0.0f);
+        float sy = ((code >> 2) & 3) == 1 ? -R.L[1] : (((code >> 2) & 3) == 2 ? R.L[1] : 0.0f);
+        float sz = ((code >> 4) & 3) == 1 ? -R.L[2] : (((code >> 4) & 3) == 2 ? R.L[2] : 0.0f);
         float4 pj = posid[j];
         float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
         float r2 = rx * rx + ry * ry + rz * rz;
@@ -292,18 +325,19 @@ __global__ __launch_bounds__(BS) void k2_force(int n, GcParams P, Grid2 g, int d
         fy += cC * ry - (ga * vy + gb * ey);
         fz += cC * rz - (ga * vz + gb * ez);
         if (P.noise_on) {
+            int ljd = ENS ? local_id(R, idj) : idj;
             float n0, n1, n2;
-            pair_normals(P, (uint32_t)id, (uint32_t)idj, n0, n1, n2);
+            pair_normals(P, (uint32_t)lid, (uint32_t)ljd, n0, n1, n2);
             float exi = ex * n0 + ey * n1 + ez * n2;
             float amp = sqrtf(P.two_kT_over_dt * gg);
-            amp = id < idj ? amp : -amp;
+            amp = lid < ljd ? amp : -amp;
             float pa = amp * P.sq_a, pe = amp * (P.sq_ab - P.sq_a) * exi;
             fx += pa * n0 + pe * ex;
             fy += pa * n1 + pe * ey;
             fz += pa * n2 + pe * ez;
         }
     };
-    visit2(g, P.per, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
+    visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
         int code = (sx < 0.f ? 1 : (sx > 0.f ? 2 : 0)) | ((sy < 0.f ? 1 : (sy > 0.f ? 2 : 0)) << 2) |
                    ((sz < 0.f ? 1 : (sz > 0.f ? 2 : 0)) << 4);
         for (int j = jb; j < je; ++j) {
@@ -330,21 +364,26 @@ __global__ __launch_bounds__(BS) void k2_force(int n, GcParams P, Grid2 g, int d
 }
 
 // [16] Morse between membrane vertices of different cells
-__global__ __launch_bounds__(BS) void k2_agg(int n, GcParams P, Grid2 g,
-    const float4* __restrict__ posid, const int* __restrict__ offs, const int* __restrict__ key,
-    const int* __restrict__ celltab, tfloat3* __restrict__ force, tfloat3* __restrict__ aggforce)
+template <bool ENS>
+__global__ __launch_bounds__(BS) void k2_agg(int n, GcRep R0, const GcRep* __restrict__ reps,
+    const int* __restrict__ rep_of, const float4* __restrict__ posid, const int* __restrict__ offs,
+    const int* __restrict__ key, const int* __restrict__ celltab, tfloat3* __restrict__ force,
+    tfloat3* __restrict__ aggforce)
 {
     int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (p >= n) return;
     float4 pi = posid[p];
     int id = __float_as_int(pi.w);
-    if (id < P.nc) return;
+    if (id < R0.P.nc) return;
+    const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
+    const GcParams& P = R.P;
+    if (P.De == 0.0f) return;
     int cell = celltab[id - P.nc];
     int cx, cy, cz;
-    key_to_cc(key[p], g, cx, cy, cz);
+    key_to_cc(key[p], R, cx, cy, cz);
     const float tb = 2.0f * P.beta * P.De;
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    visit2(g, P.per, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
+    visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
         for (int j = jb; j < je; ++j) {
             float4 pj = posid[j];
             int idj = __float_as_int(pj.w);
@@ -365,21 +404,24 @@ __global__ __launch_bounds__(BS) void k2_agg(int n, GcParams P, Grid2 g,
 }
 
 // [18] wall velocity from the corrected velocities; walls' entry of vout
-__global__ __launch_bounds__(BS) void k2_wallvel(int n, GcParams P, Grid2 g,
-    const float4* __restrict__ posid, float4* __restrict__ vout, const int* __restrict__ offs,
-    const int* __restrict__ key)
+template <bool ENS>
+__global__ __launch_bounds__(BS) void k2_wallvel(int n, GcRep R0, const GcRep* __restrict__ reps,
+    const int* __restrict__ rep_of, const float4* __restrict__ posid, float4* __restrict__ vout,
+    const int* __restrict__ offs, const int* __restrict__ key)
 {
     int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (p >= n) return;
     float4 pi = posid[p];
     int id = __float_as_int(pi.w);
-    if (id >= P.nf) return;
+    if (id >= R0.P.nf) return;
+    const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
+    const GcParams& P = R.P;
     if (id < P.nb2) { vout[p] = make_float4(0.f, 0.f, 0.f, P.rho0); return; }
     int cx, cy, cz;
-    key_to_cc(key[p], g, cx, cy, cz);
+    key_to_cc(key[p], R, cx, cy, cz);
     const float ih = 1.0f / P.h;
     float sw = 0.f, svx = 0.f, svy = 0.f, svz = 0.f;
-    visit2(g, P.per, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
+    visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
         for (int j = jb; j < je; ++j) {
             float4 pj = posid[j];
             float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
@@ -401,20 +443,22 @@ __global__ void k2_bump(unsigned long long* ctr) { *ctr += 1ull; }
 // Unwrapped membrane vertex positions in id order (FP64, x[nm] y[nm] z[nm]):
 // the membrane force kernel then reads a compact 24 B x Nm array instead of
 // gathering coor / nthrobound / curpos from the whole slot-ordered state.
-__global__ __launch_bounds__(BS) void k2_mgather(int nm, GcParams P, const int* __restrict__ curpos,
-    const tfloat3* __restrict__ coor, const tint3* __restrict__ nthro, const float* __restrict__ bbox,
-    double* __restrict__ mpos)
+__global__ __launch_bounds__(BS) void k2_mgather(int nm, GcRep R0, const GcRep* __restrict__ reps,
+    const int* __restrict__ cell_rep, const int* __restrict__ celltab,
+    const int* __restrict__ curpos, const tfloat3* __restrict__ coor,
+    const tint3* __restrict__ nthro, double* __restrict__ mpos)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nm) return;
-    int s = curpos[P.nc + v];
+    const GcRep& R = reps ? reps[cell_rep[celltab[v]]] : R0;
+    int s = curpos[R0.P.nc + v];
     tfloat3 x = coor[s];
-    tint3 n = nthro[s];
-    double L0 = (double)(bbox[1] - bbox[0]), L1 = (double)(bbox[3] - bbox[2]),
-           L2 = (double)(bbox[5] - bbox[4]);
-    mpos[v] = (double)x.x + (P.per[0] ? (double)n.x * L0 : 0.0);
-    mpos[nm + v] = (double)x.y + (P.per[1] ? (double)n.y * L1 : 0.0);
-    mpos[2 * nm + v] = (double)x.z + (P.per[2] ? (double)n.z * L2 : 0.0);
+    tint3 nn = nthro[s];
+    double L0 = (double)(R.bbox[1] - R.bbox[0]), L1 = (double)(R.bbox[3] - R.bbox[2]),
+           L2 = (double)(R.bbox[5] - R.bbox[4]);
+    mpos[v] = (double)x.x + (R.P.per[0] ? (double)nn.x * L0 : 0.0);
+    mpos[nm + v] = (double)x.y + (R.P.per[1] ? (double)nn.y * L1 : 0.0);
+    mpos[2 * nm + v] = (double)x.z + (R.P.per[2] ? (double)nn.z * L2 : 0.0);
 }
 
 template <class T>
@@ -434,19 +478,29 @@ int upload(T* d, const T* h, size_t count, hipStream_t s)
     return hip_check(hipMemcpyAsync(d, h, sizeof(T) * count, hipMemcpyHostToDevice, s), "engine H2D");
 }
 
+// Per-replica bookkeeping on the host.
+struct RepMeta {
+    int nb1, nb2, nfl, nm, ncell, ntri, nedge;   // the replica's own counts
+    int vtx_off, cell_off, tri_off, edge_off;    // its first membrane vertex / cell / triangle / edge
+    Structconst consts;                          // its own id ranges
+};
+
 }  // namespace
 
 struct gc_engine {
+    int nrep = 1;
+    std::vector<GcRep> hreps;
+    std::vector<RepMeta> meta;
     int N = 0, nm = 0, ncell = 0, ntri = 0, nedge = 0, K = 0;
-    int ncube[3]; float outer[3], invw[3], bbox[6];
-    Structconst consts;
+    Structconst consts;        // global type thresholds (physics of replica 0)
     gc_ext_consts ext;
     long step = 0;
     hipStream_t s = nullptr;
     int cur = 0;
     std::vector<void*> allocs;
 
-    float *d_bbox, *d_outer, *d_invw; int* d_ncube;
+    GcRep* d_reps = nullptr; int* d_rep_of = nullptr; int* d_cell_rep = nullptr;
+    float* d_bbox = nullptr;   // box of replica 0 (one suspension: the box)
     double* d_pos64[2]; tint3* d_nt[2]; int* d_sp[2];
     float4 *d_posid, *d_velrho, *d_vout;
     tfloat3 *d_coor, *d_velhalf, *d_force;
@@ -464,6 +518,7 @@ struct gc_engine {
     hipGraphExec_t gexec = nullptr;
     int g_driveon = -1;
     bool walls_dirty = false;
+    bool any_morse = false, any_walls = false, any_bound1 = false;
 
     template <class T> int alloc(T** p, size_t n)
     {
@@ -471,45 +526,39 @@ struct gc_engine {
         if (!st) allocs.push_back((void*)*p);
         return st;
     }
+    bool ens() const { return nrep > 1; }
 };
 
 namespace {
 
-GcParams engine_params(const gc_engine* e)
-{
-    GcParams P;
-    make_params(e->consts, e->ext, &P);
-    return P;
-}
-
-Grid2 engine_grid(const gc_engine* e)
-{
-    Grid2 g;
-    for (int d = 0; d < 3; ++d) { g.nc[d] = e->ncube[d]; g.L[d] = e->outer[d]; }
-    return g;
-}
-
 // Neighbour build: keys from the (optionally advanced) FP64 positions,
 // counting sort by (cube, id), gather (+ predictor).
-int launch_build(gc_engine* e, const GcParams& P, int move, int predict)
+template <bool ENS>
+int launch_build_t(gc_engine* e, int move, int predict)
 {
     const int N = e->N, c = e->cur, nx = 1 - c;
     hipStream_t s = e->s;
-    const int gb = blocks_for(N, BS);
-    k2_integrate<<<xcd_grid(gb), BS, 0, s>>>(N, P, move, e->d_pos64[c], e->d_vout, e->d_force, e->d_sp[c],
-        e->d_nt[c], e->d_bbox, e->d_invw, e->d_ncube, e->d_key0, e->d_rank, e->d_cnt);
+    const int gb = xcd_grid(blocks_for(N, BS));
+    const GcRep& R0 = e->hreps[0];
+    k2_integrate<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, e->d_pos64[c],
+        e->d_vout, e->d_force, e->d_sp[c], e->d_nt[c], e->d_key0, e->d_rank, e->d_cnt);
     GC_LAUNCH_CHECK("engine integrate");
     int st = exclusive_scan(e->d_cnt, e->d_offs, e->K, s, e->d_scanws);
     if (st) return st;
-    k2_scatter<<<xcd_grid(gb), BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[c], e->d_offs, e->d_tk, e->d_tv,
+    k2_scatter<<<gb, BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[c], e->d_offs, e->d_tk, e->d_tv,
                                  e->d_cnt);
     GC_LAUNCH_CHECK("engine scatter");
-    k2_sortgather<<<xcd_grid(gb), BS, 0, s>>>(N, P, predict, e->d_tk, e->d_tv, e->d_offs, e->d_curpos,
-        e->d_pos64[c], e->d_pos64[nx], e->d_vout, e->d_force, e->d_nt[c], e->d_nt[nx], e->d_sp[nx],
-        e->d_key, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf);
+    k2_sortgather<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, predict, e->d_tk, e->d_tv,
+        e->d_offs, e->d_curpos, e->d_pos64[c], e->d_pos64[nx], e->d_vout, e->d_force, e->d_nt[c],
+        e->d_nt[nx], e->d_sp[nx], e->d_key, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf);
     GC_LAUNCH_CHECK("engine sort/gather");
     e->cur = nx;
     return GC_OK;
+}
+
+int launch_build(gc_engine* e, int move, int predict)
+{
+    return e->ens() ? launch_build_t<true>(e, move, predict) : launch_build_t<false>(e, move, predict);
 }
 
 struct Prof {
@@ -526,35 +575,40 @@ void mark(Prof* pr, hipStream_t s, const char* name)
     pr->n++;
 }
 
-int launch_step(gc_engine* e, int driveon, Prof* pr)
+template <bool ENS>
+int launch_step_t(gc_engine* e, int driveon, Prof* pr)
 {
-    GcParams P = engine_params(e);
-    Grid2 g = engine_grid(e);
+    const GcRep& R0 = e->hreps[0];
     const int N = e->N;
     hipStream_t s = e->s;
-    const int gb = blocks_for(N, BS);
+    const int gb = xcd_grid(blocks_for(N, BS));
     mark(pr, s, "start");
-    int st = launch_build(e, P, 1, 1);
+    int st = launch_build_t<ENS>(e, 1, 1);
     if (st) return st;
     mark(pr, s, "neighbour build+predict");
     const int c = e->cur;
-    k2_density<<<xcd_grid(gb), BS, 0, s>>>(N, P, g, e->d_posid, e->d_velrho, e->d_offs, e->d_key);
+    k2_density<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid, e->d_velrho,
+                                      e->d_offs, e->d_key);
     GC_LAUNCH_CHECK("engine density");
     mark(pr, s, "density+wall velocity");
-    k2_force<<<xcd_grid(gb), BS, 0, s>>>(N, P, g, driveon, e->d_posid, e->d_velrho, e->d_offs, e->d_key,
-                               e->d_velhalf, e->d_force, e->d_vout, e->d_ctr);
+    k2_force<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, driveon, e->d_posid,
+                                    e->d_velrho, e->d_offs, e->d_key, e->d_velhalf, e->d_force,
+                                    e->d_vout, e->d_ctr);
     GC_LAUNCH_CHECK("engine force");
     k2_bump<<<1, 1, 0, s>>>(e->d_ctr);
     mark(pr, s, "sdpd force");
     if (e->ncell > 0) {
-        k_geo_cell<<<e->ncell, BS, 0, s>>>(P, e->step, e->d_bbox, e->d_curpos, e->d_tri, e->d_betri,
-            e->d_nt[c], e->d_coor, e->d_trinor, e->d_triarea, e->d_reftriarea, e->d_tricen,
-            e->d_refarea_para, e->d_refvol_para, e->d_area_para, e->d_vol_para, e->d_area, e->d_vol);
+        const GcParams& P = R0.P;
+        k_geo_cell<<<e->ncell, BS, 0, s>>>(P, e->step, ENS ? e->d_reps : nullptr,
+            ENS ? e->d_cell_rep : nullptr, e->d_bbox, e->d_curpos, e->d_tri,
+            e->d_betri, e->d_nt[c], e->d_coor, e->d_trinor, e->d_triarea, e->d_reftriarea,
+            e->d_tricen, e->d_refarea_para, e->d_refvol_para, e->d_area_para, e->d_vol_para,
+            e->d_area, e->d_vol);
         GC_LAUNCH_CHECK("engine geo");
         mark(pr, s, "membrane geometry");
-        if (P.De != 0.0f) {
-            k2_agg<<<xcd_grid(gb), BS, 0, s>>>(N, P, g, e->d_posid, e->d_offs, e->d_key, e->d_celltab,
-                                     e->d_force, e->d_aggforce);
+        if (e->any_morse) {
+            k2_agg<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid, e->d_offs,
+                                          e->d_key, e->d_celltab, e->d_force, e->d_aggforce);
             GC_LAUNCH_CHECK("engine morse");
         }
         mark(pr, s, "morse");
@@ -568,8 +622,9 @@ int launch_step(gc_engine* e, int driveon, Prof* pr)
         a.defforce = e->d_defforce; a.force = e->d_force;
         a.velhalf = e->d_velhalf; a.velrho = e->d_velrho; a.vout = e->d_vout;
         a.mpos = e->d_mpos; a.nm = e->nm;
-        k2_mgather<<<blocks_for(e->nm, BS), BS, 0, s>>>(e->nm, P, e->d_curpos, e->d_coor, e->d_nt[c],
-                                                        e->d_bbox, e->d_mpos);
+        if (ENS) { a.reps = e->d_reps; a.cell_rep = e->d_cell_rep; a.celltab = e->d_celltab; }
+        k2_mgather<<<blocks_for(e->nm, BS), BS, 0, s>>>(e->nm, R0, ENS ? e->d_reps : nullptr,
+            e->d_cell_rep, e->d_celltab, e->d_curpos, e->d_coor, e->d_nt[c], e->d_mpos);
         GC_LAUNCH_CHECK("engine membrane gather");
         // 64-thread groups: a few hundred RBCs give only ~Nm/256 groups of 256
         k_def_force<<<xcd_grid(blocks_for(e->nm, 64)), 64, 0, s>>>(P, e->nm, e->d_bbox, a);
@@ -579,9 +634,14 @@ int launch_step(gc_engine* e, int driveon, Prof* pr)
     // [18] CalBound2Vel after the corrector only feeds the output: the next
     // step overwrites the walls' velocity (k2_density) before anything reads
     // it.  It is computed when the state leaves the engine (finish_walls).
-    e->walls_dirty = P.nf > 0;
+    e->walls_dirty = e->any_walls;
     e->step += 1;
     return GC_OK;
+}
+
+int launch_step(gc_engine* e, int driveon, Prof* pr)
+{
+    return e->ens() ? launch_step_t<true>(e, driveon, pr) : launch_step_t<false>(e, driveon, pr);
 }
 
 // Wall velocities from the corrected flow (UpdatePar's final CalBound2Vel,
@@ -589,10 +649,13 @@ int launch_step(gc_engine* e, int driveon, Prof* pr)
 int finish_walls(gc_engine* e)
 {
     if (!e->walls_dirty) return GC_OK;
-    GcParams P = engine_params(e);
-    Grid2 g = engine_grid(e);
-    k2_wallvel<<<xcd_grid(blocks_for(e->N, BS)), BS, 0, e->s>>>(e->N, P, g, e->d_posid, e->d_vout, e->d_offs,
-                                                        e->d_key);
+    const int gb = xcd_grid(blocks_for(e->N, BS));
+    if (e->ens())
+        k2_wallvel<true><<<gb, BS, 0, e->s>>>(e->N, e->hreps[0], e->d_reps, e->d_rep_of, e->d_posid,
+                                             e->d_vout, e->d_offs, e->d_key);
+    else
+        k2_wallvel<false><<<gb, BS, 0, e->s>>>(e->N, e->hreps[0], e->d_reps, e->d_rep_of, e->d_posid,
+                                              e->d_vout, e->d_offs, e->d_key);
     GC_LAUNCH_CHECK("engine wall velocity");
     e->walls_dirty = false;
     return GC_OK;
@@ -629,63 +692,121 @@ int reorder_carried(gc_engine* e, const std::vector<float4>& vout_id,
     return hip_check(hipStreamSynchronize(e->s), "reorder sync2");
 }
 
-}  // namespace
+int validate_desc(const gc_case_desc* d)
+{
+    if (!d || !d->coor || d->nBound1Par < 0 || d->nBound2Par < 0 || d->nFluidPar < 0 ||
+        d->nCell < 0 || d->nCellPar < 0)
+        return set_error(GC_ERR_INVALID, "EngineCreate: bad descriptor");
+    if (d->nCell > 0 && (!d->tri || !d->edge || !d->celltab || !d->beginendtri ||
+                         !d->shearmod_para || !d->bendingmod_para || !d->hlarea_para ||
+                         !d->hgarea_para || !d->hvol_para || d->nAllTri <= 0 ||
+                         d->nAllEdge <= 0 || d->nCellPar <= 0))
+        return set_error(GC_ERR_INVALID, "EngineCreate: membrane arrays missing");
+    const Structconst& c = d->consts;
+    if (!(c.CutRadius > 0) || !(c.Mass > 0) || !(c.TimeStep > 0) || !(c.DenInit > 0))
+        return set_error(GC_ERR_INVALID, "EngineCreate: CutRadius, Mass, TimeStep, DenInit must be > 0");
+    if (5.0 * c.ShearVis / 3.0 - c.BulkVis < 0)
+        return set_error(GC_ERR_INVALID, "EngineCreate: need 5*ShearVis/3 >= BulkVis");
+    if (c.surEnergy != 0 && c.MaxCut > c.CutRadius)
+        return set_error(GC_ERR_GRID, "EngineCreate: MaxCut must not exceed CutRadius");
+    if (!(d->ext.wlc_ratio > 1.0f))
+        return set_error(GC_ERR_INVALID, "EngineCreate: wlc_ratio must be > 1");
+    for (int cc = 0; cc < d->nCell; ++cc)
+        if (d->beginendtri[cc].x < 0 || d->beginendtri[cc].y >= d->nAllTri ||
+            d->beginendtri[cc].x > d->beginendtri[cc].y)
+            return set_error(GC_ERR_TOPOLOGY, "EngineCreate: bad BeginEndTri");
+    for (int v = 0; v < d->nCellPar && d->nCell > 0; ++v)
+        if (d->celltab[v] < 0 || d->celltab[v] >= d->nCell)
+            return set_error(GC_ERR_TOPOLOGY, "EngineCreate: CellTab out of range");
+    return GC_OK;
+}
 
-extern "C" gc_engine* gc_EngineCreate(const gc_case_desc* d, int* status)
+gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
 {
     int dummy;
     if (!status) status = &dummy;
     *status = GC_OK;
-    if (!d || !d->coor || d->nBound1Par < 0 || d->nBound2Par < 0 || d->nFluidPar < 0 ||
-        d->nCell < 0 || d->nCellPar < 0) {
-        *status = set_error(GC_ERR_INVALID, "EngineCreate: bad descriptor");
-        return nullptr;
-    }
-    if (d->nCell > 0 && (!d->tri || !d->edge || !d->celltab || !d->beginendtri ||
-                         !d->shearmod_para || !d->bendingmod_para || !d->hlarea_para ||
-                         !d->hgarea_para || !d->hvol_para || d->nAllTri <= 0 ||
-                         d->nAllEdge <= 0 || d->nCellPar <= 0)) {
-        *status = set_error(GC_ERR_INVALID, "EngineCreate: membrane arrays missing");
-        return nullptr;
-    }
+    if (!descs || nrep < 1) { *status = set_error(GC_ERR_INVALID, "EngineCreate: no case"); return nullptr; }
+    for (int r = 0; r < nrep; ++r)
+        if ((*status = validate_desc(&descs[r]))) return nullptr;
+    for (int r = 1; r < nrep; ++r)
+        if (descs[r].ext.noise_counter != descs[0].ext.noise_counter) {
+            *status = set_error(GC_ERR_INVALID, "EnsembleCreate: replicas must share the noise counter");
+            return nullptr;
+        }
     gc_engine* e = new gc_engine();
     int st = GC_OK;
     auto fail = [&](int code) { *status = code; destroy(e); return (gc_engine*)nullptr; };
-
-    e->consts = d->consts;
-    e->ext = d->ext;
-    e->consts.nBound2Start = (unsigned)d->nBound1Par;
-    e->consts.nFluidStart = (unsigned)(d->nBound1Par + d->nBound2Par);
-    e->consts.nCellStart = e->consts.nFluidStart + (unsigned)d->nFluidPar;
-    e->consts.nCellPar = (unsigned)(d->nCell > 0 ? d->nCellPar : 0);
-    e->consts.nTotalPar = e->consts.nCellStart + e->consts.nCellPar;
-    if (!(e->consts.CutRadius > 0) || !(e->consts.Mass > 0) || !(e->consts.TimeStep > 0) ||
-        !(e->consts.DenInit > 0))
-        return fail(set_error(GC_ERR_INVALID, "EngineCreate: CutRadius, Mass, TimeStep, DenInit must be > 0"));
-    if (5.0 * e->consts.ShearVis / 3.0 - e->consts.BulkVis < 0)
-        return fail(set_error(GC_ERR_INVALID, "EngineCreate: need 5*ShearVis/3 >= BulkVis"));
-    if (e->consts.surEnergy != 0 && e->consts.MaxCut > e->consts.CutRadius)
-        return fail(set_error(GC_ERR_GRID, "EngineCreate: MaxCut must not exceed CutRadius"));
-    if (!(e->ext.wlc_ratio > 1.0f))
-        return fail(set_error(GC_ERR_INVALID, "EngineCreate: wlc_ratio must be > 1"));
-
-    e->N = (int)e->consts.nTotalPar;
-    e->nm = (int)e->consts.nCellPar;
-    e->ncell = d->nCell;
-    e->ntri = d->nCell > 0 ? d->nAllTri : 0;
-    e->nedge = d->nCell > 0 ? d->nAllEdge : 0;
-    memcpy(e->bbox, d->bbox, sizeof(e->bbox));
-    if ((st = gc_ConfigDomain(e->bbox, e->consts.CutRadius, e->ext.periodic, e->ncube, e->outer,
-                              e->invw, &e->K)))
-        return fail(st);
+    e->nrep = nrep;
+    e->meta.resize(nrep);
+    e->hreps.resize(nrep);
+    // global id layout and totals
+    long tb1 = 0, tb2 = 0, tfl = 0, tm = 0, tc = 0, tt = 0, te = 0;
+    for (int r = 0; r < nrep; ++r) {
+        const gc_case_desc& d = descs[r];
+        RepMeta& m = e->meta[r];
+        m.nb1 = d.nBound1Par; m.nb2 = d.nBound2Par; m.nfl = d.nFluidPar;
+        m.ncell = d.nCell; m.nm = d.nCell > 0 ? d.nCellPar : 0;
+        m.ntri = d.nCell > 0 ? d.nAllTri : 0; m.nedge = d.nCell > 0 ? d.nAllEdge : 0;
+        m.vtx_off = (int)tm; m.cell_off = (int)tc; m.tri_off = (int)tt; m.edge_off = (int)te;
+        tb1 += m.nb1; tb2 += m.nb2; tfl += m.nfl; tm += m.nm; tc += m.ncell; tt += m.ntri; te += m.nedge;
+        Structconst c = d.consts;
+        c.nBound2Start = (unsigned)m.nb1;
+        c.nFluidStart = (unsigned)(m.nb1 + m.nb2);
+        c.nCellStart = c.nFluidStart + (unsigned)m.nfl;
+        c.nCellPar = (unsigned)m.nm;
+        c.nTotalPar = c.nCellStart + c.nCellPar;
+        m.consts = c;
+    }
+    const long Ntot = tb1 + tb2 + tfl + tm;
+    if (Ntot >= (1L << 25))  // the force kernel packs slot << 6 into an int
+        return fail(set_error(GC_ERR_INVALID, "EngineCreate: more than 2^25 particles"));
+    e->N = (int)Ntot; e->nm = (int)tm; e->ncell = (int)tc; e->ntri = (int)tt; e->nedge = (int)te;
+    e->consts = e->meta[0].consts;
+    e->consts.nBound2Start = (unsigned)tb1;
+    e->consts.nFluidStart = (unsigned)(tb1 + tb2);
+    e->consts.nCellStart = (unsigned)(tb1 + tb2 + tfl);
+    e->consts.nCellPar = (unsigned)tm;
+    e->consts.nTotalPar = (unsigned)Ntot;
+    e->ext = descs[0].ext;
+    // per-replica tables
+    long koff = 0;
+    long b1o = 0, b2o = tb1, flo = tb1 + tb2, mo = tb1 + tb2 + tfl;
+    for (int r = 0; r < nrep; ++r) {
+        const gc_case_desc& d = descs[r];
+        RepMeta& m = e->meta[r];
+        GcRep& R = e->hreps[r];
+        memset(&R, 0, sizeof(R));
+        Structconst cg = d.consts;  // the replica's physics with the global thresholds
+        cg.nBound2Start = e->consts.nBound2Start; cg.nFluidStart = e->consts.nFluidStart;
+        cg.nCellStart = e->consts.nCellStart; cg.nCellPar = e->consts.nCellPar;
+        cg.nTotalPar = e->consts.nTotalPar;
+        make_params(cg, d.ext, &R.P);
+        int Kr = 0;
+        float outer[3];
+        if ((st = gc_ConfigDomain(d.bbox, d.consts.CutRadius, d.ext.periodic, R.nc, outer, R.invw, &Kr)))
+            return fail(st);
+        for (int q = 0; q < 3; ++q) R.L[q] = outer[q];
+        memcpy(R.bbox, d.bbox, sizeof(R.bbox));
+        R.cube_off = (int)koff;
+        koff += Kr;
+        R.id_off[0] = (int)b1o; R.id_off[1] = (int)b2o; R.id_off[2] = (int)flo; R.id_off[3] = (int)mo;
+        R.loc_start[0] = 0; R.loc_start[1] = m.nb1; R.loc_start[2] = m.nb1 + m.nb2;
+        R.loc_start[3] = m.nb1 + m.nb2 + m.nfl;
+        b1o += m.nb1; b2o += m.nb2; flo += m.nfl; mo += m.nm;
+        e->any_morse = e->any_morse || (d.consts.surEnergy != 0.0f && m.ncell > 0);
+        e->any_walls = e->any_walls || (m.nb1 + m.nb2 > 0);
+        e->any_bound1 = e->any_bound1 || m.nb1 > 0;
+    }
+    if (koff > (1L << 30)) return fail(set_error(GC_ERR_GRID, "EngineCreate: too many cubes"));
+    e->K = (int)koff;
     if ((st = hip_check(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking), "engine stream")))
         return fail(st);
 
     const size_t N = (size_t)e->N, K = (size_t)e->K;
-    if (e->N >= (1 << 25))  // the force kernel packs slot << 6 into an int
-        return fail(set_error(GC_ERR_INVALID, "EngineCreate: more than 2^25 particles"));
-    if ((st = e->alloc(&e->d_bbox, 6)) || (st = e->alloc(&e->d_outer, 3)) ||
-        (st = e->alloc(&e->d_invw, 3)) || (st = e->alloc(&e->d_ncube, 3)))
+    if ((st = e->alloc(&e->d_reps, (size_t)nrep)) || (st = e->alloc(&e->d_rep_of, N)) ||
+        (st = e->alloc(&e->d_cell_rep, (size_t)(e->ncell > 0 ? e->ncell : 1))) ||
+        (st = e->alloc(&e->d_bbox, 6)))
         return fail(st);
     for (int b = 0; b < 2; ++b)
         if ((st = e->alloc(&e->d_pos64[b], 3 * N)) || (st = e->alloc(&e->d_nt[b], N)) ||
@@ -700,7 +821,6 @@ extern "C" gc_engine* gc_EngineCreate(const gc_case_desc* d, int* status)
         (st = e->alloc(&e->d_tv, N)) || (st = e->alloc(&e->d_curpos, N)) ||
         (st = e->alloc(&e->d_scanws, scan_ws_ints(e->K))) || (st = e->alloc(&e->d_ctr, 1)))
         return fail(st);
-
     hipStream_t s = e->s;
     std::vector<float4> vout_id(N);
     std::vector<tfloat3> vh_id(N), f_id(N, tfloat3{0.f, 0.f, 0.f});
@@ -708,18 +828,29 @@ extern "C" gc_engine* gc_EngineCreate(const gc_case_desc* d, int* status)
         // Particle::SetParticle (Particle.cu:485-546): rho = DenInit, F = 0
         std::vector<double> pos(3 * N);
         std::vector<tint3> nt(N, tint3{0, 0, 0});
-        std::vector<int> idn(N), zero(K + 1, 0);
-        for (size_t i = 0; i < N; ++i) {
-            pos[i] = d->coor[i].x; pos[N + i] = d->coor[i].y; pos[2 * N + i] = d->coor[i].z;
-            tfloat3 v = d->vel ? d->vel[i] : tfloat3{0.f, 0.f, 0.f};
-            if ((int)i < (int)e->consts.nBound2Start) v = tfloat3{0.f, 0.f, 0.f};
-            vout_id[i] = make_float4(v.x, v.y, v.z, e->consts.DenInit);
-            vh_id[i] = v;
-            idn[i] = (int)i;
+        std::vector<int> idn(N), zero(K + 1, 0), rep_of(N);
+        for (int r = 0; r < nrep; ++r) {
+            const gc_case_desc& d = descs[r];
+            const RepMeta& m = e->meta[r];
+            const GcRep& R = e->hreps[r];
+            const int lstart[5] = {0, m.nb1, m.nb1 + m.nb2, m.nb1 + m.nb2 + m.nfl,
+                                   m.nb1 + m.nb2 + m.nfl + m.nm};
+            for (int t = 0; t < 4; ++t)
+                for (int l = lstart[t]; l < lstart[t + 1]; ++l) {
+                    size_t g = (size_t)(R.id_off[t] + (l - lstart[t]));
+                    pos[g] = d.coor[l].x; pos[N + g] = d.coor[l].y; pos[2 * N + g] = d.coor[l].z;
+                    tfloat3 v = d.vel ? d.vel[l] : tfloat3{0.f, 0.f, 0.f};
+                    if (t == 0) v = tfloat3{0.f, 0.f, 0.f};
+                    vout_id[g] = make_float4(v.x, v.y, v.z, d.consts.DenInit);
+                    vh_id[g] = v;
+                    rep_of[g] = r;
+                }
         }
+        for (size_t i = 0; i < N; ++i) idn[i] = (int)i;
         unsigned long long ctr = e->ext.noise_counter;
-        if ((st = upload(e->d_bbox, e->bbox, 6, s)) || (st = upload(e->d_outer, e->outer, 3, s)) ||
-            (st = upload(e->d_invw, e->invw, 3, s)) || (st = upload(e->d_ncube, e->ncube, 3, s)) ||
+        if ((st = upload(e->d_reps, e->hreps.data(), (size_t)nrep, s)) ||
+            (st = upload(e->d_bbox, e->hreps[0].bbox, 6, s)) ||
+            (st = upload(e->d_rep_of, rep_of.data(), N, s)) ||
             (st = upload(e->d_pos64[0], pos.data(), 3 * N, s)) ||
             (st = upload(e->d_vout, vout_id.data(), N, s)) || (st = upload(e->d_force, f_id.data(), N, s)) ||
             (st = upload(e->d_nt[0], nt.data(), N, s)) || (st = upload(e->d_sp[0], idn.data(), N, s)) ||
@@ -727,9 +858,8 @@ extern "C" gc_engine* gc_EngineCreate(const gc_case_desc* d, int* status)
             (st = upload(e->d_ctr, &ctr, 1, s)))
             return fail(st);
     }
-    GcParams P = engine_params(e);
     // first neighbour build (Particle::ConCubeList at init, Particle.cu:570)
-    if ((st = launch_build(e, P, 0, 0))) return fail(st);
+    if ((st = launch_build(e, 0, 0))) return fail(st);
     if ((st = reorder_carried(e, vout_id, f_id, vh_id))) return fail(st);
 
     if (e->ncell > 0) {
@@ -749,43 +879,81 @@ extern "C" gc_engine* gc_EngineCreate(const gc_case_desc* d, int* status)
             (st = e->alloc(&e->d_defforce, M)) || (st = e->alloc(&e->d_aggforce, M)) ||
             (st = e->alloc(&e->d_mpos, 3 * M)))
             return fail(st);
-        std::vector<int> es, ee, ts, te;
-        if ((st = build_incidence(e->nm, e->nedge, d->edge, e->ntri, d->tri, es, ee, ts, te)))
+        // global membrane arrays: 1-based global vertex ids, global triangle ids
+        std::vector<tint3> tri(T);
+        std::vector<int6> edge(E);
+        std::vector<int> celltab(M), cell_rep(C);
+        std::vector<tint2> betri(C);
+        std::vector<float> mu(E), kc(E), kd(T), ka(T), kv(T);
+        for (int r = 0; r < nrep; ++r) {
+            const gc_case_desc& d = descs[r];
+            const RepMeta& m = e->meta[r];
+            const int vo = m.vtx_off, to = m.tri_off;
+            for (int t = 0; t < m.ntri; ++t) {
+                tri[to + t] = tint3{d.tri[t].x + vo, d.tri[t].y + vo, d.tri[t].z + vo};
+                kd[to + t] = d.hlarea_para[t]; ka[to + t] = d.hgarea_para[t]; kv[to + t] = d.hvol_para[t];
+            }
+            for (int q = 0; q < m.nedge; ++q) {
+                const int6& E0 = d.edge[q];
+                edge[m.edge_off + q] = int6{E0.x + vo, E0.y + vo, E0.z + vo, E0.k + vo, E0.v + to, E0.w + to};
+                mu[m.edge_off + q] = d.shearmod_para[q]; kc[m.edge_off + q] = d.bendingmod_para[q];
+            }
+            for (int v = 0; v < m.nm; ++v) celltab[vo + v] = d.celltab[v] + m.cell_off;
+            for (int cc = 0; cc < m.ncell; ++cc) {
+                betri[m.cell_off + cc] = tint2{d.beginendtri[cc].x + to, d.beginendtri[cc].y + to};
+                cell_rep[m.cell_off + cc] = r;
+            }
+        }
+        std::vector<int> es, ee, ts, te2;
+        if ((st = build_incidence(e->nm, e->nedge, edge.data(), e->ntri, tri.data(), es, ee, ts, te2)))
             return fail(st);
-        for (int c = 0; c < e->ncell; ++c)
-            if (d->beginendtri[c].x < 0 || d->beginendtri[c].y >= e->ntri ||
-                d->beginendtri[c].x > d->beginendtri[c].y)
-                return fail(set_error(GC_ERR_TOPOLOGY, "EngineCreate: bad BeginEndTri"));
-        for (int v = 0; v < e->nm; ++v)
-            if (d->celltab[v] < 0 || d->celltab[v] >= e->ncell)
-                return fail(set_error(GC_ERR_TOPOLOGY, "EngineCreate: CellTab out of range"));
         if ((st = e->alloc(&e->d_es, es.size())) || (st = e->alloc(&e->d_ee, ee.size())) ||
-            (st = e->alloc(&e->d_ts, ts.size())) || (st = e->alloc(&e->d_te, te.size())))
+            (st = e->alloc(&e->d_ts, ts.size())) || (st = e->alloc(&e->d_te, te2.size())))
             return fail(st);
         std::vector<tfloat3> zero(M, tfloat3{0.f, 0.f, 0.f});
-        if ((st = upload(e->d_tri, d->tri, T, s)) || (st = upload(e->d_edge, d->edge, E, s)) ||
-            (st = upload(e->d_celltab, d->celltab, M, s)) || (st = upload(e->d_betri, d->beginendtri, C, s)) ||
-            (st = upload(e->d_mu0, d->shearmod_para, E, s)) || (st = upload(e->d_kc, d->bendingmod_para, E, s)) ||
-            (st = upload(e->d_kd, d->hlarea_para, T, s)) || (st = upload(e->d_ka, d->hgarea_para, T, s)) ||
-            (st = upload(e->d_kv, d->hvol_para, T, s)) || (st = upload(e->d_es, es.data(), es.size(), s)) ||
+        if ((st = upload(e->d_tri, tri.data(), T, s)) || (st = upload(e->d_edge, edge.data(), E, s)) ||
+            (st = upload(e->d_celltab, celltab.data(), M, s)) || (st = upload(e->d_betri, betri.data(), C, s)) ||
+            (st = upload(e->d_cell_rep, cell_rep.data(), C, s)) ||
+            (st = upload(e->d_mu0, mu.data(), E, s)) || (st = upload(e->d_kc, kc.data(), E, s)) ||
+            (st = upload(e->d_kd, kd.data(), T, s)) || (st = upload(e->d_ka, ka.data(), T, s)) ||
+            (st = upload(e->d_kv, kv.data(), T, s)) || (st = upload(e->d_es, es.data(), es.size(), s)) ||
             (st = upload(e->d_ee, ee.data(), ee.size(), s)) || (st = upload(e->d_ts, ts.data(), ts.size(), s)) ||
-            (st = upload(e->d_te, te.data(), te.size(), s)) || (st = upload(e->d_defforce, zero.data(), M, s)) ||
+            (st = upload(e->d_te, te2.data(), te2.size(), s)) || (st = upload(e->d_defforce, zero.data(), M, s)) ||
             (st = upload(e->d_aggforce, zero.data(), M, s)))
             return fail(st);
         const int c = e->cur;
         // reference geometry and mechanics (Particle::SetPar, Particle.cu:476-479)
-        k_geo_cell<<<e->ncell, BS, 0, s>>>(P, -1L, e->d_bbox, e->d_curpos, e->d_tri, e->d_betri,
-            e->d_nt[c], e->d_coor, e->d_trinor, e->d_triarea, e->d_reftriarea, e->d_tricen,
-            e->d_refarea_para, e->d_refvol_para, e->d_area_para, e->d_vol_para, e->d_area, e->d_vol);
+        k_geo_cell<<<e->ncell, BS, 0, s>>>(e->hreps[0].P, -1L, e->ens() ? e->d_reps : nullptr,
+            e->ens() ? e->d_cell_rep : nullptr, e->d_bbox, e->d_curpos, e->d_tri,
+            e->d_betri, e->d_nt[c], e->d_coor, e->d_trinor, e->d_triarea, e->d_reftriarea,
+            e->d_tricen, e->d_refarea_para, e->d_refvol_para, e->d_area_para, e->d_vol_para,
+            e->d_area, e->d_vol);
         k_par_area<<<blocks_for(e->nm, BS), BS, 0, s>>>(e->nm, e->d_ts, e->d_te, e->d_triarea, e->d_pararea);
-        k_mech_para<<<blocks_for(e->nedge, BS), BS, 0, s>>>(P, e->nedge, e->d_edge, e->d_curpos,
-            e->d_coor, e->d_l0, e->d_lmax, e->d_kc, e->d_mu0, e->d_kb, e->d_mexp, e->d_p, e->d_kp,
-            e->d_th0);
+        for (int r = 0; r < nrep; ++r) {  // per replica: its WLC-POW constants and kT
+            const RepMeta& m = e->meta[r];
+            if (m.nedge == 0) continue;
+            const int o = m.edge_off;
+            k_mech_para<<<blocks_for(m.nedge, BS), BS, 0, s>>>(e->hreps[r].P, m.nedge, e->d_edge + o,
+                e->d_curpos, e->d_coor, e->d_l0 + o, e->d_lmax + o, e->d_kc + o, e->d_mu0 + o,
+                e->d_kb + o, e->d_mexp + o, e->d_p + o, e->d_kp + o, e->d_th0 + o);
+        }
         if ((st = hip_check(hipGetLastError(), "engine membrane init"))) return fail(st);
     }
     if ((st = hip_check(hipStreamSynchronize(s), "engine init sync"))) return fail(st);
     e->step = 0;
     return e;
+}
+
+}  // namespace
+
+extern "C" gc_engine* gc_EngineCreate(const gc_case_desc* d, int* status)
+{
+    return create(d, 1, status);
+}
+
+extern "C" gc_engine* gc_EnsembleCreate(const gc_case_desc* descs, int nrep, int* status)
+{
+    return create(descs, nrep, status);
 }
 
 extern "C" void gc_EngineDestroy(gc_engine* e) { destroy(e); }
@@ -834,6 +1002,7 @@ extern "C" int gc_EngineStepGraph(gc_engine* e, int nsteps, int driveon)
         if (st) return st;
         e->step += 2;
     }
+    e->walls_dirty = e->any_walls;
     if (nsteps & 1) return launch_step(e, driveon, nullptr);
     return GC_OK;
 }
@@ -851,19 +1020,29 @@ extern "C" int gc_EngineInfo(const gc_engine* e, gc_engine_info* info)
     info->noise_counter = c;
     info->nTotalPar = e->N; info->nCellPar = e->nm; info->nCell = e->ncell; info->sizecube = e->K;
     for (int d = 0; d < 3; ++d) {
-        info->ncube[d] = e->ncube[d]; info->outerregion[d] = e->outer[d]; info->invwid[d] = e->invw[d];
+        info->ncube[d] = e->hreps[0].nc[d]; info->outerregion[d] = e->hreps[0].L[d];
+        info->invwid[d] = e->hreps[0].invw[d];
     }
     return GC_OK;
 }
 
-extern "C" int gc_EngineGetState(gc_engine* e, gc_host_state* o)
+namespace {
+
+// The whole state in global id order.
+struct Snapshot {
+    std::vector<int> sp;
+    std::vector<double> pos;
+    std::vector<float4> vo;
+    std::vector<tfloat3> vh, f, co;
+    std::vector<tint3> nt;
+};
+
+int snapshot(gc_engine* e, Snapshot& S)
 {
-    if (!e || !o) return set_error(GC_ERR_INVALID, "EngineGetState: null");
     const int N = e->N, c = e->cur;
     int st = finish_walls(e);
     if (st) return st;
-    st = hip_check(hipStreamSynchronize(e->s), "GetState sync");
-    if (st) return st;
+    if ((st = hip_check(hipStreamSynchronize(e->s), "GetState sync"))) return st;
     std::vector<int> sp(N);
     std::vector<double> pos(3 * (size_t)N);
     std::vector<float4> vo(N);
@@ -880,55 +1059,100 @@ extern "C" int gc_EngineGetState(gc_engine* e, gc_host_state* o)
         (st = dl(co.data(), e->d_coor, sizeof(tfloat3) * N)) ||
         (st = dl(nt.data(), e->d_nt[c], sizeof(tint3) * N)))
         return st;
-    const float c1sq = e->consts.SoundSpeed1 * e->consts.SoundSpeed1;
-    const float c2sq = e->consts.SoundSpeed2 * e->consts.SoundSpeed2;
+    S.sp = sp;
+    S.pos.assign(3 * (size_t)N, 0.0);
+    S.vo.resize(N); S.vh.resize(N); S.f.resize(N); S.co.resize(N); S.nt.resize(N);
     for (int slot = 0; slot < N; ++slot) {
         int id = sp[slot];
-        if (o->pos) for (int d = 0; d < 3; ++d) o->pos[3 * (size_t)id + d] = pos[(size_t)d * N + slot];
-        if (o->coor) o->coor[id] = co[slot];
-        if (o->velrhop) o->velrhop[id] = tfloat4{vo[slot].x, vo[slot].y, vo[slot].z, vo[slot].w};
-        if (o->velhalf) o->velhalf[id] = vh[slot];
-        if (o->force) o->force[id] = f[slot];
-        if (o->press) o->press[id] = (id < (int)e->consts.nCellStart ? c1sq : c2sq) * vo[slot].w;
-        if (o->nthrobound) o->nthrobound[id] = nt[slot];
-    }
-    if (e->ncell > 0) {
-        if (o->area && (st = dl(o->area, e->d_area, sizeof(float) * e->ncell))) return st;
-        if (o->vol && (st = dl(o->vol, e->d_vol, sizeof(float) * e->ncell))) return st;
-        if (o->defforce && (st = dl(o->defforce, e->d_defforce, sizeof(tfloat3) * e->nm))) return st;
-        if (o->aggforce && (st = dl(o->aggforce, e->d_aggforce, sizeof(tfloat3) * e->nm))) return st;
+        for (int d = 0; d < 3; ++d) S.pos[3 * (size_t)id + d] = pos[(size_t)d * N + slot];
+        S.vo[id] = vo[slot]; S.vh[id] = vh[slot]; S.f[id] = f[slot]; S.co[id] = co[slot];
+        S.nt[id] = nt[slot];
     }
     return GC_OK;
+}
+
+// global id of replica r's own id l
+int global_id(const gc_engine* e, int r, int l)
+{
+    const RepMeta& m = e->meta[r];
+    const GcRep& R = e->hreps[r];
+    int t = l < m.nb1 ? 0 : (l < m.nb1 + m.nb2 ? 1 : (l < m.nb1 + m.nb2 + m.nfl ? 2 : 3));
+    return R.id_off[t] + (l - R.loc_start[t]);
+}
+
+int export_state(gc_engine* e, int r, gc_host_state* o)
+{
+    Snapshot S;
+    int st = snapshot(e, S);
+    if (st) return st;
+    const RepMeta& m = e->meta[r];
+    const int n = (int)m.consts.nTotalPar;
+    const float c1sq = m.consts.SoundSpeed1 * m.consts.SoundSpeed1;
+    const float c2sq = m.consts.SoundSpeed2 * m.consts.SoundSpeed2;
+    for (int l = 0; l < n; ++l) {
+        int g = global_id(e, r, l);
+        if (o->pos) for (int d = 0; d < 3; ++d) o->pos[3 * (size_t)l + d] = S.pos[3 * (size_t)g + d];
+        if (o->coor) o->coor[l] = S.co[g];
+        if (o->velrhop) o->velrhop[l] = tfloat4{S.vo[g].x, S.vo[g].y, S.vo[g].z, S.vo[g].w};
+        if (o->velhalf) o->velhalf[l] = S.vh[g];
+        if (o->force) o->force[l] = S.f[g];
+        if (o->press) o->press[l] = (l < (int)m.consts.nCellStart ? c1sq : c2sq) * S.vo[g].w;
+        if (o->nthrobound) o->nthrobound[l] = S.nt[g];
+    }
+    if (m.ncell > 0) {
+        auto dl = [&](void* h, const void* dv, size_t b) {
+            return hip_check(hipMemcpy(h, dv, b, hipMemcpyDeviceToHost), "GetState D2H");
+        };
+        if (o->area && (st = dl(o->area, e->d_area + m.cell_off, sizeof(float) * m.ncell))) return st;
+        if (o->vol && (st = dl(o->vol, e->d_vol + m.cell_off, sizeof(float) * m.ncell))) return st;
+        if (o->defforce && (st = dl(o->defforce, e->d_defforce + m.vtx_off, sizeof(tfloat3) * m.nm))) return st;
+        if (o->aggforce && (st = dl(o->aggforce, e->d_aggforce + m.vtx_off, sizeof(tfloat3) * m.nm))) return st;
+    }
+    return GC_OK;
+}
+
+}  // namespace
+
+extern "C" int gc_EngineGetState(gc_engine* e, gc_host_state* o)
+{
+    if (!e || !o) return set_error(GC_ERR_INVALID, "EngineGetState: null");
+    if (e->nrep != 1) return set_error(GC_ERR_INVALID, "EngineGetState: batched engine (use EnsembleGetState)");
+    return export_state(e, 0, o);
+}
+
+extern "C" int gc_EnsembleGetState(gc_engine* e, int rep, gc_host_state* o)
+{
+    if (!e || !o || rep < 0 || rep >= e->nrep) return set_error(GC_ERR_INVALID, "EnsembleGetState: bad arguments");
+    return export_state(e, rep, o);
 }
 
 extern "C" int gc_EngineSetState(gc_engine* e, const gc_host_state* in, long step,
                                  uint64_t noise_counter)
 {
     if (!e || !in) return set_error(GC_ERR_INVALID, "EngineSetState: null");
+    if (e->nrep != 1) return set_error(GC_ERR_INVALID, "EngineSetState: batched engine");
     const int N = e->N;
     int st = hip_check(hipStreamSynchronize(e->s), "SetState sync");
     if (st) return st;
     if (e->gexec) { (void)hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
-    std::vector<double> pos(3 * (size_t)N);
-    std::vector<tfloat4> vr(N);
-    std::vector<tfloat3> vh(N), f(N);
-    std::vector<tint3> nt(N);
-    gc_host_state cur;
-    memset(&cur, 0, sizeof(cur));
-    cur.pos = pos.data(); cur.velrhop = vr.data(); cur.velhalf = vh.data(); cur.force = f.data();
-    cur.nthrobound = nt.data();
-    if ((st = gc_EngineGetState(e, &cur))) return st;
+    Snapshot S;
+    if ((st = snapshot(e, S))) return st;
+    std::vector<double> pos = S.pos;
     if (in->pos) memcpy(pos.data(), in->pos, sizeof(double) * 3 * N);
-    std::vector<double> pos_soa(3 * (size_t)N);
-    for (int i = 0; i < N; ++i)
-        for (int d = 0; d < 3; ++d) pos_soa[(size_t)d * N + i] = pos[3 * (size_t)i + d];
-    if (in->velrhop) memcpy(vr.data(), in->velrhop, sizeof(tfloat4) * N);
+    std::vector<float4> vo = S.vo;
+    if (in->velrhop)
+        for (int i = 0; i < N; ++i)
+            vo[i] = make_float4(in->velrhop[i].x, in->velrhop[i].y, in->velrhop[i].z, in->velrhop[i].w);
+    std::vector<tfloat3> vh = S.vh, f = S.f;
+    std::vector<tint3> nt = S.nt;
     if (in->velhalf) memcpy(vh.data(), in->velhalf, sizeof(tfloat3) * N);
     if (in->force) memcpy(f.data(), in->force, sizeof(tfloat3) * N);
     if (in->nthrobound) memcpy(nt.data(), in->nthrobound, sizeof(tint3) * N);
+    std::vector<double> pos_soa(3 * (size_t)N);
+    for (int i = 0; i < N; ++i)
+        for (int d = 0; d < 3; ++d) pos_soa[(size_t)d * N + i] = pos[3 * (size_t)i + d];
     std::vector<int> idn(N);
-    std::vector<float4> vo(N);
-    for (int i = 0; i < N; ++i) { idn[i] = i; vo[i] = make_float4(vr[i].x, vr[i].y, vr[i].z, vr[i].w); }
+    for (int i = 0; i < N; ++i) idn[i] = i;
     e->cur = 0;
     hipStream_t s = e->s;
     unsigned long long ctr = noise_counter;
@@ -937,9 +1161,9 @@ extern "C" int gc_EngineSetState(gc_engine* e, const gc_host_state* in, long ste
         (st = upload(e->d_sp[0], idn.data(), N, s)) || (st = upload(e->d_curpos, idn.data(), N, s)) ||
         (st = upload(e->d_ctr, &ctr, 1, s)))
         return st;
-    GcParams P = engine_params(e);
-    if ((st = launch_build(e, P, 0, 0))) return st;
+    if ((st = launch_build(e, 0, 0))) return st;
     if ((st = reorder_carried(e, vo, f, vh))) return st;
+    e->walls_dirty = false;
     e->step = step;
     return GC_OK;
 }
@@ -961,9 +1185,11 @@ extern "C" int gc_EngineGetCells(gc_engine* e, gc_host_cells* o)
         return st;
     if (o->cubepart) memcpy(o->cubepart, key.data(), sizeof(int) * N);
     if (o->cc)
-        for (int s = 0; s < N; ++s) {
-            int k = key[s];
-            o->cc[s] = tint3{k % e->ncube[0], (k / e->ncube[0]) % e->ncube[1], k / (e->ncube[0] * e->ncube[1])};
+        for (int s = 0, r = 0; s < N; ++s) {  // slots of a replica are contiguous
+            while (r + 1 < e->nrep && key[s] >= e->hreps[r + 1].cube_off) ++r;
+            const GcRep& R = e->hreps[r];
+            int k = key[s] - R.cube_off;
+            o->cc[s] = tint3{k % R.nc[0], (k / R.nc[0]) % R.nc[1], k / (R.nc[0] * R.nc[1])};
         }
     if (o->beginendcube)
         for (int k = 0; k < K; ++k) o->beginendcube[k] = tint2{offs[k], offs[k + 1]};
@@ -1026,3 +1252,5 @@ extern "C" int gc_EngineRanges(const gc_engine* e, int* nbound1, int* nbound12)
     *nbound12 = (int)e->consts.nFluidStart;
     return GC_OK;
 }
+
+extern "C" int gc_EnsembleCount(const gc_engine* e) { return e ? e->nrep : 0; }

@@ -381,6 +381,7 @@ extern "C" int gc_EngineWriteTecplot(gc_engine* e, const char* dir_c, long stepd
                                      const tint2* beginendtri, const int* celltab)
 {
     if (!e || !dir_c) return gc::set_error(GC_ERR_INVALID, "WriteTecplot: null argument");
+    if (gc_EnsembleCount(e) != 1) return gc::set_error(GC_ERR_INVALID, "WriteTecplot: batched engine");
     gc_engine_info info;
     int st = gc_EngineInfo(e, &info);
     if (st) return st;
@@ -586,6 +587,7 @@ struct CkHeader {
 extern "C" int gc_EngineSave(gc_engine* e, const char* path)
 {
     if (!e || !path) return gc::set_error(GC_ERR_INVALID, "EngineSave: null argument");
+    if (gc_EnsembleCount(e) != 1) return gc::set_error(GC_ERR_INVALID, "EngineSave: batched engine");
     gc_engine_info info;
     int st = gc_EngineInfo(e, &info);
     if (st) return st;
@@ -621,6 +623,7 @@ extern "C" int gc_EngineSave(gc_engine* e, const char* path)
 extern "C" int gc_EngineLoad(gc_engine* e, const char* path)
 {
     if (!e || !path) return gc::set_error(GC_ERR_INVALID, "EngineLoad: null argument");
+    if (gc_EnsembleCount(e) != 1) return gc::set_error(GC_ERR_INVALID, "EngineLoad: batched engine");
     gc_engine_info info;
     int st = gc_EngineInfo(e, &info);
     if (st) return st;

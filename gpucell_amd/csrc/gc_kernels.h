@@ -8,6 +8,33 @@
 
 namespace gc {
 
+// One suspension of a batched run (and the single suspension of an engine).
+// Replicas are concatenated with the global id layout
+//   [Bound1 of all | Bound2 of all | fluid of all | membrane of all],
+// so P's type thresholds (nb2, nf, nc, N) are the global ones; the physics
+// fields of P are the replica's own.  Each replica owns the global cube keys
+// [cube_off, cube_off + nc0 nc1 nc2).
+struct GcRep {
+    GcParams P;
+    int nc[3];          // cube grid
+    float L[3];         // OuterRegion (period of periodic axes)
+    float bbox[6];
+    float invw[3];
+    int cube_off;
+    int id_off[4];      // first global id of the replica's Bound1, Bound2, fluid, membrane
+    int loc_start[4];   // the same blocks' first ids in the replica's own numbering
+};
+
+// the replica's own id of global id `id` (used for the pair noise, so a
+// replica draws the same numbers as when it runs alone).  Selects on
+// constant indices: a runtime index into the struct would put it in scratch.
+__device__ __forceinline__ int local_id(const GcRep& R, int id)
+{
+    int d0 = R.id_off[0] - R.loc_start[0], d1 = R.id_off[1] - R.loc_start[1];
+    int d2 = R.id_off[2] - R.loc_start[2], d3 = R.id_off[3] - R.loc_start[3];
+    return id - (id < R.P.nb2 ? d0 : (id < R.P.nf ? d1 : (id < R.P.nc ? d2 : d3)));
+}
+
 // gc_sdpd.hip
 __global__ void k_density(int n, GcParams P, const tfloat3* __restrict__ coor,
                           tfloat4* __restrict__ velrhop, const tint3* __restrict__ cc,
@@ -49,9 +76,14 @@ struct DefArgs {
     // engine only: unwrapped vertex positions in id order, x[nm], y[nm],
     // z[nm] (FP64); replaces the coor/nthro/curpos gathers of the vertices
     const double* mpos = nullptr; int nm = 0;
+    // batched runs: per-vertex replica (physics of the vertex's suspension)
+    const GcRep* reps = nullptr; const int* cell_rep = nullptr; const int* celltab = nullptr;
 };
 
-__global__ void k_geo_cell(GcParams P, long stepcount, const float* __restrict__ bbox,
+// reps/cell_rep: batched runs (NULL for one suspension) -- the cell's
+// replica supplies the parameter block and the box.
+__global__ void k_geo_cell(GcParams P, long stepcount, const GcRep* __restrict__ reps,
+                           const int* __restrict__ cell_rep, const float* __restrict__ bbox,
                            const int* __restrict__ curpos, const tint3* __restrict__ tri,
                            const tint2* __restrict__ betri, const tint3* __restrict__ nthro,
                            const tfloat3* __restrict__ coor, tfloat3* __restrict__ trinor,

@@ -154,6 +154,7 @@ __device__ __forceinline__ float3 f3scale(float3 a, float s) { return make_float
 // cell's area and volume (signed tetrahedra against the cell's first
 // triangle's first vertex), then the per-triangle broadcasts.
 __global__ __launch_bounds__(BS) void k_geo_cell(GcParams P, long stepcount,
+    const GcRep* __restrict__ reps, const int* __restrict__ cell_rep,
     const float* __restrict__ bbox, const int* __restrict__ curpos, const tint3* __restrict__ tri,
     const tint2* __restrict__ betri, const tint3* __restrict__ nthro, const tfloat3* __restrict__ coor,
     tfloat3* __restrict__ trinor, float* __restrict__ triarea, float* __restrict__ reftriarea,
@@ -164,6 +165,11 @@ __global__ __launch_bounds__(BS) void k_geo_cell(GcParams P, long stepcount,
     __shared__ double sa[BS], sv[BS];
     const int c = blockIdx.x;
     const int tb = betri[c].x, te = betri[c].y;
+    if (reps) {  // batched: this cell's suspension
+        const GcRep& R = reps[cell_rep[c]];
+        P = R.P;
+        bbox = R.bbox;
+    }
     PerBox b = load_box(P, bbox);
     int so = curpos[P.nc + tri[tb].x - 1];
     tfloat3 xo = coor[so];
@@ -287,6 +293,11 @@ __global__ __launch_bounds__(BS) void k_def_force(GcParams P, int nm, const floa
 {
     int v = xcd_block((nm + (int)blockDim.x - 1) / (int)blockDim.x) * blockDim.x + threadIdx.x;
     if (v >= nm) return;
+    if (a.reps) {  // batched: the vertex's suspension
+        const GcRep& R = a.reps[a.cell_rep[a.celltab[v]]];
+        P = R.P;
+        bbox = R.bbox;
+    }
     PerBox b = load_box(P, bbox);
     int sv = a.curpos[P.nc + v];
     tfloat3 xo = a.coor[sv];
@@ -429,7 +440,7 @@ extern "C" int gc_CalGeoPara(long stepcount, int ntri, int ncell, float* d_bbox,
         !d_refvol_para || !d_area_para || !d_vol_para || !d_area || !d_vol)
         return set_error(GC_ERR_INVALID, "CalGeoPara: null pointer");
     hipStream_t s = stream();
-    k_geo_cell<<<ncell, BS, 0, s>>>(P, stepcount, d_bbox, d_curpos, d_tri, d_beginendtri,
+    k_geo_cell<<<ncell, BS, 0, s>>>(P, stepcount, nullptr, nullptr, d_bbox, d_curpos, d_tri, d_beginendtri,
                                     d_nthrobound, d_totalcoor, d_trinordir, d_triarea,
                                     d_reftriarea, d_tricenter, d_refarea_para, d_refvol_para,
                                     d_area_para, d_vol_para, d_area, d_vol);

@@ -13,6 +13,27 @@ import numpy as np
 from . import abi
 
 
+def _state_arrays(case) -> dict:
+    N, M, Cn = case.N, case.nCellPar, case.nCell
+    out = {
+        "pos": np.zeros((N, 3), np.float64), "coor": np.zeros((N, 3), np.float32),
+        "velrhop": np.zeros((N, 4), np.float32), "velhalf": np.zeros((N, 3), np.float32),
+        "force": np.zeros((N, 3), np.float32), "press": np.zeros(N, np.float32),
+        "nthrobound": np.zeros((N, 3), np.int32),
+    }
+    if Cn:
+        out.update(area=np.zeros(Cn, np.float32), vol=np.zeros(Cn, np.float32),
+                   defforce=np.zeros((M, 3), np.float32), aggforce=np.zeros((M, 3), np.float32))
+    return out
+
+
+def _host_state(out: dict) -> abi.gc_host_state:
+    hs = abi.gc_host_state()
+    for k, v in out.items():
+        setattr(hs, k, v.ctypes.data)
+    return hs
+
+
 class Engine:
     def __init__(self, case):
         self.lib = abi.load()
@@ -48,19 +69,8 @@ class Engine:
         return i
 
     def state(self) -> dict:
-        N, M, Cn = self.N, self.nm, self.ncell
-        out = {
-            "pos": np.zeros((N, 3), np.float64), "coor": np.zeros((N, 3), np.float32),
-            "velrhop": np.zeros((N, 4), np.float32), "velhalf": np.zeros((N, 3), np.float32),
-            "force": np.zeros((N, 3), np.float32), "press": np.zeros(N, np.float32),
-            "nthrobound": np.zeros((N, 3), np.int32),
-        }
-        if Cn:
-            out.update(area=np.zeros(Cn, np.float32), vol=np.zeros(Cn, np.float32),
-                       defforce=np.zeros((M, 3), np.float32), aggforce=np.zeros((M, 3), np.float32))
-        hs = abi.gc_host_state()
-        for k, v in out.items():
-            setattr(hs, k, v.ctypes.data)
+        out = _state_arrays(self.case)
+        hs = _host_state(out)
         abi.check(self.lib.gc_EngineGetState(self.h, C.byref(hs)), "EngineGetState")
         return out
 
@@ -105,6 +115,59 @@ class Engine:
         for k, v in out.items():
             setattr(hc, k, v.ctypes.data)
         abi.check(self.lib.gc_EngineGetCells(self.h, C.byref(hc)), "EngineGetCells")
+        return out
+
+    def profile(self, nsteps: int = 10, driveon: int = 1) -> dict:
+        p = abi.gc_engine_profile()
+        abi.check(self.lib.gc_EngineProfile(self.h, int(nsteps), int(driveon), C.byref(p)),
+                  "EngineProfile")
+        return {p.names[k].value.decode(): float(p.us[k]) for k in range(p.nkernels)}
+
+
+class Ensemble:
+    """Independent suspensions stepped together by one engine
+    (gc_EnsembleCreate); replica r evolves bit for bit as Engine(cases[r])."""
+
+    def __init__(self, cases):
+        self.lib = abi.load()
+        self.cases = list(cases)
+        descs = (abi.gc_case_desc * len(self.cases))()
+        self._keep = []
+        for r, c in enumerate(self.cases):
+            d, keep = c.desc()
+            descs[r] = d
+            self._keep.append(keep)
+        st = C.c_int(0)
+        self.h = self.lib.gc_EnsembleCreate(descs, len(self.cases), C.byref(st))
+        if not self.h:
+            raise abi.GcError(f"gc_EnsembleCreate failed ({st.value}): "
+                              f"{self.lib.gc_GetLastError().decode()}")
+        self.nrep = self.lib.gc_EnsembleCount(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gc_EngineDestroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int = 1, driveon: int = 1, graph: bool = False):
+        f = self.lib.gc_EngineStepGraph if graph else self.lib.gc_EngineStep
+        abi.check(f(self.h, int(n), int(driveon)), "EngineStep")
+
+    def info(self) -> abi.gc_engine_info:
+        i = abi.gc_engine_info()
+        abi.check(self.lib.gc_EngineInfo(self.h, C.byref(i)), "EngineInfo")
+        return i
+
+    def state(self, rep: int) -> dict:
+        out = _state_arrays(self.cases[rep])
+        hs = _host_state(out)
+        abi.check(self.lib.gc_EnsembleGetState(self.h, int(rep), C.byref(hs)), "EnsembleGetState")
         return out
 
     def profile(self, nsteps: int = 10, driveon: int = 1) -> dict:

@@ -327,6 +327,21 @@ int gc_EngineWriteTecplot(gc_engine* e, const char* dir, long stepdata, long ste
 int gc_EngineSave(gc_engine* e, const char* path);
 int gc_EngineLoad(gc_engine* e, const char* path);
 
+/* ---- batched ensembles (no reference counterpart) ---------------------
+ * One engine over nrep independent suspensions of any sizes and physics
+ * (parameter sweeps, ensemble statistics): the same kernels and graph step
+ * all of them, so small suspensions share the launches instead of leaving
+ * the GPU idle.  Every replica evolves bit for bit as gc_EngineCreate on its
+ * own descriptor (its own ids and seed drive its pair noise); all replicas
+ * must start from the same noise counter.  gc_EngineStep/StepGraph/Profile/
+ * Info/Destroy apply to the whole batch; state is read per replica.
+ * gc_EngineSetState/Save/Load/WriteTecplot refuse a batch of more than one. */
+gc_engine* gc_EnsembleCreate(const gc_case_desc* descs, int nrep, int* status);
+/* Replica `rep`'s state in its own id order (arrays sized for that replica). */
+int gc_EnsembleGetState(gc_engine* e, int rep, gc_host_state* out);
+/* Number of replicas (1 for gc_EngineCreate; 0 for NULL). */
+int gc_EnsembleCount(const gc_engine* e);
+
 /* Per-kernel averages of the last gc_EngineProfile call, in microseconds. */
 typedef struct {
     int nkernels;

@@ -29,8 +29,16 @@ def box_case():
     return casegen.periodic_box(n=16, rbc_diameter=7.0, subdiv=2, seed=11)
 
 
+def bound1_case():
+    """Channel whose outermost wall layer is Bound1 (repulsion only) and
+    inner layers Bound2 (no-slip velocity)."""
+    ph = util.replace_physics(kT=0.01, fext=0.05)
+    c = casegen.channel(24, 12, 2, rbc_diameter=5.0, subdiv=2, seed=8, physics=ph)
+    return util.outer_walls_as_bound1(c, ph, 6.0)
+
+
 CASES = {"channel": lambda: channel_case(), "channel_morse": lambda: channel_case(De=0.02),
-         "box": box_case}
+         "box": box_case, "channel_bound1": bound1_case}
 
 
 @pytest.mark.parametrize("name", ["channel", "box"])
@@ -65,7 +73,7 @@ def _oracle_state(o, case, S, info):
     return st
 
 
-@pytest.mark.parametrize("name", ["channel", "channel_morse", "box"])
+@pytest.mark.parametrize("name", ["channel", "channel_morse", "box", "channel_bound1"])
 def test_engine_step_parity(gpu, name):
     case = CASES[name]()
     e = Engine(case)

@@ -77,3 +77,19 @@ def b11_error(F, Fref, abs_sum, fext=0.0):
 
 def replace_physics(**kw):
     return dataclasses.replace(casegen.Physics(), **kw)
+
+
+def outer_walls_as_bound1(case, physics, half_width: float):
+    """Channel case whose outermost wall layer (beyond half_width + 2 from
+    the axis) becomes Bound1 (repulsion only); the other wall layers stay
+    Bound2.  Rebuilds the load order and the parameter block's id ranges."""
+    c = dataclasses.replace(case)
+    walls = c.coor[:c.nBound2]
+    outer = np.maximum(np.abs(walls[:, 1]), np.abs(walls[:, 2])) > half_width + 2.0
+    b1, b2 = walls[outer], walls[~outer]
+    rest = c.coor[c.nBound2:]
+    c.vel = np.concatenate([np.zeros((walls.shape[0], 3), np.float32), c.vel[c.nBound2:]])
+    c.coor = np.concatenate([b1, b2, rest]).astype(np.float32)
+    c.nBound1, c.nBound2 = int(b1.shape[0]), int(b2.shape[0])
+    c.consts = casegen.make_consts(physics, c.nBound1, c.nBound2, c.nFluid, c.nCellPar)
+    return c
