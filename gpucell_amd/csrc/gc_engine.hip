@@ -13,9 +13,11 @@
 //   k2_sortgather  [4,5,6,8] rank within the cube by id -> final slot;
 //                           gather of the state; Groot-Warren predictor
 //   k2_density     [7,9]    number density (moving) and no-slip wall
-//                           velocity (Bound2) in one neighbour pass
+//                           velocity (Bound2) in one neighbour pass; writes
+//                           each moving particle's neighbour list
 //   k2_force       [10-13,17] EOS on the fly, SDPD pair force, Bound1
-//                           repulsion, body force; corrector for fluid
+//                           repulsion, body force over that list; corrector
+//                           for fluid
 //   k_geo_cell     [14]     membrane geometry, FP64 cell area/volume
 //   k2_agg         [16]     Morse between cells (when De != 0)
 //   k2_mgather, k_def_force [15,17] membrane forces, corrector for membrane
@@ -45,8 +47,36 @@ using namespace gc;
 namespace {
 
 constexpr int BS = GC_BS;
-// Hits buffered per lane by the force kernel (LDS: FORCE_HMAX * BS ints).
-constexpr int FORCE_HMAX = 32;
+// Neighbour list written by the density pass for every moving particle:
+// entry k of slot p at nl[k * N + p] (column-major: a wave's lanes write and
+// read consecutive words), value (slot << 6) | image code.  A particle with
+// more than NL_CAP neighbours gets count -1 and the force pass scans its
+// cubes instead.
+constexpr int NL_CAP = 64;
+
+// Image code of a range: 2 bits per axis, 1 = shift -L, 2 = shift +L.
+__device__ __forceinline__ int shift_code(float sx, float sy, float sz)
+{
+    return (sx < 0.f ? 1 : (sx > 0.f ? 2 : 0)) | ((sy < 0.f ? 1 : (sy > 0.f ? 2 : 0)) << 2) |
+           ((sz < 0.f ? 1 : (sz > 0.f ? 2 : 0)) << 4);
+}
+
+__device__ __forceinline__ float code_shift(int c, float L)
+{
+    return (c & 3) == 1 ? -L : ((c & 3) == 2 ? L : 0.0f);
+}
+
+// Slots [jb, je) in order, four loads issued before their use.
+template <class Fn>
+__device__ __forceinline__ void scan_range(const float4* __restrict__ posid, int jb, int je, Fn&& fn)
+{
+    int j = jb;
+    for (; j + 4 <= je; j += 4) {
+        float4 a = posid[j], b = posid[j + 1], c = posid[j + 2], d = posid[j + 3];
+        fn(j, a); fn(j + 1, b); fn(j + 2, c); fn(j + 3, d);
+    }
+    for (; j < je; ++j) fn(j, posid[j]);
+}
 
 __device__ __forceinline__ void key_to_cc(int k, const GcRep& R, int& cx, int& cy, int& cz)
 {
@@ -69,34 +99,43 @@ __device__ __forceinline__ const GcRep& rep_of_id(const GcRep& R0, const GcRep* 
 }
 
 // Contiguous slot ranges of the 27 neighbour cubes (x-fastest keys: one
-// range per non-wrapping (y, z) row), with the image shift of each range.
+// range per non-wrapping (y, z) row, plus the periodic image cube on a
+// wrapping x side), with the image shift of each range.  Order per row:
+// image at -x, image at +x, main range.  fn has a single call site, so a
+// heavy body is inlined once.
 template <class Fn>
 __device__ __forceinline__ void visit2(const GcRep& R, const int* __restrict__ offs_all, int cx,
                                        int cy, int cz, Fn&& fn)
 {
     const int* offs = offs_all + R.cube_off;
     const int* per = R.P.per;
+    const int nx = R.nc[0];
+    const bool wlo = per[0] && cx == 0, whi = per[0] && cx == nx - 1;
+    const int xlo = cx > 0 ? cx - 1 : 0, xhi = cx < nx - 1 ? cx + 1 : nx - 1;
+    const int np = 1 + (int)wlo + (int)whi;
+#pragma unroll 1
     for (int dz = -1; dz <= 1; ++dz) {
         int z = cz + dz;
         float sz = 0.0f;
         if (z < 0) { if (!per[2]) continue; z += R.nc[2]; sz = -R.L[2]; }
         else if (z >= R.nc[2]) { if (!per[2]) continue; z -= R.nc[2]; sz = R.L[2]; }
+#pragma unroll 1
         for (int dy = -1; dy <= 1; ++dy) {
             int y = cy + dy;
             float sy = 0.0f;
             if (y < 0) { if (!per[1]) continue; y += R.nc[1]; sy = -R.L[1]; }
             else if (y >= R.nc[1]) { if (!per[1]) continue; y -= R.nc[1]; sy = R.L[1]; }
-            int row = (z * R.nc[1] + y) * R.nc[0];
-            int xlo = cx - 1, xhi = cx + 1;
-            if (xlo < 0) {
-                if (per[0]) fn(offs[row + R.nc[0] - 1], offs[row + R.nc[0]], -R.L[0], sy, sz);
-                xlo = 0;
+            const int row = (z * R.nc[1] + y) * nx;
+#pragma unroll 1
+            for (int k = 0; k < np; ++k) {
+                // piece k: the -x image (if wlo), the +x image (if whi), the main range
+                bool img_lo = wlo && k == 0;
+                bool img_hi = whi && k == (int)wlo;
+                int a = img_lo ? nx - 1 : (img_hi ? 0 : xlo);
+                int b = img_lo ? nx : (img_hi ? 1 : xhi + 1);
+                float sx = img_lo ? -R.L[0] : (img_hi ? R.L[0] : 0.0f);
+                fn(offs[row + a], offs[row + b], sx, sy, sz);
             }
-            if (xhi >= R.nc[0]) {
-                if (per[0]) fn(offs[row], offs[row + 1], R.L[0], sy, sz);
-                xhi = R.nc[0] - 1;
-            }
-            fn(offs[row + xlo], offs[row + xhi + 1], 0.0f, sy, sz);
         }
     }
 }
@@ -211,7 +250,8 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
 template <bool ENS>
 __global__ __launch_bounds__(BS) void k2_density(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, const float4* __restrict__ posid, float4* __restrict__ velrho,
-    const int* __restrict__ offs, const int* __restrict__ key)
+    const int* __restrict__ offs, const int* __restrict__ key, int* __restrict__ nl,
+    int* __restrict__ nlc)
 {
     int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (p >= n) return;
@@ -225,32 +265,37 @@ __global__ __launch_bounds__(BS) void k2_density(int n, GcRep R0, const GcRep* _
     const float ih = 1.0f / P.h;
     if (id >= P.nf) {
         float d = 0.0f;  // self term added below
+        int cnt = 0;
         visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
-            for (int j = jb; j < je; ++j) {
-                float4 pj = posid[j];
+            const int code = shift_code(sx, sy, sz);
+            scan_range(posid, jb, je, [&](int j, float4 pj) {
                 float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
                 float r2 = rx * rx + ry * ry + rz * rz;
-                if (r2 < P.h2 && __float_as_int(pj.w) >= P.nb2 && j != p) {
-                    float q = sqrtf(r2) * ih, om = 1.0f - q;
-                    d += (1.0f + 3.0f * q) * om * om * om;
+                if (r2 < P.h2 && j != p) {
+                    if (cnt < NL_CAP) nl[cnt * n + p] = (j << 6) | code;
+                    ++cnt;
+                    if (__float_as_int(pj.w) >= P.nb2) {
+                        float q = __builtin_amdgcn_sqrtf(r2) * ih, om = 1.0f - q;
+                        d += (1.0f + 3.0f * q) * om * om * om;
+                    }
                 }
-            }
+            });
         });
         velrho[p].w = P.m * P.wnorm * (d + 1.0f);
+        nlc[p] = cnt <= NL_CAP ? cnt : -1;
     } else {
         float sw = 0.f, svx = 0.f, svy = 0.f, svz = 0.f;
         visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
-            for (int j = jb; j < je; ++j) {
-                float4 pj = posid[j];
+            scan_range(posid, jb, je, [&](int j, float4 pj) {
                 float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
                 float r2 = rx * rx + ry * ry + rz * rz;
                 if (r2 < P.h2 && __float_as_int(pj.w) >= P.nf) {
-                    float q = sqrtf(r2) * ih, om = 1.0f - q;
+                    float q = __builtin_amdgcn_sqrtf(r2) * ih, om = 1.0f - q;
                     float w = (1.0f + 3.0f * q) * om * om * om;
                     float4 vj = velrho[j];
                     sw += w; svx += w * vj.x; svy += w * vj.y; svz += w * vj.z;
                 }
-            }
+            });
         });
         float s = sw > 0.f ? -1.0f / sw : 0.0f;
         float4 v = velrho[p];
@@ -266,6 +311,7 @@ template <bool ENS>
 __global__ __launch_bounds__(BS) void k2_force(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, int driveon, const float4* __restrict__ posid,
     const float4* __restrict__ velrho, const int* __restrict__ offs, const int* __restrict__ key,
+    const int* __restrict__ nl, const int* __restrict__ nlc,
     const tfloat3* __restrict__ velhalf, tfloat3* __restrict__ force, float4* __restrict__ vout,
     const unsigned long long* __restrict__ ctr)
 {
@@ -282,29 +328,26 @@ __global__ __launch_bounds__(BS) void k2_force(int n, GcRep R0, const GcRep* __r
     const int lid = ENS ? local_id(R, id) : id;
     float4 vi = velrho[p];
     float di = vi.w * P.inv_m;
-    float pi_d2 = (id < P.nc ? P.c1sq : P.c2sq) * vi.w / (di * di);
+    // P_i / d_i^2 with P = c^2 rho and d = rho / m is c^2 m / d_i; the
+    // reciprocals are v_rcp_f32 / v_rsq_f32 (1 ulp), well inside the 1e-5
+    // force tolerance and much cheaper than IEEE division
+    const float rdi = __builtin_amdgcn_rcpf(di);
+    float pi_d2 = (id < P.nc ? P.c1sq : P.c2sq) * P.m * rdi;
     int cx, cy, cz;
     key_to_cc(key[p], R, cx, cy, cz);
     const float ih = 1.0f / P.h;
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    // Pass 1 scans the candidates and keeps the hits (slot << 6 | image
-    // code) in this lane's column of an LDS list; pass 2 evaluates them with
-    // the wave's lanes in step, so the expensive pair body (EOS, viscous
-    // terms, Philox + Box-Muller) runs with nearly all lanes busy instead of
-    // the ~15 % that are inside the cut-off at any one candidate.
-    __shared__ int hits[FORCE_HMAX * BS];
-    int* mine = hits + threadIdx.x;
-    int nh = 0;
     auto pair = [&](int code) {
         int j = code >> 6;
-        float sx = (code & 3) == 1 ? -R.L[0] : ((code & 3) == 2 ? R.L[0] : 0.0f);
-        float sy = ((code >> 2) & 3) == 1 ? -R.L[1] : (((code >> 2) & 3) == 2 ? R.L[1] : 0.0f);
-        float sz = ((code >> 4) & 3) == 1 ? -R.L[2] : (((code >> 4) & 3) == 2 ? R.L[2] : 0.0f);
+        float sx = code_shift(code, R.L[0]), sy = code_shift(code >> 2, R.L[1]),
+              sz = code_shift(code >> 4, R.L[2]);
         float4 pj = posid[j];
         float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
         float r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 == 0.f) return;  // coincident particles exert no pair force
         int idj = __float_as_int(pj.w);
-        float r = sqrtf(r2);
+        float ir = __builtin_amdgcn_rsqf(r2);
+        float r = r2 * ir;
         float om = 1.0f - r * ih;
         float F = P.fnorm * om * om;
         if (idj < P.nb2) {  // Bound1 wall: repulsion only
@@ -314,12 +357,12 @@ __global__ __launch_bounds__(BS) void k2_force(int n, GcRep R0, const GcRep* __r
         }
         float4 vj = velrho[j];
         float dj = vj.w * P.inv_m;
-        float cC = (pi_d2 + (idj < P.nc ? P.c1sq : P.c2sq) * vj.w / (dj * dj)) * F;
-        float ir = 1.0f / r;
+        float rdj = __builtin_amdgcn_rcpf(dj);
+        float cC = (pi_d2 + (idj < P.nc ? P.c1sq : P.c2sq) * P.m * rdj) * F;
         float ex = rx * ir, ey = ry * ir, ez = rz * ir;
         float vx = vi.x - vj.x, vy = vi.y - vj.y, vz = vi.z - vj.z;
         float ev = ex * vx + ey * vy + ez * vz;
-        float gg = F / (di * dj);
+        float gg = F * rdi * rdj;
         float ga = gg * P.visc_a, gb = gg * P.visc_b * ev;
         fx += cC * rx - (ga * vx + gb * ex);
         fy += cC * ry - (ga * vy + gb * ey);
@@ -329,7 +372,7 @@ __global__ __launch_bounds__(BS) void k2_force(int n, GcRep R0, const GcRep* __r
             float n0, n1, n2;
             pair_normals(P, (uint32_t)lid, (uint32_t)ljd, n0, n1, n2);
             float exi = ex * n0 + ey * n1 + ez * n2;
-            float amp = sqrtf(P.two_kT_over_dt * gg);
+            float amp = __builtin_amdgcn_sqrtf(P.two_kT_over_dt * gg);
             amp = lid < ljd ? amp : -amp;
             float pa = amp * P.sq_a, pe = amp * (P.sq_ab - P.sq_a) * exi;
             fx += pa * n0 + pe * ex;
@@ -337,23 +380,22 @@ __global__ __launch_bounds__(BS) void k2_force(int n, GcRep R0, const GcRep* __r
             fz += pa * n2 + pe * ez;
         }
     };
-    visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
-        int code = (sx < 0.f ? 1 : (sx > 0.f ? 2 : 0)) | ((sy < 0.f ? 1 : (sy > 0.f ? 2 : 0)) << 2) |
-                   ((sz < 0.f ? 1 : (sz > 0.f ? 2 : 0)) << 4);
-        for (int j = jb; j < je; ++j) {
-            float4 pj = posid[j];
-            float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
-            float r2 = rx * rx + ry * ry + rz * rz;
-            if (r2 < P.h2 && r2 != 0.f) {
-                mine[nh * BS] = (j << 6) | code;
-                if (++nh == FORCE_HMAX) {  // list full (dense spot): drain it here
-                    for (int k = 0; k < nh; ++k) pair(mine[k * BS]);
-                    nh = 0;
-                }
+    // The density pass listed the neighbours (r < h, j != p) in the scan
+    // order of the cubes, so the sums below run in that order either way.
+    const int cnt = nlc[p];
+    if (cnt >= 0) {
+        for (int k = 0; k < cnt; ++k) pair(nl[k * n + p]);
+    } else {  // more than NL_CAP neighbours: scan the cubes again
+        visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
+            const int code = shift_code(sx, sy, sz);
+            for (int j = jb; j < je; ++j) {
+                float4 pj = posid[j];
+                float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
+                float r2 = rx * rx + ry * ry + rz * rz;
+                if (r2 < P.h2 && j != p) pair((j << 6) | code);
             }
-        }
-    });
-    for (int k = 0; k < nh; ++k) pair(mine[k * BS]);
+        });
+    }
     if (driveon) fx += ext_force_x(P, pi.y);
     force[p] = tfloat3{fx, fy, fz};
     if (id < P.nc) {
@@ -391,9 +433,10 @@ __global__ __launch_bounds__(BS) void k2_agg(int n, GcRep R0, const GcRep* __res
             float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
             float r2 = rx * rx + ry * ry + rz * rz;
             if (r2 >= P.mcut2 || r2 == 0.f || celltab[idj - P.nc] == cell) continue;
-            float r = sqrtf(r2);
+            float ir = __builtin_amdgcn_rsqf(r2);
+            float r = r2 * ir;
             float ex = expf(P.beta * (P.r0 - r));
-            float fm = tb * (ex * ex - ex) / r;
+            float fm = tb * (ex * ex - ex) * ir;
             fx += fm * rx; fy += fm * ry; fz += fm * rz;
         }
     });
@@ -427,7 +470,7 @@ __global__ __launch_bounds__(BS) void k2_wallvel(int n, GcRep R0, const GcRep* _
             float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
             float r2 = rx * rx + ry * ry + rz * rz;
             if (r2 < P.h2 && __float_as_int(pj.w) >= P.nf) {
-                float q = sqrtf(r2) * ih, om = 1.0f - q;
+                float q = __builtin_amdgcn_sqrtf(r2) * ih, om = 1.0f - q;
                 float w = (1.0f + 3.0f * q) * om * om * om;
                 float4 vj = vout[j];
                 sw += w; svx += w * vj.x; svy += w * vj.y; svz += w * vj.z;
@@ -505,6 +548,7 @@ struct gc_engine {
     float4 *d_posid, *d_velrho, *d_vout;
     tfloat3 *d_coor, *d_velhalf, *d_force;
     int *d_key0, *d_key, *d_rank, *d_cnt, *d_offs, *d_tk, *d_tv, *d_curpos, *d_scanws;
+    int *d_nl, *d_nlc;   // neighbour lists of moving particles (density -> force)
     unsigned long long* d_ctr;
     // membrane
     tint3* d_tri; int6* d_edge; int* d_celltab; tint2* d_betri;
@@ -588,12 +632,12 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
     mark(pr, s, "neighbour build+predict");
     const int c = e->cur;
     k2_density<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid, e->d_velrho,
-                                      e->d_offs, e->d_key);
+                                      e->d_offs, e->d_key, e->d_nl, e->d_nlc);
     GC_LAUNCH_CHECK("engine density");
     mark(pr, s, "density+wall velocity");
     k2_force<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, driveon, e->d_posid,
-                                    e->d_velrho, e->d_offs, e->d_key, e->d_velhalf, e->d_force,
-                                    e->d_vout, e->d_ctr);
+                                    e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc,
+                                    e->d_velhalf, e->d_force, e->d_vout, e->d_ctr);
     GC_LAUNCH_CHECK("engine force");
     k2_bump<<<1, 1, 0, s>>>(e->d_ctr);
     mark(pr, s, "sdpd force");
@@ -819,6 +863,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
         (st = e->alloc(&e->d_rank, N)) || (st = e->alloc(&e->d_cnt, K + 1)) ||
         (st = e->alloc(&e->d_offs, K + 1)) || (st = e->alloc(&e->d_tk, N)) ||
         (st = e->alloc(&e->d_tv, N)) || (st = e->alloc(&e->d_curpos, N)) ||
+        (st = e->alloc(&e->d_nl, (size_t)NL_CAP * N)) || (st = e->alloc(&e->d_nlc, N)) ||
         (st = e->alloc(&e->d_scanws, scan_ws_ints(e->K))) || (st = e->alloc(&e->d_ctr, 1)))
         return fail(st);
     hipStream_t s = e->s;

@@ -79,8 +79,10 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t
 {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
-        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        // one v_mad_u64_u32 per product instead of a mul_lo + mul_hi pair
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+        uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
         uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
         c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
         k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
@@ -96,8 +98,9 @@ __device__ __forceinline__ float u01f(uint32_t x)
 // Three standard normals of the pair {ida, idb}: Philox counter
 // (min id, max id, counter lo, counter hi), key (seed lo, seed hi),
 // Box-Muller on (u0, u1) and (u2, u3).  The transcendentals are the
-// hardware instructions: v_log_f32 (log2) and v_sin_f32 / v_cos_f32, whose
-// argument is in revolutions, so cos(2 pi u) is one instruction.
+// hardware instructions: v_log_f32 (log2), v_sqrt_f32 and v_sin_f32 /
+// v_cos_f32, whose argument is in revolutions, so cos(2 pi u) is one
+// instruction.
 __device__ __forceinline__ void pair_normals(const GcParams& P, uint32_t ida, uint32_t idb,
                                              float& n0, float& n1, float& n2)
 {
@@ -105,8 +108,8 @@ __device__ __forceinline__ void pair_normals(const GcParams& P, uint32_t ida, ui
     uint32_t o[4];
     philox4x32_10(lo, hi, P.ctr_lo, P.ctr_hi, P.seed_lo, P.seed_hi, o);
     const float m2ln2 = -1.3862943611198906f;  // -2 ln 2
-    float r0 = sqrtf(m2ln2 * __builtin_amdgcn_logf(u01f(o[0])));
-    float r1 = sqrtf(m2ln2 * __builtin_amdgcn_logf(u01f(o[2])));
+    float r0 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u01f(o[0])));
+    float r1 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u01f(o[2])));
     float u1 = u01f(o[1]), u3 = u01f(o[3]);
     n0 = r0 * __builtin_amdgcn_cosf(u1);
     n1 = r0 * __builtin_amdgcn_sinf(u1);
