@@ -150,6 +150,15 @@ PROTOTYPES = {
     "gc_EnsembleCreate": (P, [C.POINTER(gc_case_desc), I, C.POINTER(C.c_int)]),
     "gc_EnsembleGetState": (I, [P, I, C.POINTER(gc_host_state)]),
     "gc_EnsembleCount": (I, [P]),
+    # learned membrane substepper (gpucell/surrogate.h)
+    "gc_MlpParamCount": (I, []),
+    "gc_MlpCreate": (P, [C.c_uint64, C.POINTER(C.c_int)]),
+    "gc_MlpDestroy": (None, [P]),
+    "gc_MlpSetParams": (I, [P, P, I]),
+    "gc_MlpGetParams": (I, [P, P, I]),
+    "gc_MlpForward": (I, [P, P, P, I]),
+    "gc_MlpGradient": (I, [P, P, P, I, P]),
+    "gc_MlpTrain": (I, [P, P, P, I, I, I, C.c_float, C.c_uint64, P]),
 }
 
 # Entry points that replace the reference's cupar/cudiv functions.

@@ -1,17 +1,18 @@
 """The C-ABI library loads and exports every function include/gpucell/*.h
 declares (no GPU needed: nothing here launches a kernel)."""
 import ctypes
+import glob
 import os
 import re
 
 from gpucell_amd import abi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HDR = os.path.join(ROOT, "include", "gpucell", "gpucell.h")
+HDRS = sorted(glob.glob(os.path.join(ROOT, "include", "gpucell", "*.h")))
 
 
 def declared_functions():
-    text = open(HDR).read()
+    text = "".join(open(h).read() for h in HDRS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(gc_[A-Za-z0-9_]+)\s*\(", text)))
 
