@@ -86,6 +86,11 @@ class gc_host_cells(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("sortpart", "cubepart", "cc", "curpos", "beginendcube")]
 
 
+class gc_membrane_stats(C.Structure):
+    _fields_ = [("steps", C.c_long), ("fallbacks", C.c_long), ("samples", C.c_long),
+                ("last_residual", C.c_float)]
+
+
 class gc_engine_profile(C.Structure):
     _fields_ = [("nkernels", C.c_int), ("names", (C.c_char * 32) * 16), ("us", C.c_float * 16)]
 
@@ -159,6 +164,9 @@ PROTOTYPES = {
     "gc_MlpForward": (I, [P, P, P, I]),
     "gc_MlpGradient": (I, [P, P, P, I, P]),
     "gc_MlpTrain": (I, [P, P, P, I, I, I, C.c_float, C.c_uint64, P]),
+    "gc_EngineSetMembraneMode": (I, [P, I, I, P, P, C.c_float]),
+    "gc_EngineRecordSamples": (I, [P, P, P, I]),
+    "gc_EngineMembraneStats": (I, [P, C.POINTER(gc_membrane_stats)]),
 }
 
 # Entry points that replace the reference's cupar/cudiv functions.

@@ -22,7 +22,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgpucell_hip.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 
-SOURCES = ["gc_abi.hip", "gc_sdpd.hip", "gc_membrane.hip", "gc_engine.hip", "gc_mlp.hip", "gc_io.cpp"]
+SOURCES = ["gc_abi.hip", "gc_sdpd.hip", "gc_membrane.hip", "gc_engine.hip", "gc_mlp.hip", "gc_substep.hip", "gc_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

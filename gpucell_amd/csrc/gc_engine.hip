@@ -36,7 +36,7 @@
 // contiguous and ordered by (cube, id) as when it runs alone, and the pair
 // noise uses the replica's own ids and seed, so every replica evolves bit
 // for bit as its solo run.
-#include "gc_kernels.h"
+#include "gc_engine_state.h"
 
 #include <math.h>
 #include <string.h>
@@ -149,7 +149,7 @@ __device__ __forceinline__ float ext_force_x(const GcParams& P, float y)
 // [1,2,3] integrate (FP64), wrap, key, histogram rank
 template <bool ENS>
 __global__ __launch_bounds__(BS) void k2_integrate(int n, GcRep R0, const GcRep* __restrict__ reps,
-    const int* __restrict__ rep_of, int move, double* __restrict__ pos64,
+    const int* __restrict__ rep_of, int move, int mmove, double* __restrict__ pos64,
     const float4* __restrict__ vout, const tfloat3* __restrict__ force, const int* __restrict__ sp,
     tint3* __restrict__ nthro, int* __restrict__ key, int* __restrict__ rank, int* __restrict__ cnt)
 {
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(BS) void k2_integrate(int n, GcRep R0, const GcRep*
     const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
     const GcParams& P = R.P;
     double x = pos64[p], y = pos64[n + p], z = pos64[2 * n + p];
-    if (move && id >= P.nf) {
+    if (move && id >= P.nf && (mmove || id < P.nc)) {
         double dt = P.dt, a = 0.5 * (double)P.dt * (double)P.dt * (double)P.inv_m;
         float4 v = vout[p];
         tfloat3 f = force[p];
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(BS) void k2_scatter(int n, const int* __restrict__ 
 // gather of the particle from its old slot curpos[id]; predictor.
 template <bool ENS>
 __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep* __restrict__ reps,
-    const int* __restrict__ rep_of, int predict, const int* __restrict__ tk,
+    const int* __restrict__ rep_of, int predict, int mpred, const int* __restrict__ tk,
     const int* __restrict__ tv, const int* __restrict__ offs, int* __restrict__ curpos,
     const double* __restrict__ pos_old, double* __restrict__ pos_new,
     const float4* __restrict__ vout, const tfloat3* __restrict__ force,
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     sp_new[s] = id;
     key_new[s] = k;
     float4 v = vout[o];
-    if (predict && id >= R0.P.nf) {
+    if (predict && id >= R0.P.nf && (mpred || id < R0.P.nc)) {
         const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
         const GcParams& P = R.P;
         tfloat3 f = force[o];
@@ -504,74 +504,7 @@ __global__ __launch_bounds__(BS) void k2_mgather(int nm, GcRep R0, const GcRep* 
     mpos[2 * nm + v] = (double)x.z + (R.P.per[2] ? (double)nn.z * L2 : 0.0);
 }
 
-template <class T>
-int dalloc(T** p, size_t count)
-{
-    if (hipMalloc((void**)p, sizeof(T) * (count ? count : 1)) != hipSuccess) {
-        *p = nullptr;
-        return set_error(GC_ERR_NOMEM, "engine allocation failed");
-    }
-    return GC_OK;
-}
-
-template <class T>
-int upload(T* d, const T* h, size_t count, hipStream_t s)
-{
-    if (!count) return GC_OK;
-    return hip_check(hipMemcpyAsync(d, h, sizeof(T) * count, hipMemcpyHostToDevice, s), "engine H2D");
-}
-
-// Per-replica bookkeeping on the host.
-struct RepMeta {
-    int nb1, nb2, nfl, nm, ncell, ntri, nedge;   // the replica's own counts
-    int vtx_off, cell_off, tri_off, edge_off;    // its first membrane vertex / cell / triangle / edge
-    Structconst consts;                          // its own id ranges
-};
-
 }  // namespace
-
-struct gc_engine {
-    int nrep = 1;
-    std::vector<GcRep> hreps;
-    std::vector<RepMeta> meta;
-    int N = 0, nm = 0, ncell = 0, ntri = 0, nedge = 0, K = 0;
-    Structconst consts;        // global type thresholds (physics of replica 0)
-    gc_ext_consts ext;
-    long step = 0;
-    hipStream_t s = nullptr;
-    int cur = 0;
-    std::vector<void*> allocs;
-
-    GcRep* d_reps = nullptr; int* d_rep_of = nullptr; int* d_cell_rep = nullptr;
-    float* d_bbox = nullptr;   // box of replica 0 (one suspension: the box)
-    double* d_pos64[2]; tint3* d_nt[2]; int* d_sp[2];
-    float4 *d_posid, *d_velrho, *d_vout;
-    tfloat3 *d_coor, *d_velhalf, *d_force;
-    int *d_key0, *d_key, *d_rank, *d_cnt, *d_offs, *d_tk, *d_tv, *d_curpos, *d_scanws;
-    int *d_nl, *d_nlc;   // neighbour lists of moving particles (density -> force)
-    unsigned long long* d_ctr;
-    // membrane
-    tint3* d_tri; int6* d_edge; int* d_celltab; tint2* d_betri;
-    float *d_mu0, *d_kc, *d_kd, *d_ka, *d_kv;
-    float *d_l0, *d_lmax, *d_kb, *d_mexp, *d_p, *d_kp, *d_th0;
-    tfloat3 *d_trinor, *d_tricen; float *d_triarea, *d_reftriarea;
-    float *d_refarea_para, *d_refvol_para, *d_area_para, *d_vol_para, *d_area, *d_vol, *d_pararea;
-    tfloat3 *d_defforce, *d_aggforce;
-    double* d_mpos;
-    int *d_es, *d_ee, *d_ts, *d_te;
-    hipGraphExec_t gexec = nullptr;
-    int g_driveon = -1;
-    bool walls_dirty = false;
-    bool any_morse = false, any_walls = false, any_bound1 = false;
-
-    template <class T> int alloc(T** p, size_t n)
-    {
-        int st = dalloc(p, n);
-        if (!st) allocs.push_back((void*)*p);
-        return st;
-    }
-    bool ens() const { return nrep > 1; }
-};
 
 namespace {
 
@@ -584,7 +517,10 @@ int launch_build_t(gc_engine* e, int move, int predict)
     hipStream_t s = e->s;
     const int gb = xcd_grid(blocks_for(N, BS));
     const GcRep& R0 = e->hreps[0];
-    k2_integrate<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, e->d_pos64[c],
+    // with membrane substepping the vertices are advanced by ms_step and
+    // carry their end-of-step velocity (no predictor)
+    const int mfree = e->ms == nullptr;
+    k2_integrate<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, mfree, e->d_pos64[c],
         e->d_vout, e->d_force, e->d_sp[c], e->d_nt[c], e->d_key0, e->d_rank, e->d_cnt);
     GC_LAUNCH_CHECK("engine integrate");
     int st = exclusive_scan(e->d_cnt, e->d_offs, e->K, s, e->d_scanws);
@@ -592,7 +528,7 @@ int launch_build_t(gc_engine* e, int move, int predict)
     k2_scatter<<<gb, BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[c], e->d_offs, e->d_tk, e->d_tv,
                                  e->d_cnt);
     GC_LAUNCH_CHECK("engine scatter");
-    k2_sortgather<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, predict, e->d_tk, e->d_tv,
+    k2_sortgather<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, predict, mfree, e->d_tk, e->d_tv,
         e->d_offs, e->d_curpos, e->d_pos64[c], e->d_pos64[nx], e->d_vout, e->d_force, e->d_nt[c],
         e->d_nt[nx], e->d_sp[nx], e->d_key, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf);
     GC_LAUNCH_CHECK("engine sort/gather");
@@ -641,7 +577,16 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
     GC_LAUNCH_CHECK("engine force");
     k2_bump<<<1, 1, 0, s>>>(e->d_ctr);
     mark(pr, s, "sdpd force");
-    if (e->ncell > 0) {
+    if (e->ncell > 0 && e->ms) {
+        if (e->any_morse) {
+            k2_agg<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid, e->d_offs,
+                                          e->d_key, e->d_celltab, e->d_force, e->d_aggforce);
+            GC_LAUNCH_CHECK("engine morse");
+        }
+        mark(pr, s, "morse");
+        if ((st = ms_step(e))) return st;
+        mark(pr, s, "membrane substeps");
+    } else if (e->ncell > 0) {
         const GcParams& P = R0.P;
         k_geo_cell<<<e->ncell, BS, 0, s>>>(P, e->step, ENS ? e->d_reps : nullptr,
             ENS ? e->d_cell_rep : nullptr, e->d_bbox, e->d_curpos, e->d_tri,
@@ -710,6 +655,7 @@ void destroy(gc_engine* e)
     if (!e) return;
     if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
     if (e->s) (void)hipStreamSynchronize(e->s);
+    if (e->ms) ms_destroy(e->ms);
     for (void* p : e->allocs) (void)hipFree(p);
     if (e->s) (void)hipStreamDestroy(e->s);
     delete e;
@@ -1017,6 +963,7 @@ extern "C" int gc_EngineStepGraph(gc_engine* e, int nsteps, int driveon)
 {
     if (!e || nsteps < 0) return set_error(GC_ERR_INVALID, "EngineStepGraph: bad arguments");
     if (nsteps == 0) return GC_OK;
+    if (e->ms) return gc_EngineStep(e, nsteps, driveon);   // host decides fallbacks: eager
     // A step alternates the double buffers, so one graph covers two steps
     // starting from buffer 0.
     if (e->cur != 0) {
@@ -1210,6 +1157,7 @@ extern "C" int gc_EngineSetState(gc_engine* e, const gc_host_state* in, long ste
     if ((st = reorder_carried(e, vo, f, vh))) return st;
     e->walls_dirty = false;
     e->step = step;
+    if (e->ms && (st = ms_reset(e))) return st;
     return GC_OK;
 }
 

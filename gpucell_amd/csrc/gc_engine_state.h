@@ -1,0 +1,98 @@
+// gc_engine_state.h -- the engine's library-owned state, shared by the step
+// (gc_engine.hip) and the membrane time stepping (gc_substep.hip).
+#pragma once
+
+#include <vector>
+
+#include "gc_kernels.h"
+
+namespace gc {
+
+template <class T>
+int dalloc(T** p, size_t count)
+{
+    if (hipMalloc((void**)p, sizeof(T) * (count ? count : 1)) != hipSuccess) {
+        *p = nullptr;
+        return set_error(GC_ERR_NOMEM, "engine allocation failed");
+    }
+    return GC_OK;
+}
+
+template <class T>
+int upload(T* d, const T* h, size_t count, hipStream_t s)
+{
+    if (!count) return GC_OK;
+    return hip_check(hipMemcpyAsync(d, h, sizeof(T) * count, hipMemcpyHostToDevice, s), "engine H2D");
+}
+
+// Per-replica bookkeeping on the host.
+struct RepMeta {
+    int nb1, nb2, nfl, nm, ncell, ntri, nedge;   // the replica's own counts
+    int vtx_off, cell_off, tri_off, edge_off;    // its first membrane vertex / cell / triangle / edge
+    Structconst consts;                          // its own id ranges
+};
+
+// Membrane substepping / surrogate state (gc_substep.hip); NULL while the
+// membrane moves with the fluid step.
+struct MembraneSub;
+void ms_destroy(MembraneSub* m);
+
+}  // namespace gc
+
+struct gc_engine {
+    int nrep = 1;
+    std::vector<gc::GcRep> hreps;
+    std::vector<gc::RepMeta> meta;
+    int N = 0, nm = 0, ncell = 0, ntri = 0, nedge = 0, K = 0;
+    Structconst consts;        // global type thresholds (physics of replica 0)
+    gc_ext_consts ext;
+    long step = 0;
+    hipStream_t s = nullptr;
+    int cur = 0;
+    std::vector<void*> allocs;
+
+    gc::GcRep* d_reps = nullptr; int* d_rep_of = nullptr; int* d_cell_rep = nullptr;
+    float* d_bbox = nullptr;   // box of replica 0 (one suspension: the box)
+    double* d_pos64[2]; tint3* d_nt[2]; int* d_sp[2];
+    float4 *d_posid, *d_velrho, *d_vout;
+    tfloat3 *d_coor, *d_velhalf, *d_force;
+    int *d_key0, *d_key, *d_rank, *d_cnt, *d_offs, *d_tk, *d_tv, *d_curpos, *d_scanws;
+    int *d_nl, *d_nlc;   // neighbour lists of moving particles (density -> force)
+    unsigned long long* d_ctr;
+    // membrane
+    tint3* d_tri; int6* d_edge; int* d_celltab; tint2* d_betri;
+    float *d_mu0, *d_kc, *d_kd, *d_ka, *d_kv;
+    float *d_l0, *d_lmax, *d_kb, *d_mexp, *d_p, *d_kp, *d_th0;
+    tfloat3 *d_trinor, *d_tricen; float *d_triarea, *d_reftriarea;
+    float *d_refarea_para, *d_refvol_para, *d_area_para, *d_vol_para, *d_area, *d_vol, *d_pararea;
+    tfloat3 *d_defforce, *d_aggforce;
+    double* d_mpos;
+    int *d_es, *d_ee, *d_ts, *d_te;
+    hipGraphExec_t gexec = nullptr;
+    int g_driveon = -1;
+    bool walls_dirty = false;
+    bool any_morse = false, any_walls = false, any_bound1 = false;
+    gc::MembraneSub* ms = nullptr;   // membrane substepping (mode 1/2)
+
+    template <class T> int alloc(T** p, size_t n)
+    {
+        int st = gc::dalloc(p, n);
+        if (!st) allocs.push_back((void*)*p);
+        return st;
+    }
+    bool ens() const { return nrep > 1; }
+};
+
+namespace gc {
+// One outer step of the membrane under substepping or the surrogate, after
+// the fluid forces of the step (gc_substep.hip).
+int ms_step(gc_engine* e);
+// Re-reads the membrane state from the engine arrays (after SetState).
+int ms_reset(gc_engine* e);
+}  // namespace gc
+
+struct gc_mlp;
+namespace gc {
+// Inference of the substepper network on stream s (gc_mlp.hip).
+int mlp_forward(gc_mlp* m, const float* x, float* y, int batch, hipStream_t s);
+}  // namespace gc

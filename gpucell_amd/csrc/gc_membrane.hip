@@ -363,6 +363,7 @@ __global__ __launch_bounds__(BS) void k_def_force(GcParams P, int nm, const floa
         fz += -0.5f * cA * dA.z - cV * dV.z * (1.0f / 6.0f);
     }
     a.defforce[v] = tfloat3{fx, fy, fz};
+    if (!a.force) return;   // membrane-only evaluation (substepping)
     tfloat3 F = a.force[sv];
     F.x += fx; F.y += fy; F.z += fz;
     a.force[sv] = F;

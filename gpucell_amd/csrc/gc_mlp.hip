@@ -480,9 +480,9 @@ int ensure_ws(gc_mlp* m, int Bp)
     return GC_OK;
 }
 
-int launch_fused(gc_mlp* m, int B, const float* x, float* y, int ycols, int ldy, bool train)
+int launch_fused(gc_mlp* m, int B, const float* x, float* y, int ycols, int ldy, bool train,
+                 hipStream_t s)
 {
-    hipStream_t s = stream();
     int nrows = train ? ((B + 255) / 256) * 256 : B;
     int grid = std::min(fused_grid(), (nrows + 63) / 64);
     if (grid < 1) return GC_OK;
@@ -620,8 +620,16 @@ extern "C" int gc_MlpForward(gc_mlp* m, const float* x, float* y, int batch)
 {
     if (!m || batch < 0 || (batch > 0 && (!x || !y))) return set_error(GC_ERR_INVALID, "MlpForward: bad arguments");
     if (batch == 0) return GC_OK;
-    return launch_fused(m, batch, x, y, OUT, OUT, false);
+    return launch_fused(m, batch, x, y, OUT, OUT, false, stream());
 }
+
+namespace gc {
+int mlp_forward(gc_mlp* m, const float* x, float* y, int batch, hipStream_t s)
+{
+    if (batch == 0) return GC_OK;
+    return launch_fused(m, batch, x, y, OUT, OUT, false, s);
+}
+}  // namespace gc
 
 extern "C" int gc_MlpGradient(gc_mlp* m, const float* x, const float* dy, int batch, float* grad)
 {
@@ -629,7 +637,7 @@ extern "C" int gc_MlpGradient(gc_mlp* m, const float* x, const float* dy, int ba
     const int Bp = ((batch + 255) / 256) * 256;
     int st = ensure_ws(m, Bp);
     if (st) return st;
-    if ((st = launch_fused(m, batch, x, m->y, OUTP, OUTP, true))) return st;
+    if ((st = launch_fused(m, batch, x, m->y, OUTP, OUTP, true, stream()))) return st;
     k_dy_in<<<blocks_for(Bp * OUTP, 256), 256, 0, stream()>>>(batch, Bp, dy, m->g4);
     GC_LAUNCH_CHECK("mlp dy");
     if ((st = backward(m, Bp))) return st;
@@ -672,7 +680,7 @@ extern "C" int gc_MlpTrain(gc_mlp* m, const float* x, const float* y, int nsampl
             const int B = std::min(batch, nsamples - b * batch);
             k_gather_batch<<<blocks_for(Bp * (IN + OUT), 256), 256, 0, s>>>(B, Bp, d_idx + (size_t)b * batch,
                                                                              x, y, m->xf, m->tf);
-            if ((st = launch_fused(m, B, m->xf, m->y, OUTP, OUTP, true))) break;
+            if ((st = launch_fused(m, B, m->xf, m->y, OUTP, OUTP, true, s))) break;
             k_mse<<<nlp, MLP_BS, 0, s>>>(B, Bp, m->y, m->tf, m->g4, m->lpart);
             k_sum_into<<<1, 1, 0, s>>>(nlp, m->lpart, 1.0f / ((float)B * OUT * nb), d_loss + ep);
             if ((st = hip_check(hipGetLastError(), "MlpTrain loss"))) break;

@@ -123,6 +123,31 @@ class Engine:
                   "EngineProfile")
         return {p.names[k].value.decode(): float(p.us[k]) for k in range(p.nkernels)}
 
+    # --- membrane time stepping (include/gpucell/surrogate.h) ------------
+    def set_membrane_mode(self, mode: int, k: int = 8, mlp=None, norm=None, threshold: float = 0.0):
+        """0: membrane moves with the fluid step; 1: k plain substeps;
+        2: the MLP (gpucell_amd.surrogate.Mlp) predicts the 8 substeps, with
+        fallback to plain substepping when the residual exceeds threshold."""
+        self._norm = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        abi.check(self.lib.gc_EngineSetMembraneMode(
+            self.h, int(mode), int(k), mlp.h if mlp is not None else None,
+            self._norm.ctypes.data if self._norm is not None else None, C.c_float(threshold)),
+            "EngineSetMembraneMode")
+
+    def record_samples(self, x=None, y=None):
+        """Record mode-1 training rows into CUDA tensors x [n][36], y [n][24]
+        (None stops)."""
+        self._rec = (x, y)
+        abi.check(self.lib.gc_EngineRecordSamples(
+            self.h, x.data_ptr() if x is not None else None, y.data_ptr() if y is not None else None,
+            int(x.shape[0]) if x is not None else 0), "EngineRecordSamples")
+
+    def membrane_stats(self) -> dict:
+        m = abi.gc_membrane_stats()
+        abi.check(self.lib.gc_EngineMembraneStats(self.h, C.byref(m)), "EngineMembraneStats")
+        return {"steps": m.steps, "fallbacks": m.fallbacks, "samples": m.samples,
+                "last_residual": m.last_residual}
+
 
 class Ensemble:
     """Independent suspensions stepped together by one engine

@@ -54,6 +54,45 @@ int gc_MlpGradient(gc_mlp* m, const float* x, const float* dy, int batch, float*
 int gc_MlpTrain(gc_mlp* m, const float* x, const float* y, int nsamples, int batch, int epochs,
                 float lr, uint64_t seed, float* loss_per_epoch);
 
+/* Part 2: membrane time stepping of an engine (one suspension, not a batch).
+ *
+ * mode 0: the reference's: membrane vertices move with the fluid step.
+ * mode 1: plain substepping: per engine step of length dt, the membrane
+ *         takes k velocity-Verlet substeps of dt/k under the fluid force of
+ *         the step (SDPD pair force, Morse, body force; held fixed) plus
+ *         the membrane force re-evaluated at every substep (FP64 positions,
+ *         FP64 cell area/volume reductions).
+ * mode 2: surrogate: the MLP predicts the k = 8 substep positions of every
+ *         vertex from its 1-ring in a local frame; the membrane force is then
+ *         evaluated once at the predicted end state, and the residual
+ *           R = rms_v |d2x_v - dts^2 a_v| / (|d2x_v| + dts^2 |a_v| + 1e-4 l_v)
+ *         (d2x = x8 - 2 x7 + x6, a = acceleration at x8, l = the vertex's
+ *         rest edge length) decides: R > threshold -> the step's membrane
+ *         part is recomputed by plain substepping (the fallback).
+ * The fluid keeps the engine's step in every mode.  In modes 1 and 2 the
+ * exported force of a membrane vertex is the fluid part (the membrane part
+ * is in defforce).  norm: 120 floats (feature mean[36], feature std[36],
+ * target mean[24], target std[24]) or NULL for none.  Steps in modes 1/2
+ * run eagerly (gc_EngineStepGraph falls back to gc_EngineStep). */
+#ifndef GPUCELL_GPUCELL_H
+typedef struct gc_engine gc_engine;
+#endif
+int gc_EngineSetMembraneMode(gc_engine* e, int mode, int k, gc_mlp* mlp, const float* norm,
+                             float threshold);
+/* Mode 1 with k = 8: every step appends one row per membrane vertex to the
+ * device buffers x[capacity][36] (raw features) and y[capacity][24] (raw
+ * targets: the 8 substep displacements minus the ballistic part s dts v0,
+ * in the vertex frame, in units of its rest edge length) until capacity
+ * rows are written.  x = NULL stops recording. */
+int gc_EngineRecordSamples(gc_engine* e, float* x, float* y, int capacity);
+typedef struct {
+    long steps;          /* membrane steps in mode 1/2 */
+    long fallbacks;      /* surrogate steps recomputed by plain substepping */
+    long samples;        /* rows recorded */
+    float last_residual; /* R of the last surrogate step */
+} gc_membrane_stats;
+int gc_EngineMembraneStats(gc_engine* e, gc_membrane_stats* out);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
