@@ -26,11 +26,17 @@ HBM_PEAK_GBS = 8000.0
 # force+corrector 72) and per membrane vertex (440).
 STEP_BYTES_PER_PARTICLE = 286.0
 STEP_BYTES_PER_VERTEX = 440.0
-# Dominant kernel (SDPD pair force with EOS, walls, body force and
-# corrector fused): reads posid 16 B + predicted velocity/density 16 B +
-# half-step velocity 12 B, writes force 12 B + velocity/density 16 B, per
-# particle (neighbour data come from cache).
-FORCE_BYTES_PER_PARTICLE = 72.0
+# Algorithmic bytes per particle of the single-kernel passes (neighbour
+# data from cache): integrate+wrap+key reads the FP64 position 24 B,
+# velocity 16 B and force 12 B and writes the key 4 B; density reads the
+# FP32 position 12 B and writes the density 4 B; the SDPD force pass (EOS,
+# walls, body force and corrector fused) reads position 12 B + predicted
+# velocity 12 B + density 4 B + id 4 B + half-step velocity 12 B and writes
+# velocity/density 16 B + force 12 B.
+KERNEL_BYTES_PER_PARTICLE = {"integrate+key": 56.0, "density+wall velocity": 16.0,
+                             "sdpd force": 72.0}
+KERNEL_NAMES = {"integrate+key": "k2_integrate", "density+wall velocity": "k2_density",
+                "sdpd force": "k2_force"}
 
 
 def log(msg):
@@ -116,12 +122,13 @@ def main():
     log(f"[bench] {a.steps} steps: {ms:.4f} ms/step, finite={finite}")
     prof = eng.profile(20)
     log("[bench] per-phase us: " + json.dumps(prof))
-    dom = max(prof, key=prof.get)
-    force_us = prof.get("sdpd force", prof[dom])
+    # dominant single kernel of the step (HIP events on the engine stream)
+    dom = max(KERNEL_BYTES_PER_PARTICLE, key=lambda k: prof.get(k, 0.0))
+    dom_us = prof[dom]
     N, Nm = case.N, case.nCellPar
-    achieved = FORCE_BYTES_PER_PARTICLE * N / (force_us * 1e-6) / 1e9
+    achieved = KERNEL_BYTES_PER_PARTICLE[dom] * N / (dom_us * 1e-6) / 1e9
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", f"traffic_sdpd_force_{a.case}.json")
+    pmc = os.path.join(ROOT, "profiles", f"traffic_{KERNEL_NAMES[dom]}_{a.case}.json")
     if os.path.exists(pmc):
         traffic = json.load(open(pmc)).get("bytes_per_launch")
     step_bytes = STEP_BYTES_PER_PARTICLE * N + STEP_BYTES_PER_VERTEX * Nm
@@ -141,9 +148,10 @@ def main():
         "data": "synthetic (seeded lattice fluid, walls and biconcave RBCs)",
         "config": {"workload": workload, "N": N, "Nm": Nm, "h": float(case.consts.CutRadius),
                    "dt": float(case.consts.TimeStep), "graph": graph, "finite": finite},
-        "roofline": {"bound": "hbm", "kernel": "sdpd force", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "avg_us": force_us},
+                     "traffic": traffic, "avg_us": dom_us,
+                     "algorithmic_bytes_per_particle": KERNEL_BYTES_PER_PARTICLE[dom]},
         "step_roofline": {"bytes_per_step": step_bytes, "achieved": step_gbs,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
         "phases_us": prof,

@@ -88,7 +88,7 @@ class gc_host_cells(C.Structure):
 
 class gc_membrane_stats(C.Structure):
     _fields_ = [("steps", C.c_long), ("fallbacks", C.c_long), ("samples", C.c_long),
-                ("last_residual", C.c_float)]
+                ("last_residual", C.c_float), ("plain_residual", C.c_float)]
 
 
 class gc_engine_profile(C.Structure):

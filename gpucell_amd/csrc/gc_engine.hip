@@ -247,56 +247,148 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
 
 // [7,9] density of moving particles; velocity of Bound2 walls from the
 // predicted velocities of their moving neighbours.
+// One (y, z) row of a lane's 27-cube neighbourhood: the main x range
+// [ma, mb) (no x shift) and, for a cube on a periodic x face, the image
+// cube [wa, wb) on the far face with x shift wsx; (sy, sz) the row's image
+// shift.  ok = false for a row beyond a non-periodic face.
+struct RowRanges {
+    int ma, mb, wa, wb;
+    float wsx, sy, sz;
+    bool ok;
+};
+
+__device__ __forceinline__ RowRanges row_ranges(const GcRep& R, const int* __restrict__ offs_all, int cx,
+                                                int cy, int cz, int q)
+{
+    RowRanges r;
+    r.ok = false; r.ma = r.mb = r.wa = r.wb = 0; r.wsx = r.sy = r.sz = 0.0f;
+    const int* offs = offs_all + R.cube_off;
+    const int* per = R.P.per;
+    int z = cz + q / 3 - 1, y = cy + q % 3 - 1;
+    if (z < 0) { if (!per[2]) return r; z += R.nc[2]; r.sz = -R.L[2]; }
+    else if (z >= R.nc[2]) { if (!per[2]) return r; z -= R.nc[2]; r.sz = R.L[2]; }
+    if (y < 0) { if (!per[1]) return r; y += R.nc[1]; r.sy = -R.L[1]; }
+    else if (y >= R.nc[1]) { if (!per[1]) return r; y -= R.nc[1]; r.sy = R.L[1]; }
+    const int nx = R.nc[0], row = (z * R.nc[1] + y) * nx;
+    const int xlo = cx > 0 ? cx - 1 : 0, xhi = cx < nx - 1 ? cx + 1 : nx - 1;
+    r.ma = offs[row + xlo];
+    r.mb = offs[row + xhi + 1];
+    if (per[0] && cx == 0) { r.wa = offs[row + nx - 1]; r.wb = offs[row + nx]; r.wsx = -R.L[0]; }
+    else if (per[0] && cx == nx - 1) { r.wa = offs[row]; r.wb = offs[row + 1]; r.wsx = R.L[0]; }
+    r.ok = true;
+    return r;
+}
+
+__device__ __forceinline__ int wave_min(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    return v;
+}
+// LDS written by some lanes of a wave, then read by others
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Particles staged in LDS per wave and neighbour row.
+constexpr int STAGE = 192;
+
+// [7,9] density of moving particles (and their neighbour lists); velocity
+// of Bound2 walls from the predicted velocities of their moving neighbours.
+// A wave's 64 slots lie in a few consecutive cubes of one row, so for each
+// of the 9 neighbour rows the union of its lanes' main ranges is one short
+// slot interval: the wave loads it into LDS once (coalesced) and every lane
+// scans its own sub-range there instead of issuing ~10 scattered global
+// loads per row.  A union longer than STAGE, and the periodic image cubes,
+// are read from global memory.  Scan order (image cube, then main range,
+// row by row) is the same either way.
 template <bool ENS>
 __global__ __launch_bounds__(BS) void k2_density(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, const float4* __restrict__ posid, float4* __restrict__ velrho,
     const int* __restrict__ offs, const int* __restrict__ key, int* __restrict__ nl,
     int* __restrict__ nlc)
 {
-    int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
-    if (p >= n) return;
-    float4 pi = posid[p];
-    int id = __float_as_int(pi.w);
-    if (id < R0.P.nb2) return;
-    const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
+    __shared__ float4 stage[BS / 64][STAGE];
+    const int lane = threadIdx.x & 63;
+    float4* st = stage[threadIdx.x >> 6];
+    const int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
+    const bool inb = p < n;
+    const float4 pi = inb ? posid[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int id = inb ? __float_as_int(pi.w) : 0;
+    const bool act = inb && id >= R0.P.nb2;
+    if (__ballot(act) == 0) return;   // wave-uniform
+    const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, act ? id : 0);
     const GcParams& P = R.P;
-    int cx, cy, cz;
-    key_to_cc(key[p], R, cx, cy, cz);
+    int cx = 0, cy = 0, cz = 0;
+    if (act) key_to_cc(key[p], R, cx, cy, cz);
+    const bool mov = act && id >= P.nf;
     const float ih = 1.0f / P.h;
-    if (id >= P.nf) {
-        float d = 0.0f;  // self term added below
-        int cnt = 0;
-        visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
-            const int code = shift_code(sx, sy, sz);
-            scan_range(posid, jb, je, [&](int j, float4 pj) {
-                float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
-                float r2 = rx * rx + ry * ry + rz * rz;
-                if (r2 < P.h2 && j != p) {
-                    if (cnt < NL_CAP) nl[cnt * n + p] = (j << 6) | code;
-                    ++cnt;
-                    if (__float_as_int(pj.w) >= P.nb2) {
-                        float q = __builtin_amdgcn_sqrtf(r2) * ih, om = 1.0f - q;
-                        d += (1.0f + 3.0f * q) * om * om * om;
-                    }
+    float d = 0.0f;  // self term added below
+    float sw = 0.f, svx = 0.f, svy = 0.f, svz = 0.f;
+    int cnt = 0;
+    auto consider = [&](int j, float4 pj, float sx, float sy, float sz, int code) {
+        float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
+        float r2 = rx * rx + ry * ry + rz * rz;
+        if (!(r2 < P.h2)) return;
+        const int idj = __float_as_int(pj.w);
+        if (mov) {
+            if (j == p) return;
+            if (cnt < NL_CAP) nl[cnt * n + p] = (j << 6) | code;
+            ++cnt;
+            if (idj >= P.nb2) {
+                float q = __builtin_amdgcn_sqrtf(r2) * ih, om = 1.0f - q;
+                d += (1.0f + 3.0f * q) * om * om * om;
+            }
+        } else if (idj >= P.nf) {
+            float q = __builtin_amdgcn_sqrtf(r2) * ih, om = 1.0f - q;
+            float w = (1.0f + 3.0f * q) * om * om * om;
+            float4 vj = velrho[j];
+            sw += w; svx += w * vj.x; svy += w * vj.y; svz += w * vj.z;
+        }
+    };
+    for (int q = 0; q < 9; ++q) {
+        RowRanges rr = act ? row_ranges(R, offs, cx, cy, cz, q) : RowRanges{0, 0, 0, 0, 0.f, 0.f, 0.f, false};
+        const int lo = wave_min(rr.ok && rr.mb > rr.ma ? rr.ma : 0x7fffffff);
+        const int hi = wave_max(rr.ok && rr.mb > rr.ma ? rr.mb : -1);
+        if (rr.ok && rr.wb > rr.wa) {
+            const int code = shift_code(rr.wsx, rr.sy, rr.sz);
+            scan_range(posid, rr.wa, rr.wb,
+                       [&](int j, float4 pj) { consider(j, pj, rr.wsx, rr.sy, rr.sz, code); });
+        }
+        const int code = shift_code(0.0f, rr.sy, rr.sz);
+        if (hi > lo && hi - lo <= STAGE) {   // wave-uniform
+            for (int i = lane; i < hi - lo; i += 64) st[i] = posid[lo + i];
+            wave_lds_sync();
+            if (rr.ok) {
+                int j = rr.ma;
+                for (; j + 4 <= rr.mb; j += 4) {
+                    float4 a = st[j - lo], b = st[j + 1 - lo], c = st[j + 2 - lo], e4 = st[j + 3 - lo];
+                    consider(j, a, 0.0f, rr.sy, rr.sz, code);
+                    consider(j + 1, b, 0.0f, rr.sy, rr.sz, code);
+                    consider(j + 2, c, 0.0f, rr.sy, rr.sz, code);
+                    consider(j + 3, e4, 0.0f, rr.sy, rr.sz, code);
                 }
-            });
-        });
+                for (; j < rr.mb; ++j) consider(j, st[j - lo], 0.0f, rr.sy, rr.sz, code);
+            }
+            wave_lds_sync();
+        } else if (rr.ok) {
+            scan_range(posid, rr.ma, rr.mb,
+                       [&](int j, float4 pj) { consider(j, pj, 0.0f, rr.sy, rr.sz, code); });
+        }
+    }
+    if (mov) {
         velrho[p].w = P.m * P.wnorm * (d + 1.0f);
         nlc[p] = cnt <= NL_CAP ? cnt : -1;
-    } else {
-        float sw = 0.f, svx = 0.f, svy = 0.f, svz = 0.f;
-        visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
-            scan_range(posid, jb, je, [&](int j, float4 pj) {
-                float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
-                float r2 = rx * rx + ry * ry + rz * rz;
-                if (r2 < P.h2 && __float_as_int(pj.w) >= P.nf) {
-                    float q = __builtin_amdgcn_sqrtf(r2) * ih, om = 1.0f - q;
-                    float w = (1.0f + 3.0f * q) * om * om * om;
-                    float4 vj = velrho[j];
-                    sw += w; svx += w * vj.x; svy += w * vj.y; svz += w * vj.z;
-                }
-            });
-        });
+    } else if (act && id < P.nf) {
         float s = sw > 0.f ? -1.0f / sw : 0.0f;
         float4 v = velrho[p];
         v.x = svx * s; v.y = svy * s; v.z = svz * s;
@@ -508,10 +600,24 @@ __global__ __launch_bounds__(BS) void k2_mgather(int nm, GcRep R0, const GcRep* 
 
 namespace {
 
+struct Prof {
+    hipEvent_t ev[32];
+    const char* name[32];
+    int n = 0;
+};
+
+void mark(Prof* pr, hipStream_t s, const char* name)
+{
+    if (!pr || pr->n >= 32) return;
+    (void)hipEventRecord(pr->ev[pr->n], s);
+    pr->name[pr->n] = name;
+    pr->n++;
+}
+
 // Neighbour build: keys from the (optionally advanced) FP64 positions,
 // counting sort by (cube, id), gather (+ predictor).
 template <bool ENS>
-int launch_build_t(gc_engine* e, int move, int predict)
+int launch_build_t(gc_engine* e, int move, int predict, Prof* pr = nullptr)
 {
     const int N = e->N, c = e->cur, nx = 1 - c;
     hipStream_t s = e->s;
@@ -523,8 +629,10 @@ int launch_build_t(gc_engine* e, int move, int predict)
     k2_integrate<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, mfree, e->d_pos64[c],
         e->d_vout, e->d_force, e->d_sp[c], e->d_nt[c], e->d_key0, e->d_rank, e->d_cnt);
     GC_LAUNCH_CHECK("engine integrate");
+    mark(pr, s, "integrate+key");
     int st = exclusive_scan(e->d_cnt, e->d_offs, e->K, s, e->d_scanws);
     if (st) return st;
+    mark(pr, s, "cube offsets");
     k2_scatter<<<gb, BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[c], e->d_offs, e->d_tk, e->d_tv,
                                  e->d_cnt);
     GC_LAUNCH_CHECK("engine scatter");
@@ -541,20 +649,6 @@ int launch_build(gc_engine* e, int move, int predict)
     return e->ens() ? launch_build_t<true>(e, move, predict) : launch_build_t<false>(e, move, predict);
 }
 
-struct Prof {
-    hipEvent_t ev[32];
-    const char* name[32];
-    int n = 0;
-};
-
-void mark(Prof* pr, hipStream_t s, const char* name)
-{
-    if (!pr || pr->n >= 32) return;
-    (void)hipEventRecord(pr->ev[pr->n], s);
-    pr->name[pr->n] = name;
-    pr->n++;
-}
-
 template <bool ENS>
 int launch_step_t(gc_engine* e, int driveon, Prof* pr)
 {
@@ -563,9 +657,9 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
     hipStream_t s = e->s;
     const int gb = xcd_grid(blocks_for(N, BS));
     mark(pr, s, "start");
-    int st = launch_build_t<ENS>(e, 1, 1);
+    int st = launch_build_t<ENS>(e, 1, 1, pr);
     if (st) return st;
-    mark(pr, s, "neighbour build+predict");
+    mark(pr, s, "sort+gather+predict");
     const int c = e->cur;
     k2_density<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid, e->d_velrho,
                                       e->d_offs, e->d_key, e->d_nl, e->d_nlc);
@@ -575,8 +669,9 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
                                     e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc,
                                     e->d_velhalf, e->d_force, e->d_vout, e->d_ctr);
     GC_LAUNCH_CHECK("engine force");
-    k2_bump<<<1, 1, 0, s>>>(e->d_ctr);
     mark(pr, s, "sdpd force");
+    k2_bump<<<1, 1, 0, s>>>(e->d_ctr);
+    mark(pr, s, "noise counter");
     if (e->ncell > 0 && e->ms) {
         if (e->any_morse) {
             k2_agg<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid, e->d_offs,

@@ -295,8 +295,24 @@ __global__ void k_ms_apply(int nm, float dts, const float* __restrict__ pred, co
                      del[2].z - 2.0f * del[1].z + del[0].z};
 }
 
-// surrogate end kick + residual partial sums (block tree, fixed order)
-__global__ __launch_bounds__(BS) void k_ms_finish(int nm, float dts, float inv_m, tfloat3* __restrict__ mv,
+// plain trajectory: d2x = x8 - 2 x7 + x6 from the substep positions
+__global__ void k_ms_d2x(int nm, const double* __restrict__ x8, const double* __restrict__ x7,
+                         const double* __restrict__ x6, tfloat3* __restrict__ d2x)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nm) return;
+    float c[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        size_t i = (size_t)d * nm + v;
+        c[d] = (float)((x8[i] - x7[i]) - (x7[i] - x6[i]));
+    }
+    d2x[v] = tfloat3{c[0], c[1], c[2]};
+}
+
+// residual partial sums (block tree, fixed order); with kick, first the
+// surrogate's end kick v8 = v7.5 + dts/2 a8
+__global__ __launch_bounds__(BS) void k_ms_finish(int nm, int kick, float dts, float inv_m, tfloat3* __restrict__ mv,
     const tfloat3* __restrict__ fe, const tfloat3* __restrict__ fm, const tfloat3* __restrict__ d2x,
     const float* __restrict__ lref, float* __restrict__ part)
 {
@@ -306,12 +322,14 @@ __global__ __launch_bounds__(BS) void k_ms_finish(int nm, float dts, float inv_m
     if (v < nm) {
         tfloat3 a = fe[v], b = fm[v], d = d2x[v];
         float3 acc = make_float3((a.x + b.x) * inv_m, (a.y + b.y) * inv_m, (a.z + b.z) * inv_m);
-        const float h = 0.5f * dts;
-        tfloat3 u = mv[v];
-        mv[v] = tfloat3{u.x + h * acc.x, u.y + h * acc.y, u.z + h * acc.z};
+        if (kick) {
+            const float h = 0.5f * dts;
+            tfloat3 u = mv[v];
+            mv[v] = tfloat3{u.x + h * acc.x, u.y + h * acc.y, u.z + h * acc.z};
+        }
         float3 e = scl3(acc, dts * dts);
         float3 dd = make_float3(d.x, d.y, d.z);
-        float r = norm3(sub3(dd, e)) / (norm3(dd) + norm3(e) + 1e-4f * lref[v]);
+        float r = norm3(sub3(dd, e)) / lref[v];
         q = r * r;
     }
     sh[threadIdx.x] = q;
@@ -341,7 +359,7 @@ struct MembraneSub {
     float thr = 0.f;
     GcParams Pm;
     int nm = 0;
-    double *mx = nullptr, *mx0 = nullptr;
+    double *mx = nullptr, *mx0 = nullptr, *mx6 = nullptr, *mx7 = nullptr;
     tfloat3 *mv, *mv0, *fm, *fm0, *fe, *mcoor, *d2x;
     int* mid; tint3* mnt;
     int* ring; float* lref;
@@ -444,6 +462,16 @@ int plain_substeps(gc_engine* e, MembraneSub* m, bool record, long row0)
             k_ms_target<<<nb, BS, 0, s>>>(m->nm, q, dts, m->mx, m->mx0, m->mv0, m->frame, m->lref,
                                            m->rec_y + (size_t)NO * row0);
         GC_LAUNCH_CHECK("membrane substep");
+        if (m->k == KSUB && (q == KSUB - 2 || q == KSUB - 1) &&
+            (st = hip_check(hipMemcpyAsync(q == KSUB - 2 ? m->mx6 : m->mx7, m->mx, sizeof(double) * 3 * (size_t)m->nm,
+                                           hipMemcpyDeviceToDevice, s), "membrane substep copy")))
+            return st;
+    }
+    if (m->k == KSUB) {  // the residual of the exact trajectory (scale for the threshold)
+        k_ms_d2x<<<nb, BS, 0, s>>>(m->nm, m->mx, m->mx7, m->mx6, m->d2x);
+        k_ms_finish<<<nb, BS, 0, s>>>(m->nm, 0, dts, inv_m, m->mv, m->fe, m->fm, m->d2x, m->lref, m->part);
+        k_ms_res_final<<<1, 1, 0, s>>>(nb, m->nm, m->part, m->res + 1);
+        GC_LAUNCH_CHECK("membrane plain residual");
     }
     return GC_OK;
 }
@@ -493,7 +521,7 @@ int ms_step(gc_engine* e)
                                      m->mx0, m->mv0, m->mx, m->mv, m->mcoor, m->d2x);
         GC_LAUNCH_CHECK("membrane surrogate apply");
         if ((st = ms_force(e, m))) return st;
-        k_ms_finish<<<nb, BS, 0, s>>>(nm, dts, inv_m, m->mv, m->fe, m->fm, m->d2x, m->lref, m->part);
+        k_ms_finish<<<nb, BS, 0, s>>>(nm, 1, dts, inv_m, m->mv, m->fe, m->fm, m->d2x, m->lref, m->part);
         k_ms_res_final<<<1, 1, 0, s>>>(nb, nm, m->part, m->res);
         GC_LAUNCH_CHECK("membrane residual");
         float r = 0.f;
@@ -580,14 +608,15 @@ extern "C" int gc_EngineSetMembraneMode(gc_engine* e, int mode, int k, gc_mlp* m
         MembraneSub* m = new MembraneSub();
         const size_t nm = (size_t)e->nm;
         m->nm = e->nm;
-        if ((st = m->alloc(&m->mx, 3 * nm)) || (st = m->alloc(&m->mx0, 3 * nm)) || (st = m->alloc(&m->mv, nm)) ||
+        if ((st = m->alloc(&m->mx, 3 * nm)) || (st = m->alloc(&m->mx0, 3 * nm)) ||
+            (st = m->alloc(&m->mx6, 3 * nm)) || (st = m->alloc(&m->mx7, 3 * nm)) || (st = m->alloc(&m->mv, nm)) ||
             (st = m->alloc(&m->mv0, nm)) || (st = m->alloc(&m->fm, nm)) || (st = m->alloc(&m->fm0, nm)) ||
             (st = m->alloc(&m->fe, nm)) || (st = m->alloc(&m->mcoor, nm)) || (st = m->alloc(&m->d2x, nm)) ||
             (st = m->alloc(&m->mid, nm)) || (st = m->alloc(&m->mnt, nm)) || (st = m->alloc(&m->ring, 6 * nm)) ||
             (st = m->alloc(&m->lref, nm)) || (st = m->alloc(&m->frame, 9 * nm)) ||
             (st = m->alloc(&m->feat, (size_t)NF * nm)) || (st = m->alloc(&m->pred, (size_t)NO * nm)) ||
             (st = m->alloc(&m->norm, 2 * (NF + NO))) || (st = m->alloc(&m->part, (size_t)blocks_for(m->nm, BS))) ||
-            (st = m->alloc(&m->res, 1))) {
+            (st = m->alloc(&m->res, 2))) {
             ms_destroy(m);
             return st;
         }
@@ -668,5 +697,9 @@ extern "C" int gc_EngineMembraneStats(gc_engine* e, gc_membrane_stats* out)
     out->fallbacks = e->ms->fallbacks;
     out->samples = e->ms->rec_count;
     out->last_residual = e->ms->last_res;
-    return GC_OK;
+    float r[2] = {0.f, 0.f};
+    int st = hip_check(hipStreamSynchronize(e->s), "MembraneStats sync");
+    if (!st) st = hip_check(hipMemcpy(r, e->ms->res, sizeof(r), hipMemcpyDeviceToHost), "MembraneStats");
+    out->plain_residual = r[1];
+    return st;
 }

@@ -146,7 +146,7 @@ class Engine:
         m = abi.gc_membrane_stats()
         abi.check(self.lib.gc_EngineMembraneStats(self.h, C.byref(m)), "EngineMembraneStats")
         return {"steps": m.steps, "fallbacks": m.fallbacks, "samples": m.samples,
-                "last_residual": m.last_residual}
+                "last_residual": m.last_residual, "plain_residual": m.plain_residual}
 
 
 class Ensemble:

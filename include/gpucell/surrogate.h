@@ -65,10 +65,12 @@ int gc_MlpTrain(gc_mlp* m, const float* x, const float* y, int nsamples, int bat
  * mode 2: surrogate: the MLP predicts the k = 8 substep positions of every
  *         vertex from its 1-ring in a local frame; the membrane force is then
  *         evaluated once at the predicted end state, and the residual
- *           R = rms_v |d2x_v - dts^2 a_v| / (|d2x_v| + dts^2 |a_v| + 1e-4 l_v)
+ *           R = rms_v |d2x_v - dts^2 a_v| / l_v
  *         (d2x = x8 - 2 x7 + x6, a = acceleration at x8, l = the vertex's
- *         rest edge length) decides: R > threshold -> the step's membrane
- *         part is recomputed by plain substepping (the fallback).
+ *         rest edge length: the velocity-Verlet relation of the last
+ *         substeps, which an exact trajectory meets to O(dts^3), in edge
+ *         lengths) decides: R > threshold -> the step's membrane part is
+ *         recomputed by plain substepping (the fallback).
  * The fluid keeps the engine's step in every mode.  In modes 1 and 2 the
  * exported force of a membrane vertex is the fluid part (the membrane part
  * is in defforce).  norm: 120 floats (feature mean[36], feature std[36],
@@ -90,6 +92,9 @@ typedef struct {
     long fallbacks;      /* surrogate steps recomputed by plain substepping */
     long samples;        /* rows recorded */
     float last_residual; /* R of the last surrogate step */
+    float plain_residual;/* R of the last plain 8-substep trajectory (the
+                            same formula on the exact trajectory: the
+                            scale a threshold is set against) */
 } gc_membrane_stats;
 int gc_EngineMembraneStats(gc_engine* e, gc_membrane_stats* out);
 
