@@ -146,45 +146,66 @@ __device__ __forceinline__ float ext_force_x(const GcParams& P, float y)
     return (P.force_mode == GC_FORCE_REVERSE_POISEUILLE && y >= P.force_split) ? -f : f;
 }
 
-// [1,2,3] integrate (FP64), wrap, key, histogram rank
+// [1,2,3] integrate (FP64), wrap, key, histogram rank.  Slots are in the
+// previous step's (cube, id) order and few particles change cube per step,
+// so consecutive lanes mostly share a key: each run of equal keys in a wave
+// takes its ranks with one atomic (base + offset in the run), about a third
+// of the device-scope atomics of one per particle.
 template <bool ENS>
 __global__ __launch_bounds__(BS) void k2_integrate(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, int move, int mmove, double* __restrict__ pos64,
     const float4* __restrict__ vout, const tfloat3* __restrict__ force, const int* __restrict__ sp,
     tint3* __restrict__ nthro, int* __restrict__ key, int* __restrict__ rank, int* __restrict__ cnt)
 {
-    int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
-    if (p >= n) return;
-    int id = sp[p];
-    const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
-    const GcParams& P = R.P;
-    double x = pos64[p], y = pos64[n + p], z = pos64[2 * n + p];
-    if (move && id >= P.nf && (mmove || id < P.nc)) {
-        double dt = P.dt, a = 0.5 * (double)P.dt * (double)P.dt * (double)P.inv_m;
-        float4 v = vout[p];
-        tfloat3 f = force[p];
-        double u[3] = {x + (dt * v.x + a * f.x), y + (dt * v.y + a * f.y), z + (dt * v.z + a * f.z)};
-        tint3 im = nthro[p];
-        int img[3] = {im.x, im.y, im.z};
-        bool wrapped = false;
+    const int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
+    const bool act = p < n;
+    if (__ballot(act) == 0) return;   // wave-uniform
+    int k = -1;
+    if (act) {
+        int id = sp[p];
+        const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
+        const GcParams& P = R.P;
+        double x = pos64[p], y = pos64[n + p], z = pos64[2 * n + p];
+        if (move && id >= P.nf && (mmove || id < P.nc)) {
+            double dt = P.dt, a = 0.5 * (double)P.dt * (double)P.dt * (double)P.inv_m;
+            float4 v = vout[p];
+            tfloat3 f = force[p];
+            double u[3] = {x + (dt * v.x + a * f.x), y + (dt * v.y + a * f.y), z + (dt * v.z + a * f.z)};
+            tint3 im = nthro[p];
+            int img[3] = {im.x, im.y, im.z};
+            bool wrapped = false;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            if (!P.per[d]) continue;
-            double lo = R.bbox[2 * d], hi = R.bbox[2 * d + 1];
-            double L = (double)(R.bbox[2 * d + 1] - R.bbox[2 * d]);
-            if (u[d] < lo) { u[d] += L; img[d] -= 1; wrapped = true; }
-            else if (u[d] >= hi) { u[d] -= L; img[d] += 1; wrapped = true; }
+            for (int d = 0; d < 3; ++d) {
+                if (!P.per[d]) continue;
+                double lo = R.bbox[2 * d], hi = R.bbox[2 * d + 1];
+                double L = (double)(R.bbox[2 * d + 1] - R.bbox[2 * d]);
+                if (u[d] < lo) { u[d] += L; img[d] -= 1; wrapped = true; }
+                else if (u[d] >= hi) { u[d] -= L; img[d] += 1; wrapped = true; }
+            }
+            if (wrapped) nthro[p] = tint3{img[0], img[1], img[2]};
+            x = u[0]; y = u[1]; z = u[2];
+            pos64[p] = x; pos64[n + p] = y; pos64[2 * n + p] = z;
         }
-        if (wrapped) nthro[p] = tint3{img[0], img[1], img[2]};
-        x = u[0]; y = u[1]; z = u[2];
-        pos64[p] = x; pos64[n + p] = y; pos64[2 * n + p] = z;
+        int cx = cube_coord((float)x, R.bbox[0], P.off[0], R.invw[0], R.nc[0]);
+        int cy = cube_coord((float)y, R.bbox[2], P.off[1], R.invw[1], R.nc[1]);
+        int cz = cube_coord((float)z, R.bbox[4], P.off[2], R.invw[2], R.nc[2]);
+        k = R.cube_off + cx + R.nc[0] * (cy + R.nc[1] * cz);
+        key[p] = k;
     }
-    int cx = cube_coord((float)x, R.bbox[0], P.off[0], R.invw[0], R.nc[0]);
-    int cy = cube_coord((float)y, R.bbox[2], P.off[1], R.invw[1], R.nc[1]);
-    int cz = cube_coord((float)z, R.bbox[4], P.off[2], R.invw[2], R.nc[2]);
-    int k = R.cube_off + cx + R.nc[0] * (cy + R.nc[1] * cz);
-    key[p] = k;
-    rank[p] = atomicAdd(&cnt[k], 1);
+    // runs of equal keys among the active lanes (active lanes are a prefix)
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(k, 1);
+    const bool head = act && (lane == 0 || prev != k);
+    const unsigned long long heads = __ballot(head);
+    const int nact = __popcll(__ballot(act));
+    const unsigned long long above = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+    const int next = above ? (int)__ffsll((long long)above) - 1 : nact;
+    int base = 0;
+    if (head) base = atomicAdd(&cnt[k], next - lane);
+    const unsigned long long upto = lane == 63 ? heads : heads & ((2ull << lane) - 1ull);
+    const int myhead = upto ? 63 - __clzll((long long)upto) : 0;
+    base = __shfl(base, myhead);
+    if (act) rank[p] = base + (lane - myhead);
 }
 
 // [4] (cube, id) into cube segments; clears the histogram for the next step
