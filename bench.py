@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--case", default="suspension_1m")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--eager", action="store_true", help="launch kernels without a graph")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes (no autotuning before the timed run)")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -105,6 +107,12 @@ def main():
     case, workload = make_case(a.case)
     log(f"[bench] case ready: N={case.N} Nm={case.nCellPar} ({time.perf_counter() - t:.1f}s)")
     eng = Engine(case)
+    tune_rows = None
+    if not a.no_autotune:
+        log("[bench] autotuning launch shapes")
+        tune_rows = eng.autotune(steps=5)
+        for r in tune_rows:
+            log(f"[bench]   {r['family']:15s} {r['shape']:22s} {r['us']:9.2f} us{'  <-' if r['chosen'] else ''}")
     log("[bench] engine created; warmup")
     graph = not a.eager
     eng.step(a.warmup, graph=graph)
@@ -155,6 +163,8 @@ def main():
         "step_roofline": {"bytes_per_step": step_bytes, "achieved": step_gbs,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
         "phases_us": prof,
+        "tuning": None if tune_rows is None else {
+            r["family"]: r["shape"] for r in tune_rows if r["chosen"]},
     }
     if not a.no_cpu:
         log("[bench] CPU baseline (FP64 oracle)")

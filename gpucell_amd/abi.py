@@ -91,6 +91,11 @@ class gc_membrane_stats(C.Structure):
                 ("last_residual", C.c_float), ("plain_residual", C.c_float)]
 
 
+class gc_tune_report(C.Structure):
+    _fields_ = [("n", C.c_int), ("family", (C.c_char * 24) * 64), ("variant", C.c_int * 64),
+                ("shape", (C.c_char * 32) * 64), ("us", C.c_float * 64), ("chosen", C.c_int * 64)]
+
+
 class gc_engine_profile(C.Structure):
     _fields_ = [("nkernels", C.c_int), ("names", (C.c_char * 32) * 16), ("us", C.c_float * 16)]
 
@@ -152,6 +157,9 @@ PROTOTYPES = {
     "gc_EngineWriteTecplot": (I, [P, C.c_char_p, L, L, C.c_float, L, P, P, P]),
     "gc_EngineSave": (I, [P, C.c_char_p]),
     "gc_EngineLoad": (I, [P, C.c_char_p]),
+    "gc_EngineAutotune": (I, [P, I, I, C.POINTER(gc_tune_report)]),
+    "gc_EngineGetTuning": (I, [P, P]),
+    "gc_EngineSetTuning": (I, [P, P]),
     "gc_EnsembleCreate": (P, [C.POINTER(gc_case_desc), I, C.POINTER(C.c_int)]),
     "gc_EnsembleGetState": (I, [P, I, C.POINTER(gc_host_state)]),
     "gc_EnsembleCount": (I, [P]),

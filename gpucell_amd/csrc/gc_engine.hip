@@ -39,6 +39,7 @@
 #include "gc_engine_state.h"
 
 #include <math.h>
+#include <stdio.h>
 #include <string.h>
 #include <vector>
 
@@ -151,13 +152,13 @@ __device__ __forceinline__ float ext_force_x(const GcParams& P, float y)
 // so consecutive lanes mostly share a key: each run of equal keys in a wave
 // takes its ranks with one atomic (base + offset in the run), about a third
 // of the device-scope atomics of one per particle.
-template <bool ENS>
-__global__ __launch_bounds__(BS) void k2_integrate(int n, GcRep R0, const GcRep* __restrict__ reps,
+template <bool ENS, int TBS>
+__global__ __launch_bounds__(TBS) void k2_integrate(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, int move, int mmove, double* __restrict__ pos64,
     const float4* __restrict__ vout, const tfloat3* __restrict__ force, const int* __restrict__ sp,
     tint3* __restrict__ nthro, int* __restrict__ key, int* __restrict__ rank, int* __restrict__ cnt)
 {
-    const int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
+    const int p = xcd_block((n + TBS - 1) / TBS) * TBS + threadIdx.x;
     const bool act = p < n;
     if (__ballot(act) == 0) return;   // wave-uniform
     int k = -1;
@@ -260,7 +261,9 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
         velhalf[s] = tfloat3{v.x + h * f.x, v.y + h * f.y, v.z + h * f.z};
         v.x += l * f.x; v.y += l * f.y; v.z += l * f.z;
     } else {
-        velhalf[s] = tfloat3{v.x, v.y, v.z};
+        // walls do not move: their half-step velocity is zero (independent of
+        // the exported wall velocity, so reading the state changes nothing)
+        velhalf[s] = id < R0.P.nf ? tfloat3{0.f, 0.f, 0.f} : tfloat3{v.x, v.y, v.z};
     }
     velrho[s] = v;
     curpos[id] = s;
@@ -320,8 +323,6 @@ __device__ __forceinline__ void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Particles staged in LDS per wave and neighbour row.
-constexpr int STAGE = 192;
 
 // [7,9] density of moving particles (and their neighbour lists); velocity
 // of Bound2 walls from the predicted velocities of their moving neighbours.
@@ -329,19 +330,19 @@ constexpr int STAGE = 192;
 // of the 9 neighbour rows the union of its lanes' main ranges is one short
 // slot interval: the wave loads it into LDS once (coalesced) and every lane
 // scans its own sub-range there instead of issuing ~10 scattered global
-// loads per row.  A union longer than STAGE, and the periodic image cubes,
+// loads per row.  A union longer than TSTAGE, and the periodic image cubes,
 // are read from global memory.  Scan order (image cube, then main range,
 // row by row) is the same either way.
-template <bool ENS>
-__global__ __launch_bounds__(BS) void k2_density(int n, GcRep R0, const GcRep* __restrict__ reps,
+template <bool ENS, int TBS, int TSTAGE>
+__global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, const float4* __restrict__ posid, float4* __restrict__ velrho,
     const int* __restrict__ offs, const int* __restrict__ key, int* __restrict__ nl,
     int* __restrict__ nlc)
 {
-    __shared__ float4 stage[BS / 64][STAGE];
+    __shared__ float4 stage[TBS / 64][TSTAGE > 0 ? TSTAGE : 1];
     const int lane = threadIdx.x & 63;
     float4* st = stage[threadIdx.x >> 6];
-    const int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
+    const int p = xcd_block((n + TBS - 1) / TBS) * TBS + threadIdx.x;
     const bool inb = p < n;
     const float4 pi = inb ? posid[p] : make_float4(0.f, 0.f, 0.f, 0.f);
     const int id = inb ? __float_as_int(pi.w) : 0;
@@ -356,20 +357,21 @@ __global__ __launch_bounds__(BS) void k2_density(int n, GcRep R0, const GcRep* _
     float d = 0.0f;  // self term added below
     float sw = 0.f, svx = 0.f, svy = 0.f, svz = 0.f;
     int cnt = 0;
+    int woff = p;   // entry cnt of this lane's list is nl[p + cnt n]
     auto consider = [&](int j, float4 pj, float sx, float sy, float sz, int code) {
         float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
         float r2 = rx * rx + ry * ry + rz * rz;
-        if (!(r2 < P.h2)) return;
         const int idj = __float_as_int(pj.w);
         if (mov) {
-            if (j == p) return;
-            if (cnt < NL_CAP) nl[cnt * n + p] = (j << 6) | code;
-            ++cnt;
-            if (idj >= P.nb2) {
+            if (r2 < P.h2 && j != p) {
+                if (cnt < NL_CAP) nl[woff] = (j << 6) | code;
+                woff += n;
+                ++cnt;
                 float q = __builtin_amdgcn_sqrtf(r2) * ih, om = 1.0f - q;
-                d += (1.0f + 3.0f * q) * om * om * om;
+                float w = (1.0f + 3.0f * q) * om * om * om;
+                d += idj >= P.nb2 ? w : 0.0f;
             }
-        } else if (idj >= P.nf) {
+        } else if (r2 < P.h2 && idj >= P.nf) {
             float q = __builtin_amdgcn_sqrtf(r2) * ih, om = 1.0f - q;
             float w = (1.0f + 3.0f * q) * om * om * om;
             float4 vj = velrho[j];
@@ -386,7 +388,7 @@ __global__ __launch_bounds__(BS) void k2_density(int n, GcRep R0, const GcRep* _
                        [&](int j, float4 pj) { consider(j, pj, rr.wsx, rr.sy, rr.sz, code); });
         }
         const int code = shift_code(0.0f, rr.sy, rr.sz);
-        if (hi > lo && hi - lo <= STAGE) {   // wave-uniform
+        if (TSTAGE > 0 && hi > lo && hi - lo <= TSTAGE) {   // wave-uniform
             for (int i = lane; i < hi - lo; i += 64) st[i] = posid[lo + i];
             wave_lds_sync();
             if (rr.ok) {
@@ -420,15 +422,15 @@ __global__ __launch_bounds__(BS) void k2_density(int n, GcRep R0, const GcRep* _
 // [10-13,17] SDPD pair force with the pressure from the EOS on the fly,
 // Bound1 repulsion, body force; corrector for fluid (membrane vertices get
 // it after their membrane force).
-template <bool ENS>
-__global__ __launch_bounds__(BS) void k2_force(int n, GcRep R0, const GcRep* __restrict__ reps,
+template <bool ENS, int TBS>
+__global__ __launch_bounds__(TBS) void k2_force(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, int driveon, const float4* __restrict__ posid,
     const float4* __restrict__ velrho, const int* __restrict__ offs, const int* __restrict__ key,
     const int* __restrict__ nl, const int* __restrict__ nlc,
     const tfloat3* __restrict__ velhalf, tfloat3* __restrict__ force, float4* __restrict__ vout,
     const unsigned long long* __restrict__ ctr)
 {
-    int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
+    int p = xcd_block((n + TBS - 1) / TBS) * TBS + threadIdx.x;
     if (p >= n) return;
     float4 pi = posid[p];
     int id = __float_as_int(pi.w);
@@ -647,8 +649,17 @@ int launch_build_t(gc_engine* e, int move, int predict, Prof* pr = nullptr)
     // with membrane substepping the vertices are advanced by ms_step and
     // carry their end-of-step velocity (no predictor)
     const int mfree = e->ms == nullptr;
-    k2_integrate<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, mfree, e->d_pos64[c],
-        e->d_vout, e->d_force, e->d_sp[c], e->d_nt[c], e->d_key0, e->d_rank, e->d_cnt);
+    {
+        auto L = [&](auto kern, int tbs) {
+            kern<<<xcd_grid(blocks_for(N, tbs)), tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, mfree,
+                e->d_pos64[c], e->d_vout, e->d_force, e->d_sp[c], e->d_nt[c], e->d_key0, e->d_rank, e->d_cnt);
+        };
+        switch (e->tune.v[TUNE_INTEGRATE]) {
+            case 0: L(k2_integrate<ENS, 64>, 64); break;
+            case 1: L(k2_integrate<ENS, 128>, 128); break;
+            default: L(k2_integrate<ENS, 256>, 256); break;
+        }
+    }
     GC_LAUNCH_CHECK("engine integrate");
     mark(pr, s, "integrate+key");
     int st = exclusive_scan(e->d_cnt, e->d_offs, e->K, s, e->d_scanws);
@@ -682,13 +693,40 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
     if (st) return st;
     mark(pr, s, "sort+gather+predict");
     const int c = e->cur;
-    k2_density<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid, e->d_velrho,
-                                      e->d_offs, e->d_key, e->d_nl, e->d_nlc);
+    {
+        auto L = [&](auto kern, int tbs) {
+            kern<<<xcd_grid(blocks_for(N, tbs)), tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid,
+                e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc);
+        };
+        switch (e->tune.v[TUNE_DENSITY]) {   // block size x LDS row stage
+            case 0: L(k2_density<ENS, 64, 0>, 64); break;
+            case 1: L(k2_density<ENS, 64, 128>, 64); break;
+            case 2: L(k2_density<ENS, 64, 192>, 64); break;
+            case 3: L(k2_density<ENS, 64, 320>, 64); break;
+            case 4: L(k2_density<ENS, 128, 0>, 128); break;
+            case 5: L(k2_density<ENS, 128, 128>, 128); break;
+            case 6: L(k2_density<ENS, 128, 192>, 128); break;
+            case 7: L(k2_density<ENS, 128, 320>, 128); break;
+            case 8: L(k2_density<ENS, 256, 0>, 256); break;
+            case 9: L(k2_density<ENS, 256, 128>, 256); break;
+            case 11: L(k2_density<ENS, 256, 320>, 256); break;
+            default: L(k2_density<ENS, 256, 192>, 256); break;
+        }
+    }
     GC_LAUNCH_CHECK("engine density");
     mark(pr, s, "density+wall velocity");
-    k2_force<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, driveon, e->d_posid,
-                                    e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc,
-                                    e->d_velhalf, e->d_force, e->d_vout, e->d_ctr);
+    {
+        auto L = [&](auto kern, int tbs) {
+            kern<<<xcd_grid(blocks_for(N, tbs)), tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, driveon,
+                e->d_posid, e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc, e->d_velhalf, e->d_force,
+                e->d_vout, e->d_ctr);
+        };
+        switch (e->tune.v[TUNE_FORCE]) {
+            case 0: L(k2_force<ENS, 64>, 64); break;
+            case 1: L(k2_force<ENS, 128>, 128); break;
+            default: L(k2_force<ENS, 256>, 256); break;
+        }
+    }
     GC_LAUNCH_CHECK("engine force");
     mark(pr, s, "sdpd force");
     k2_bump<<<1, 1, 0, s>>>(e->d_ctr);
@@ -732,7 +770,8 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
             e->d_cell_rep, e->d_celltab, e->d_curpos, e->d_coor, e->d_nt[c], e->d_mpos);
         GC_LAUNCH_CHECK("engine membrane gather");
         // 64-thread groups: a few hundred RBCs give only ~Nm/256 groups of 256
-        k_def_force<<<xcd_grid(blocks_for(e->nm, 64)), 64, 0, s>>>(P, e->nm, e->d_bbox, a);
+        const int dg = 64 << e->tune.v[TUNE_MEMBRANE];   // 64, 128 or 256 threads
+        k_def_force<<<xcd_grid(blocks_for(e->nm, dg)), dg, 0, s>>>(P, e->nm, e->d_bbox, a);
         GC_LAUNCH_CHECK("engine membrane force");
         mark(pr, s, "membrane force+corrector");
     }
@@ -949,7 +988,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
                     tfloat3 v = d.vel ? d.vel[l] : tfloat3{0.f, 0.f, 0.f};
                     if (t == 0) v = tfloat3{0.f, 0.f, 0.f};
                     vout_id[g] = make_float4(v.x, v.y, v.z, d.consts.DenInit);
-                    vh_id[g] = v;
+                    vh_id[g] = t < 2 ? tfloat3{0.f, 0.f, 0.f} : v;
                     rep_of[g] = r;
                 }
         }
@@ -1234,6 +1273,38 @@ extern "C" int gc_EnsembleGetState(gc_engine* e, int rep, gc_host_state* o)
     return export_state(e, rep, o);
 }
 
+namespace {
+
+// Loads a state given in global id order (positions AoS) and re-sorts.
+int restore_state(gc_engine* e, const std::vector<double>& pos, const std::vector<float4>& vo,
+                  const std::vector<tfloat3>& f, const std::vector<tfloat3>& vh,
+                  const std::vector<tint3>& nt, long step, unsigned long long ctr)
+{
+    const int N = e->N;
+    if (e->gexec) { (void)hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
+    std::vector<double> pos_soa(3 * (size_t)N);
+    for (int i = 0; i < N; ++i)
+        for (int d = 0; d < 3; ++d) pos_soa[(size_t)d * N + i] = pos[3 * (size_t)i + d];
+    std::vector<int> idn(N);
+    for (int i = 0; i < N; ++i) idn[i] = i;
+    e->cur = 0;
+    hipStream_t s = e->s;
+    int st;
+    if ((st = upload(e->d_pos64[0], pos_soa.data(), 3 * (size_t)N, s)) || (st = upload(e->d_vout, vo.data(), N, s)) ||
+        (st = upload(e->d_force, f.data(), N, s)) || (st = upload(e->d_nt[0], nt.data(), N, s)) ||
+        (st = upload(e->d_sp[0], idn.data(), N, s)) || (st = upload(e->d_curpos, idn.data(), N, s)) ||
+        (st = upload(e->d_ctr, &ctr, 1, s)))
+        return st;
+    if ((st = launch_build(e, 0, 0))) return st;
+    if ((st = reorder_carried(e, vo, f, vh))) return st;
+    e->walls_dirty = false;
+    e->step = step;
+    if (e->ms && (st = ms_reset(e))) return st;
+    return GC_OK;
+}
+
+}  // namespace
+
 extern "C" int gc_EngineSetState(gc_engine* e, const gc_host_state* in, long step,
                                  uint64_t noise_counter)
 {
@@ -1242,7 +1313,6 @@ extern "C" int gc_EngineSetState(gc_engine* e, const gc_host_state* in, long ste
     const int N = e->N;
     int st = hip_check(hipStreamSynchronize(e->s), "SetState sync");
     if (st) return st;
-    if (e->gexec) { (void)hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
     Snapshot S;
     if ((st = snapshot(e, S))) return st;
     std::vector<double> pos = S.pos;
@@ -1256,25 +1326,7 @@ extern "C" int gc_EngineSetState(gc_engine* e, const gc_host_state* in, long ste
     if (in->velhalf) memcpy(vh.data(), in->velhalf, sizeof(tfloat3) * N);
     if (in->force) memcpy(f.data(), in->force, sizeof(tfloat3) * N);
     if (in->nthrobound) memcpy(nt.data(), in->nthrobound, sizeof(tint3) * N);
-    std::vector<double> pos_soa(3 * (size_t)N);
-    for (int i = 0; i < N; ++i)
-        for (int d = 0; d < 3; ++d) pos_soa[(size_t)d * N + i] = pos[3 * (size_t)i + d];
-    std::vector<int> idn(N);
-    for (int i = 0; i < N; ++i) idn[i] = i;
-    e->cur = 0;
-    hipStream_t s = e->s;
-    unsigned long long ctr = noise_counter;
-    if ((st = upload(e->d_pos64[0], pos_soa.data(), 3 * (size_t)N, s)) || (st = upload(e->d_vout, vo.data(), N, s)) ||
-        (st = upload(e->d_force, f.data(), N, s)) || (st = upload(e->d_nt[0], nt.data(), N, s)) ||
-        (st = upload(e->d_sp[0], idn.data(), N, s)) || (st = upload(e->d_curpos, idn.data(), N, s)) ||
-        (st = upload(e->d_ctr, &ctr, 1, s)))
-        return st;
-    if ((st = launch_build(e, 0, 0))) return st;
-    if ((st = reorder_carried(e, vo, f, vh))) return st;
-    e->walls_dirty = false;
-    e->step = step;
-    if (e->ms && (st = ms_reset(e))) return st;
-    return GC_OK;
+    return restore_state(e, pos, vo, f, vh, nt, step, noise_counter);
 }
 
 extern "C" int gc_EngineGetCells(gc_engine* e, gc_host_cells* o)
@@ -1336,6 +1388,108 @@ extern "C" int gc_EngineProfile(gc_engine* e, int nsteps, int driveon, gc_engine
         out->us[m] = (float)(acc[k] * 1000.0 / nsteps);
     }
     out->nkernels = m;
+    return GC_OK;
+}
+
+namespace {
+
+const char* const TUNE_PHASE[TUNE_N] = {"integrate+key", "density+wall velocity", "sdpd force",
+                                        "membrane force+corrector"};
+const char* const TUNE_FAMILY[TUNE_N] = {"integrate", "density", "sdpd force", "membrane force"};
+
+void tune_shape(int f, int v, char* out, size_t n)
+{
+    static const int dbs[12] = {64, 64, 64, 64, 128, 128, 128, 128, 256, 256, 256, 256};
+    static const int dst[12] = {0, 128, 192, 320, 0, 128, 192, 320, 0, 128, 192, 320};
+    if (f == TUNE_DENSITY) snprintf(out, n, "block %d, stage %d", dbs[v], dst[v]);
+    else if (f == TUNE_MEMBRANE) snprintf(out, n, "group %d", 64 << v);
+    else snprintf(out, n, "block %d", 64 << v);
+}
+
+// Average time (us) of one phase over nsteps eager steps.
+int phase_us(gc_engine* e, int nsteps, int driveon, const char* name, float* us)
+{
+    Prof pr;
+    for (int k = 0; k < 32; ++k) (void)hipEventCreate(&pr.ev[k]);
+    double acc = 0.0;
+    int st = GC_OK;
+    for (int it = 0; it < nsteps && !st; ++it) {
+        pr.n = 0;
+        if ((st = launch_step(e, driveon, &pr))) break;
+        st = hip_check(hipEventSynchronize(pr.ev[pr.n - 1]), "autotune sync");
+        for (int k = 1; k < pr.n && !st; ++k)
+            if (!strcmp(pr.name[k], name)) {
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, pr.ev[k - 1], pr.ev[k]);
+                acc += ms;
+            }
+    }
+    for (int k = 0; k < 32; ++k) (void)hipEventDestroy(pr.ev[k]);
+    *us = (float)(acc * 1000.0 / (nsteps > 0 ? nsteps : 1));
+    return st;
+}
+
+}  // namespace
+
+extern "C" int gc_EngineAutotune(gc_engine* e, int steps, int driveon, gc_tune_report* out)
+{
+    if (!e || steps < 1) return set_error(GC_ERR_INVALID, "EngineAutotune: bad arguments");
+    if (e->ms) return set_error(GC_ERR_INVALID, "EngineAutotune: membrane substepping active");
+    int st = hip_check(hipStreamSynchronize(e->s), "Autotune sync");
+    if (st) return st;
+    // the state is restored afterwards: tuning leaves the trajectory untouched
+    Snapshot S;
+    if ((st = snapshot(e, S))) return st;
+    const long step0 = e->step;
+    unsigned long long ctr0 = 0;
+    if ((st = hip_check(hipMemcpy(&ctr0, e->d_ctr, sizeof(ctr0), hipMemcpyDeviceToHost), "Autotune ctr")))
+        return st;
+    if (out) memset(out, 0, sizeof(*out));
+    int row = 0;
+    for (int f = 0; f < TUNE_N && !st; ++f) {
+        if (f == TUNE_MEMBRANE && e->ncell == 0) continue;
+        float best = 3.0e38f;
+        int bestv = e->tune.v[f];
+        const int row0 = row;
+        for (int v = 0; v < TUNE_NVARIANT[f] && !st; ++v) {
+            e->tune.v[f] = v;
+            float us = 0.f;
+            if ((st = launch_step(e, driveon, nullptr))) break;   // warm-up (code load)
+            if ((st = phase_us(e, steps, driveon, TUNE_PHASE[f], &us))) break;
+            if (us < best) { best = us; bestv = v; }
+            if (out && row < 64) {
+                snprintf(out->family[row], sizeof(out->family[row]), "%s", TUNE_FAMILY[f]);
+                out->variant[row] = v;
+                tune_shape(f, v, out->shape[row], sizeof(out->shape[row]));
+                out->us[row] = us;
+                ++row;
+            }
+        }
+        e->tune.v[f] = bestv;
+        if (out)
+            for (int r = row0; r < row; ++r) out->chosen[r] = out->variant[r] == bestv;
+    }
+    if (out) out->n = row;
+    int st2 = restore_state(e, S.pos, S.vo, S.f, S.vh, S.nt, step0, ctr0);
+    return st ? st : st2;
+}
+
+extern "C" int gc_EngineGetTuning(const gc_engine* e, int* v)
+{
+    if (!e || !v) return set_error(GC_ERR_INVALID, "EngineGetTuning: null");
+    for (int f = 0; f < TUNE_N; ++f) v[f] = e->tune.v[f];
+    return GC_OK;
+}
+
+extern "C" int gc_EngineSetTuning(gc_engine* e, const int* v)
+{
+    if (!e || !v) return set_error(GC_ERR_INVALID, "EngineSetTuning: null");
+    for (int f = 0; f < TUNE_N; ++f)
+        if (v[f] < 0 || v[f] >= TUNE_NVARIANT[f]) return set_error(GC_ERR_INVALID, "EngineSetTuning: variant out of range");
+    int st = hip_check(hipStreamSynchronize(e->s), "SetTuning sync");
+    if (st) return st;
+    if (e->gexec) { (void)hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
+    for (int f = 0; f < TUNE_N; ++f) e->tune.v[f] = v[f];
     return GC_OK;
 }
 

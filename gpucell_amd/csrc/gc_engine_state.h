@@ -32,6 +32,14 @@ struct RepMeta {
     Structconst consts;                          // its own id ranges
 };
 
+// Launch shapes picked by the autotuner (gc_EngineAutotune): one variant
+// index per kernel family; every variant computes bit-identical results.
+enum { TUNE_INTEGRATE = 0, TUNE_DENSITY = 1, TUNE_FORCE = 2, TUNE_MEMBRANE = 3, TUNE_N = 4 };
+struct TuneCfg {
+    int v[TUNE_N] = {2, 10, 2, 0};   // integrate 256, density 256 x 192, force 256, membrane 64
+};
+constexpr int TUNE_NVARIANT[TUNE_N] = {3, 12, 3, 3};
+
 // Membrane substepping / surrogate state (gc_substep.hip); NULL while the
 // membrane moves with the fluid step.
 struct MembraneSub;
@@ -73,6 +81,7 @@ struct gc_engine {
     bool walls_dirty = false;
     bool any_morse = false, any_walls = false, any_bound1 = false;
     gc::MembraneSub* ms = nullptr;   // membrane substepping (mode 1/2)
+    gc::TuneCfg tune;
 
     template <class T> int alloc(T** p, size_t n)
     {

@@ -407,7 +407,8 @@ int ms_force(gc_engine* e, MembraneSub* m)
     a.V0c = e->d_refvol_para; a.es = e->d_es; a.ee = e->d_ee; a.ts = e->d_ts; a.te = e->d_te;
     a.defforce = m->fm; a.force = nullptr;
     a.mpos = m->mx; a.nm = m->nm;
-    k_def_force<<<xcd_grid(blocks_for(m->nm, 64)), 64, 0, s>>>(m->Pm, m->nm, e->d_bbox, a);
+    const int dg = 64 << e->tune.v[TUNE_MEMBRANE];
+    k_def_force<<<xcd_grid(blocks_for(m->nm, dg)), dg, 0, s>>>(m->Pm, m->nm, e->d_bbox, a);
     GC_LAUNCH_CHECK("membrane substep force");
     return GC_OK;
 }

@@ -123,6 +123,26 @@ class Engine:
                   "EngineProfile")
         return {p.names[k].value.decode(): float(p.us[k]) for k in range(p.nkernels)}
 
+    # --- autotuner ---------------------------------------------------------
+    def autotune(self, steps: int = 5, driveon: int = 1) -> list:
+        """Times every launch-shape variant, keeps the fastest per kernel
+        family and restores the state; returns the timing table."""
+        r = abi.gc_tune_report()
+        abi.check(self.lib.gc_EngineAutotune(self.h, int(steps), int(driveon), C.byref(r)),
+                  "EngineAutotune")
+        return [{"family": r.family[k].value.decode(), "variant": r.variant[k],
+                 "shape": r.shape[k].value.decode(), "us": float(r.us[k]), "chosen": bool(r.chosen[k])}
+                for k in range(r.n)]
+
+    def tuning(self) -> list:
+        v = (C.c_int * 4)()
+        abi.check(self.lib.gc_EngineGetTuning(self.h, v), "EngineGetTuning")
+        return list(v)
+
+    def set_tuning(self, v):
+        a = (C.c_int * 4)(*[int(x) for x in v])
+        abi.check(self.lib.gc_EngineSetTuning(self.h, a), "EngineSetTuning")
+
     # --- membrane time stepping (include/gpucell/surrogate.h) ------------
     def set_membrane_mode(self, mode: int, k: int = 8, mlp=None, norm=None, threshold: float = 0.0):
         """0: membrane moves with the fluid step; 1: k plain substeps;

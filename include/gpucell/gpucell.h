@@ -327,6 +327,27 @@ int gc_EngineWriteTecplot(gc_engine* e, const char* dir, long stepdata, long ste
 int gc_EngineSave(gc_engine* e, const char* path);
 int gc_EngineLoad(gc_engine* e, const char* path);
 
+/* ---- autotuner (no reference counterpart) -----------------------------
+ * Launch shapes of the hot kernels: integrate (block 64/128/256), density
+ * (block 64/128/256 x LDS row stage 0/128/192/320 particles per wave), SDPD
+ * force (block 64/128/256), membrane force (group 64/128/256).  Every shape
+ * computes bit-identical results; only the time differs.  Autotune times
+ * each variant over `steps` eager steps with HIP events on the engine
+ * stream, keeps the fastest per family, and restores the state it started
+ * from (the trajectory is unchanged).  One report row per variant. */
+typedef struct {
+    int n;
+    char family[64][24];
+    int variant[64];
+    char shape[64][32];
+    float us[64];
+    int chosen[64];
+} gc_tune_report;
+int gc_EngineAutotune(gc_engine* e, int steps, int driveon, gc_tune_report* out);
+/* Current variant per family (4 ints: integrate, density, force, membrane). */
+int gc_EngineGetTuning(const gc_engine* e, int* variants);
+int gc_EngineSetTuning(gc_engine* e, const int* variants);
+
 /* ---- batched ensembles (no reference counterpart) ---------------------
  * One engine over nrep independent suspensions of any sizes and physics
  * (parameter sweeps, ensemble statistics): the same kernels and graph step
