@@ -82,6 +82,38 @@ def cpu_baseline(case, seconds_target=15.0):
                       f"{threads} OpenMP threads, after 1 untimed step)"}
 
 
+SURROGATE_THRESHOLD = 2e-5   # residual of the Verlet relation, in rest edge lengths
+
+
+def surrogate_leg(case, case_name):
+    """Learned membrane substepper on this suspension: ~1e6 training rows
+    from plain 8-substep trajectories of the 32^3 one-RBC box and of this
+    case, training with the library's BF16-MFMA kernels, then surrogate vs
+    plain substepping from the same state (wall time per step at equal dt,
+    deformation index and cell volume against plain)."""
+    from gpucell_amd import casegen, substepper
+    import torch
+    t0 = time.perf_counter()
+    box = casegen.periodic_box()
+    X1, Y1 = substepper.collect([box], steps=150, every=1, warmup=5)
+    X2, Y2 = substepper.collect([case], steps=56, every=4, warmup=5,
+                                max_rows=14 * case.nCellPar)
+    X, Y = torch.cat([X1, X2]), torch.cat([Y1, Y2])
+    t_data = time.perf_counter() - t0
+    log(f"[bench] surrogate: {X.shape[0]} training rows in {t_data:.1f}s")
+    net, norm, loss, t_train = substepper.train(X, Y, epochs=8, batch=4096, lr=2e-3)
+    log(f"[bench] surrogate: trained in {t_train:.1f}s, loss {loss[0]:.4g} -> {loss[-1]:.4g}")
+    r = substepper.compare(case, net, norm, SURROGATE_THRESHOLD, steps=40)
+    log("[bench] surrogate vs plain: " + json.dumps(r))
+    r.update({"metric": "simulated time per wall-second, surrogate / plain substepping "
+                        "(k=8, equal dt)",
+              "case": case_name, "threshold": SURROGATE_THRESHOLD,
+              "train_rows": int(X.shape[0]), "train_epochs": 8,
+              "train_seconds": t_train, "data_seconds": t_data,
+              "loss_first_epoch": float(loss[0]), "loss_last_epoch": float(loss[-1])})
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +122,8 @@ def main():
     ap.add_argument("--case", default="suspension_1m")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--eager", action="store_true", help="launch kernels without a graph")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the learned-substepper leg (training + surrogate vs plain)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (no autotuning before the timed run)")
     a = ap.parse_args()
@@ -166,6 +200,9 @@ def main():
         "tuning": None if tune_rows is None else {
             r["family"]: r["shape"] for r in tune_rows if r["chosen"]},
     }
+    eng.close()
+    if not a.no_surrogate and case.nCell > 0:
+        out["surrogate"] = surrogate_leg(case, a.case)
     if not a.no_cpu:
         log("[bench] CPU baseline (FP64 oracle)")
         out["cpu_baseline"] = cpu_baseline(case)
