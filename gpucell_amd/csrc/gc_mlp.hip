@@ -41,8 +41,11 @@ constexpr int LDI = INP + 8;   // LDS row strides in BF16 elements (16-byte mult
 constexpr int LDW = H + 8;     // one extra 16-byte chunk against bank conflicts)
 constexpr int NPARAM = H * IN + H + 2 * (H * H + H) + OUT * H + OUT;
 constexpr int MLP_BS = 256;
+// fused forward: 8 waves (two per SIMD) share the weights in LDS; each wave
+// keeps its 16 rows' activations in one buffer, overwritten layer by layer
+constexpr int FWD_BS = 512;
 constexpr size_t FUSED_LDS = sizeof(uint16_t) * ((size_t)H * LDI + 2 * (size_t)H * LDW +
-                                                 (size_t)OUTP * LDW + 4 * 2 * 16 * (size_t)LDW);
+                                                 (size_t)OUTP * LDW + (FWD_BS / 64) * 16 * (size_t)LDW);
 
 // Offsets of the blocks of the flat parameter vector.
 constexpr int OW1 = 0, OB1 = OW1 + H * IN, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 + H,
@@ -50,6 +53,7 @@ constexpr int OW1 = 0, OB1 = OW1 + H * IN, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3
 static_assert(OB4 + OUT == NPARAM, "parameter layout");
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 
 // FP32 -> BF16, round to nearest even (quiet NaN kept)
@@ -77,37 +81,71 @@ __device__ inline float tanh_hw(float z)
     return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
 }
 
+// LDS written by some lanes of a wave, then read (or overwritten) by others
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 struct MlpDev {
     const uint16_t *w1, *w2, *w3, *w4;   // [H][INP], [H][H], [H][H], [OUTP][H] (zero padded)
     const float *b1, *b2, *b3, *b4;      // b4 padded to OUTP
 };
 
-// rows x cols BF16 matrix (cols multiple of 8) from global to LDS with stride ld
+// rows x cols BF16 matrix (cols multiple of 8) from global to LDS with
+// stride ld; eight 16-byte loads in flight per thread before their stores
+// (one at a time, each would wait out a full memory latency)
 __device__ inline void stage(uint16_t* dst, int ld, const uint16_t* __restrict__ src, int rows, int cols)
 {
-    const int chunks = cols / 8;
-    for (int c = threadIdx.x; c < rows * chunks; c += blockDim.x) {
-        int r = c / chunks, q = c % chunks;
-        *(uint4*)(dst + r * ld + 8 * q) = *(const uint4*)(src + (size_t)r * cols + 8 * q);
+    const int chunks = cols / 8, total = rows * chunks;
+    for (int c0 = threadIdx.x; c0 < total; c0 += 4 * FWD_BS) {
+        uint4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = min(c0 + u * FWD_BS, total - 1);
+            v[u] = *(const uint4*)(src + 8 * (size_t)c);   // row-major: chunk c at 8 c
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int c = c0 + u * FWD_BS;
+            if (c < total) *(uint4*)(dst + (c / chunks) * ld + 8 * (c % chunks)) = v[u];
+        }
     }
 }
 
-// One layer for the 16 samples of a wave: acc[nt] = in[16][K] * W[nt*16 .. +15][K]^T
+// One layer for the 16 samples of a wave, computed transposed: tile t is
+// D = W[t*16 .. +15][K] * in[16][K]^T (weights as the A operand, samples as
+// B), so lane (g, i) receives neurons t*16 + 4g .. +3 of sample i: four
+// adjacent outputs of one row, written back with one 8-byte store.  All
+// activation fragments are read first (the output may then overwrite `in`),
+// and each k-step's weight fragments are issued before its MFMAs, so the
+// matrix cores do not wait out one LDS latency per instruction.
 template <int K, int NT>
 __device__ inline void wave_layer(const uint16_t* in, int ldi, const uint16_t* w, int ldw, f32x4* acc)
 {
     const int lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
+    uint4 x[K / 32];
+#pragma unroll
+    for (int kk = 0; kk < K / 32; ++kk) x[kk] = *(const uint4*)(in + li * ldi + 32 * kk + 8 * lg);
 #pragma unroll
     for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        uint4 a = *(const uint4*)(in + li * ldi + k0 + 8 * lg);
+    for (int kk = 0; kk < K / 32; ++kk) {
+        uint4 wf[NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            uint4 b = *(const uint4*)(w + (t * 16 + li) * ldw + k0 + 8 * lg);
-            acc[t] = mfma(a, b, acc[t]);
-        }
+        for (int t = 0; t < NT; ++t) wf[t] = *(const uint4*)(w + (t * 16 + li) * ldw + 32 * kk + 8 * lg);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = mfma(wf[t], x[kk], acc[t]);
     }
+}
+
+// four FP32 -> four BF16 (round to nearest even, v_cvt_pk_bf16_f32)
+__device__ inline uint2 pack4(float a, float b, float c, float d)
+{
+    bf16x2 lo = {(__bf16)a, (__bf16)b}, hi = {(__bf16)c, (__bf16)d};
+    return make_uint2(__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi));
 }
 
 // bias + tanh, BF16 into out[16][LDW] (and optionally global [row][H])
@@ -115,23 +153,23 @@ __device__ inline void wave_tanh_out(const f32x4* acc, const float* __restrict__
                                      uint16_t* __restrict__ gout, int row0, int nrows)
 {
     const int lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
+    const int row = row0 + li;
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-        const int j = t * 16 + li;
-        const float b = bias[j];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            uint16_t v = f2bf(tanh_hw(acc[t][r] + b));
-            out[(4 * lg + r) * LDW + j] = v;
-            int row = row0 + 4 * lg + r;
-            if (gout && row < nrows) gout[(size_t)row * H + j] = v;
-        }
+        const int n0 = t * 16 + 4 * lg;
+        const float4 b = *(const float4*)(bias + n0);
+        uint2 v = pack4(tanh_hw(acc[t][0] + b.x), tanh_hw(acc[t][1] + b.y), tanh_hw(acc[t][2] + b.z),
+                        tanh_hw(acc[t][3] + b.w));
+        *(uint2*)(out + li * LDW + n0) = v;
+        if (gout && row < nrows) *(uint2*)(gout + (size_t)row * H + n0) = v;
     }
 }
 
 // Fused forward.  Inference: y = out[B][ycols] (ycols = 24, ldy = 24).
 // Training: also xb [Bp][INP], h1..h3 [Bp][H] (BF16) and y as [Bp][OUTP].
-__global__ __launch_bounds__(MLP_BS) void k_mlp_fused(int B, const float* __restrict__ X, MlpDev w,
+// Waves run their tiles independently (no block barrier after the weights
+// are staged).
+__global__ __launch_bounds__(FWD_BS) void k_mlp_fused(int B, const float* __restrict__ X, MlpDev w,
     float* __restrict__ Y, int ycols, int ldy, uint16_t* __restrict__ xb, uint16_t* __restrict__ h1,
     uint16_t* __restrict__ h2, uint16_t* __restrict__ h3)
 {
@@ -145,43 +183,52 @@ __global__ __launch_bounds__(MLP_BS) void k_mlp_fused(int B, const float* __rest
     stage(s2, LDW, w.w2, H, H);
     stage(s3, LDW, w.w3, H, H);
     stage(s4, LDW, w.w4, OUTP, H);
+    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
-    uint16_t* bufA = sa + wave * 2 * 16 * LDW;
-    uint16_t* bufB = bufA + 16 * LDW;
+    const int nwaves = FWD_BS / 64;
+    uint16_t* buf = sa + wave * 16 * LDW;
     const int nrows = xb ? ((B + 255) / 256) * 256 : B;   // training rows are padded
-    const int ntiles = (nrows + 63) / 64;
+    const int nchunks = (nrows + 15) / 16;                // 16 rows per wave task
     f32x4 acc[8];
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const int r0 = tile * 64 + wave * 16;
-        __syncthreads();  // weights staged / previous tile's buffers free
-        for (int e = lane; e < 16 * INP; e += 64) {
-            int i = e / INP, k = e % INP, row = r0 + i;
-            float v = (row < B && k < IN) ? X[(size_t)row * IN + k] : 0.0f;
-            uint16_t q = f2bf(v);
-            bufA[i * LDW + k] = q;
-            if (xb && row < nrows) xb[(size_t)row * INP + k] = q;
+    for (int task = blockIdx.x * nwaves + wave; task < nchunks; task += gridDim.x * nwaves) {
+        const int r0 = task * 16;
+        {   // the wave's 16 input rows (lane = column): loads issued first
+            const float* xp = X + (size_t)r0 * IN + lane;
+            const bool col = lane < IN;
+            float v[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) v[i] = (col && r0 + i < B) ? xp[i * IN] : 0.0f;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const uint16_t q = __builtin_bit_cast(uint16_t, (__bf16)v[i]);
+                buf[i * LDW + lane] = q;
+                if (xb && r0 + i < nrows) xb[(size_t)(r0 + i) * INP + lane] = q;
+            }
         }
-        __syncthreads();
-        wave_layer<INP, 8>(bufA, LDW, s1, LDI, acc);
-        wave_tanh_out(acc, w.b1, bufB, h1, r0, nrows);
-        __syncthreads();
-        wave_layer<H, 8>(bufB, LDW, s2, LDW, acc);
-        wave_tanh_out(acc, w.b2, bufA, h2, r0, nrows);
-        __syncthreads();
-        wave_layer<H, 8>(bufA, LDW, s3, LDW, acc);
-        wave_tanh_out(acc, w.b3, bufB, h3, r0, nrows);
-        __syncthreads();
-        wave_layer<H, 2>(bufB, LDW, s4, LDW, acc);
+        wave_lds_sync();
+        wave_layer<INP, 8>(buf, LDW, s1, LDI, acc);
+        wave_lds_sync();
+        wave_tanh_out(acc, w.b1, buf, h1, r0, nrows);
+        wave_lds_sync();
+        wave_layer<H, 8>(buf, LDW, s2, LDW, acc);
+        wave_lds_sync();
+        wave_tanh_out(acc, w.b2, buf, h2, r0, nrows);
+        wave_lds_sync();
+        wave_layer<H, 8>(buf, LDW, s3, LDW, acc);
+        wave_lds_sync();
+        wave_tanh_out(acc, w.b3, buf, h3, r0, nrows);
+        wave_lds_sync();
+        wave_layer<H, 2>(buf, LDW, s4, LDW, acc);
+        wave_lds_sync();
+        const int row = r0 + li;
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-            const int j = t * 16 + li;
-            if (j >= ycols) continue;
-            const float b = w.b4[j];
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                int row = r0 + 4 * lg + r;
-                if (row < nrows) Y[(size_t)row * ldy + j] = acc[t][r] + b;
-            }
+            const int j0 = t * 16 + 4 * lg;
+            if (j0 >= ycols || row >= nrows) continue;   // ycols is a multiple of 4
+            const float4 b = *(const float4*)(w.b4 + j0);
+            float* yp = Y + (size_t)row * ldy + j0;
+            yp[0] = acc[t][0] + b.x; yp[1] = acc[t][1] + b.y;
+            yp[2] = acc[t][2] + b.z; yp[3] = acc[t][3] + b.w;
         }
     }
 }
@@ -484,9 +531,9 @@ int launch_fused(gc_mlp* m, int B, const float* x, float* y, int ycols, int ldy,
                  hipStream_t s)
 {
     int nrows = train ? ((B + 255) / 256) * 256 : B;
-    int grid = std::min(fused_grid(), (nrows + 63) / 64);
+    int grid = std::min(fused_grid(), (nrows + 16 * (FWD_BS / 64) - 1) / (16 * (FWD_BS / 64)));
     if (grid < 1) return GC_OK;
-    k_mlp_fused<<<grid, MLP_BS, FUSED_LDS, s>>>(B, x, m->dev(), y, ycols, ldy, train ? m->xb : nullptr,
+    k_mlp_fused<<<grid, FWD_BS, FUSED_LDS, s>>>(B, x, m->dev(), y, ycols, ldy, train ? m->xb : nullptr,
         train ? m->hh[0] : nullptr, train ? m->hh[1] : nullptr, train ? m->hh[2] : nullptr);
     GC_LAUNCH_CHECK("mlp forward");
     return GC_OK;
