@@ -122,6 +122,9 @@ def main():
     ap.add_argument("--case", default="suspension_1m")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--eager", action="store_true", help="launch kernels without a graph")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 256 MiB device copy three times (the reference kernel "
+                         "for scaling FETCH_SIZE / WRITE_SIZE counters to bytes) and exit")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the learned-substepper leg (training + surrogate vs plain)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -134,6 +137,14 @@ def main():
         sys.exit(2)
 
     import torch
+    if a.calibrate_copy:
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copied {src.numel() * 4} bytes x 3")
+        return
     from gpucell_amd.engine import Engine
 
     log(f"[bench] generating case {a.case}")
