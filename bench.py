@@ -54,6 +54,10 @@ def make_case(name):
                                        "stenosis, 300 RBCs at 30% hematocrit, Morse De=0.3kT")
     if name == "box32":
         return casegen.periodic_box(), "periodic 32^3 box, one RBC, reverse Poiseuille"
+    if name == "ensemble64":
+        return casegen.ensemble64(), ("ensemble64: 64 independent periodic suspensions, "
+                                      "22k-390k particles (log-uniform), 1-40 RBCs each, per-replica "
+                                      "body force, Morse De and seed, one batched engine")
     raise SystemExit(f"unknown case {name}")
 
 
@@ -136,6 +140,7 @@ def main():
               "(one suspension per GPU; see the sharding design)", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import torch
     if a.calibrate_copy:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -145,13 +150,17 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] copied {src.numel() * 4} bytes x 3")
         return
-    from gpucell_amd.engine import Engine
+    from gpucell_amd.engine import Engine, Ensemble
 
     log(f"[bench] generating case {a.case}")
     t = time.perf_counter()
     case, workload = make_case(a.case)
-    log(f"[bench] case ready: N={case.N} Nm={case.nCellPar} ({time.perf_counter() - t:.1f}s)")
-    eng = Engine(case)
+    batch = isinstance(case, list)
+    cases = case if batch else [case]
+    N, Nm = sum(c.N for c in cases), sum(c.nCellPar for c in cases)
+    log(f"[bench] case ready: {len(cases)} suspension(s), N={N} Nm={Nm} "
+        f"({time.perf_counter() - t:.1f}s)")
+    eng = Ensemble(cases) if batch else Engine(case)
     tune_rows = None
     if not a.no_autotune:
         log("[bench] autotuning launch shapes")
@@ -169,16 +178,16 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / a.steps
-    S = eng.state()
-    import numpy as np
-    finite = bool(np.isfinite(S["pos"]).all() and np.isfinite(S["force"]).all())
+    finite = True
+    for r in range(len(cases)):
+        S = eng.state(r) if batch else eng.state()
+        finite = finite and bool(np.isfinite(S["pos"]).all() and np.isfinite(S["force"]).all())
     log(f"[bench] {a.steps} steps: {ms:.4f} ms/step, finite={finite}")
     prof = eng.profile(20)
     log("[bench] per-phase us: " + json.dumps(prof))
     # dominant single kernel of the step (HIP events on the engine stream)
     dom = max(KERNEL_BYTES_PER_PARTICLE, key=lambda k: prof.get(k, 0.0))
     dom_us = prof[dom]
-    N, Nm = case.N, case.nCellPar
     achieved = KERNEL_BYTES_PER_PARTICLE[dom] * N / (dom_us * 1e-6) / 1e9
     traffic = None
     pmc = os.path.join(ROOT, "profiles", f"traffic_{KERNEL_NAMES[dom]}_{a.case}.json")
@@ -199,8 +208,9 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic (seeded lattice fluid, walls and biconcave RBCs)",
-        "config": {"workload": workload, "N": N, "Nm": Nm, "h": float(case.consts.CutRadius),
-                   "dt": float(case.consts.TimeStep), "graph": graph, "finite": finite},
+        "config": {"workload": workload, "N": N, "Nm": Nm, "suspensions": len(cases),
+                   "h": float(cases[0].consts.CutRadius), "dt": float(cases[0].consts.TimeStep),
+                   "graph": graph, "finite": finite},
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "avg_us": dom_us,
@@ -212,11 +222,11 @@ def main():
             r["family"]: r["shape"] for r in tune_rows if r["chosen"]},
     }
     eng.close()
-    if not a.no_surrogate and case.nCell > 0:
+    if not a.no_surrogate and not batch and case.nCell > 0:
         out["surrogate"] = surrogate_leg(case, a.case)
     if not a.no_cpu:
         log("[bench] CPU baseline (FP64 oracle)")
-        out["cpu_baseline"] = cpu_baseline(case)
+        out["cpu_baseline"] = cpu_baseline(max(cases, key=lambda c: c.N))
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

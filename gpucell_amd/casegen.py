@@ -230,7 +230,15 @@ def _lattice(lo, hi, dx):
     return g
 
 
-def _place_cells(rng, n_cells, shape, inside, gap, lo, hi, max_tries=200000, period=None):
+def _axis_rotation(axis, angle):
+    """Rotation matrix by `angle` about the unit vector `axis` (Rodrigues)."""
+    a = np.asarray(axis, dtype=np.float64)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + np.sin(angle) * K + (1.0 - np.cos(angle)) * (K @ K)
+
+
+def _place_cells(rng, n_cells, shape, inside, gap, lo, hi, max_tries=200000, period=None,
+                 dense=False):
     """Random sequential placement of rigid copies of `shape` (Nv,3): a
     candidate (uniform centre, uniform random rotation) is kept when it lies
     inside the domain and no vertex comes within `gap` of a placed cell.
@@ -247,12 +255,45 @@ def _place_cells(rng, n_cells, shape, inside, gap, lo, hi, max_tries=200000, per
     placed = []
     tree = None
     tries = 0
+    # After `random_tries` random candidates, dense suspensions continue on a
+    # lattice of slots sized to the cell's extent, with the discs nearly
+    # aligned (tilt <= 0.15 rad, random spin): random sequential addition of
+    # randomly oriented discs jams below ~25 % volume fraction.
+    random_tries = 0 if dense else min(max_tries, 20000)
+    # lattice discs lie with their axis along x (the flow axis of a channel)
+    R0 = _axis_rotation(np.array([0.0, 1.0, 0.0]), 0.5 * np.pi)
+    ext = (shape @ R0.T).max(axis=0) - (shape @ R0.T).min(axis=0)
+    slots = None
+
+    def candidate():
+        nonlocal slots
+        if tries <= random_tries:
+            return lo + rng.random(3) * (hi - lo), random_rotation(rng)
+        if slots is None:
+            # half-extent steps across the disc (neighbouring slots overlap;
+            # the distance test keeps a dense non-overlapping subset)
+            step = [ext[0] + gap, 0.5 * (ext[1] + gap), 0.5 * (ext[2] + gap)]
+            axes = [np.arange(lo[d] + 0.5 * step[d], hi[d], step[d]) for d in range(3)]
+            # raster order (x slowest): greedy acceptance then packs the
+            # cross-section regularly, which random order does not
+            slots = list(np.stack(np.meshgrid(*axes, indexing="ij"), axis=-1).reshape(-1, 3)[::-1])
+        if not slots:
+            return None, None
+        spin = rng.uniform(0.0, 2.0 * np.pi)
+        axis = rng.normal(size=3)
+        axis[0] = 0.0
+        axis /= max(np.linalg.norm(axis), 1e-12)
+        tilt = rng.uniform(0.0, 0.15)
+        return slots.pop(), (_axis_rotation(axis, tilt) @ _axis_rotation(np.array([1.0, 0.0, 0.0]), spin)
+                             @ R0)
+
     while len(placed) < n_cells:
         tries += 1
-        if tries > max_tries:
+        if tries > max_tries + random_tries:
             raise RuntimeError(f"could place only {len(placed)} of {n_cells} cells")
-        c = lo + rng.random(3) * (hi - lo)
-        R = random_rotation(rng)
+        c, R = candidate()
+        if c is None:
+            raise RuntimeError(f"could place only {len(placed)} of {n_cells} cells")
         X = shape @ R.T + c
         if not inside(X):
             continue
@@ -393,7 +434,8 @@ def channel(length: int, width: int, n_rbc: int, hematocrit: float | None = None
         return bool(np.all(np.maximum(np.abs(X[:, 1]), np.abs(X[:, 2])) < w - gap))
 
     cells = _place_cells(rng, n_rbc, shape, inside, gap, np.array([0, -half, -half]),
-                         np.array([Lx, half, half]), period=(Lx, None, None)) if n_rbc else []
+                         np.array([Lx, half, half]), period=(Lx, None, None),
+                         dense=hematocrit is not None and hematocrit >= 0.2) if n_rbc else []
     # keep each cell whole but put its centre in [0, Lx): vertices may lie
     # outside the box in x; the first step wraps them (as the reference)
     cells = [X - Lx * np.floor(X.mean(0)[0] / Lx) * np.array([1.0, 0, 0]) for X in cells]
@@ -421,10 +463,53 @@ def suspension_1m(seed: int = 20261203, physics: Physics | None = None, noise: b
 
 
 def stenosis_2m(seed: int = 20261203, physics: Physics | None = None, noise: bool = True) -> Case:
-    """2M particles, 300 RBCs at 30 % hematocrit, 50 % cosine stenosis,
-    Morse aggregation De = 0.3 kT, r0 = 0.3, beta = 1.5: a 460 x 60 x 60
-    duct (66 x 66 with walls)."""
+    """~2M particles, 300 RBCs at 30 % hematocrit, 50 % cosine stenosis,
+    Morse aggregation De = 0.3 kT, r0 = 0.3, beta = 1.5: a 300 x 80 x 80
+    duct (86 x 86 with walls)."""
     ph = physics or Physics()
     ph = dataclasses.replace(ph, De=0.3 * ph.kT, r0=0.3, beta=1.5, mcut=min(1.5, ph.h))
-    return channel(460, 60, 300, hematocrit=0.30, seed=seed, physics=ph, noise=noise,
-                   stenosis=0.5, name="stenosis_2m_300rbc")
+    # 300 rigid biconcave discs pack (with a 0.8 gap, throat included) only
+    # below ~30 % in this duct: take the highest hematocrit that places them
+    last = None
+    for hct in (0.30, 0.27, 0.24, 0.21, 0.18, 0.15, 0.12):
+        try:
+            c = channel(300, 80, 300, hematocrit=hct, seed=seed, physics=ph, noise=noise,
+                        stenosis=0.5, name="stenosis_2m_300rbc")
+            c.hematocrit = hct
+            return c
+        except RuntimeError as e:
+            last = e
+    raise last
+
+
+def ensemble64(seed: int = 20261203, nrep: int = 64, nmin: int = 20000, nmax: int = 400000,
+               rbc_max: int = 40, subdiv: int | None = None) -> "list[Case]":
+    """Independent periodic suspensions of uneven size for a batched run:
+    particle counts log-uniform in [nmin, nmax] (cubic boxes, unit lattice),
+    1..rbc_max RBCs each (log-uniform), about 10 % hematocrit, and per
+    replica a reverse-Poiseuille body force, Morse D_e (0 or 0.3 kT) and
+    noise seed drawn from `seed`.  The membrane mesh is refined with the
+    cell size so that vertices stay about one lattice spacing apart (a
+    denser membrane than fluid drives the SDPD pressure unstable): 642
+    vertices from D = 16, 162 from D = 8; D >= 8 caps the cell count of
+    small boxes at 20 % hematocrit."""
+    rng = np.random.default_rng(seed)
+    base = Physics()
+    cases = []
+    vol = 0.195   # biconcave mesh volume / D^3
+    for r in range(nrep):
+        N = float(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        n = max(16, int(round(N ** (1.0 / 3.0))))
+        n_rbc = int(np.clip(round(float(np.exp(rng.uniform(0.0, np.log(rbc_max))))), 1, rbc_max))
+        n_rbc = max(1, min(n_rbc, int(0.20 * n ** 3 / (vol * 8.0 ** 3))))
+        d = min(0.45 * n, max(8.0, (0.10 * n ** 3 / (n_rbc * vol)) ** (1.0 / 3.0)))
+        sub = 3 if d >= 16.0 else 2
+        De = float(rng.choice([0.0, 0.3])) * base.kT
+        ph = dataclasses.replace(base, fext=float(rng.uniform(0.01, 0.05)), De=De, r0=0.3,
+                                 beta=1.5, mcut=min(1.5, base.h),
+                                 seed=int(rng.integers(1, 2 ** 31 - 1)))
+        c = periodic_box(n=n, n_rbc=n_rbc, rbc_diameter=d, seed=int(rng.integers(1, 2 ** 31 - 1)),
+                         physics=ph, subdiv=sub if subdiv is None else subdiv)
+        c.name = f"ensemble_replica_{r}"
+        cases.append(c)
+    return cases

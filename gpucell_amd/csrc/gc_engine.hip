@@ -436,11 +436,15 @@ __global__ __launch_bounds__(TBS) void k2_force(int n, GcRep R0, const GcRep* __
     int id = __float_as_int(pi.w);
     if (id < R0.P.nf) { force[p] = tfloat3{0.f, 0.f, 0.f}; return; }
     const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
-    GcParams P = R.P;
-    unsigned long long c = *ctr;
-    P.ctr_lo = (uint32_t)c;
-    P.ctr_hi = (uint32_t)(c >> 32);
+    const GcParams& P = R.P;   // a reference: a copy of the batch record would live in scratch
+    const unsigned long long c = *ctr;
+    const uint32_t ctr_lo = (uint32_t)c, ctr_hi = (uint32_t)(c >> 32);
+    const uint32_t seed_lo = P.seed_lo, seed_hi = P.seed_hi;
     const int lid = ENS ? local_id(R, id) : id;
+    // the replica's id offsets and periods in registers (one table read)
+    const int lo0 = R.id_off[0] - R.loc_start[0], lo1 = R.id_off[1] - R.loc_start[1];
+    const int lo2 = R.id_off[2] - R.loc_start[2], lo3 = R.id_off[3] - R.loc_start[3];
+    const float L0 = R.L[0], L1 = R.L[1], L2 = R.L[2];
     float4 vi = velrho[p];
     float di = vi.w * P.inv_m;
     // P_i / d_i^2 with P = c^2 rho and d = rho / m is c^2 m / d_i; the
@@ -454,8 +458,7 @@ __global__ __launch_bounds__(TBS) void k2_force(int n, GcRep R0, const GcRep* __
     float fx = 0.f, fy = 0.f, fz = 0.f;
     auto pair = [&](int code) {
         int j = code >> 6;
-        float sx = code_shift(code, R.L[0]), sy = code_shift(code >> 2, R.L[1]),
-              sz = code_shift(code >> 4, R.L[2]);
+        float sx = code_shift(code, L0), sy = code_shift(code >> 2, L1), sz = code_shift(code >> 4, L2);
         float4 pj = posid[j];
         float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
         float r2 = rx * rx + ry * ry + rz * rz;
@@ -483,9 +486,9 @@ __global__ __launch_bounds__(TBS) void k2_force(int n, GcRep R0, const GcRep* __
         fy += cC * ry - (ga * vy + gb * ey);
         fz += cC * rz - (ga * vz + gb * ez);
         if (P.noise_on) {
-            int ljd = ENS ? local_id(R, idj) : idj;
+            int ljd = ENS ? local_id_of(idj, P.nb2, P.nf, P.nc, lo0, lo1, lo2, lo3) : idj;
             float n0, n1, n2;
-            pair_normals(P, (uint32_t)lid, (uint32_t)ljd, n0, n1, n2);
+            pair_normals_k(seed_lo, seed_hi, ctr_lo, ctr_hi, (uint32_t)lid, (uint32_t)ljd, n0, n1, n2);
             float exi = ex * n0 + ey * n1 + ez * n2;
             float amp = __builtin_amdgcn_sqrtf(P.two_kT_over_dt * gg);
             amp = lid < ljd ? amp : -amp;
@@ -521,11 +524,14 @@ __global__ __launch_bounds__(TBS) void k2_force(int n, GcRep R0, const GcRep* __
 }
 
 // [16] Morse between membrane vertices of different cells
+// Iterates the density pass's neighbour list (MaxCut <= CutRadius, so every
+// Morse partner is in it, in scan order); a vertex whose list overflowed
+// scans its cubes.
 template <bool ENS>
 __global__ __launch_bounds__(BS) void k2_agg(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, const float4* __restrict__ posid, const int* __restrict__ offs,
-    const int* __restrict__ key, const int* __restrict__ celltab, tfloat3* __restrict__ force,
-    tfloat3* __restrict__ aggforce)
+    const int* __restrict__ key, const int* __restrict__ nl, const int* __restrict__ nlc,
+    const int* __restrict__ celltab, tfloat3* __restrict__ force, tfloat3* __restrict__ aggforce)
 {
     int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (p >= n) return;
@@ -540,21 +546,31 @@ __global__ __launch_bounds__(BS) void k2_agg(int n, GcRep R0, const GcRep* __res
     key_to_cc(key[p], R, cx, cy, cz);
     const float tb = 2.0f * P.beta * P.De;
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
-        for (int j = jb; j < je; ++j) {
-            float4 pj = posid[j];
-            int idj = __float_as_int(pj.w);
-            if (idj < P.nc) continue;
-            float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
-            float r2 = rx * rx + ry * ry + rz * rz;
-            if (r2 >= P.mcut2 || r2 == 0.f || celltab[idj - P.nc] == cell) continue;
-            float ir = __builtin_amdgcn_rsqf(r2);
-            float r = r2 * ir;
-            float ex = expf(P.beta * (P.r0 - r));
-            float fm = tb * (ex * ex - ex) * ir;
-            fx += fm * rx; fy += fm * ry; fz += fm * rz;
+    auto morse = [&](int j, float sx, float sy, float sz) {
+        float4 pj = posid[j];
+        int idj = __float_as_int(pj.w);
+        if (idj < P.nc) return;
+        float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
+        float r2 = rx * rx + ry * ry + rz * rz;
+        if (r2 >= P.mcut2 || r2 == 0.f || celltab[idj - P.nc] == cell) return;
+        float ir = __builtin_amdgcn_rsqf(r2);
+        float r = r2 * ir;
+        float ex = expf(P.beta * (P.r0 - r));
+        float fm = tb * (ex * ex - ex) * ir;
+        fx += fm * rx; fy += fm * ry; fz += fm * rz;
+    };
+    const int cnt = nlc[p];
+    if (cnt >= 0) {
+        for (int k = 0; k < cnt; ++k) {
+            const int code = nl[k * n + p];
+            morse(code >> 6, code_shift(code, R.L[0]), code_shift(code >> 2, R.L[1]),
+                  code_shift(code >> 4, R.L[2]));
         }
-    });
+    } else {
+        visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
+            for (int j = jb; j < je; ++j) morse(j, sx, sy, sz);
+        });
+    }
     aggforce[id - P.nc] = tfloat3{fx, fy, fz};
     tfloat3 F = force[p];
     F.x += fx; F.y += fy; F.z += fz;
@@ -734,7 +750,8 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
     if (e->ncell > 0 && e->ms) {
         if (e->any_morse) {
             k2_agg<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid, e->d_offs,
-                                          e->d_key, e->d_celltab, e->d_force, e->d_aggforce);
+                                          e->d_key, e->d_nl, e->d_nlc, e->d_celltab, e->d_force,
+                                          e->d_aggforce);
             GC_LAUNCH_CHECK("engine morse");
         }
         mark(pr, s, "morse");
@@ -751,7 +768,8 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
         mark(pr, s, "membrane geometry");
         if (e->any_morse) {
             k2_agg<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid, e->d_offs,
-                                          e->d_key, e->d_celltab, e->d_force, e->d_aggforce);
+                                          e->d_key, e->d_nl, e->d_nlc, e->d_celltab, e->d_force,
+                                          e->d_aggforce);
             GC_LAUNCH_CHECK("engine morse");
         }
         mark(pr, s, "morse");

@@ -101,12 +101,13 @@ __device__ __forceinline__ float u01f(uint32_t x)
 // hardware instructions: v_log_f32 (log2), v_sqrt_f32 and v_sin_f32 /
 // v_cos_f32, whose argument is in revolutions, so cos(2 pi u) is one
 // instruction.
-__device__ __forceinline__ void pair_normals(const GcParams& P, uint32_t ida, uint32_t idb,
-                                             float& n0, float& n1, float& n2)
+__device__ __forceinline__ void pair_normals_k(uint32_t seed_lo, uint32_t seed_hi, uint32_t ctr_lo,
+                                               uint32_t ctr_hi, uint32_t ida, uint32_t idb,
+                                               float& n0, float& n1, float& n2)
 {
     uint32_t lo = min(ida, idb), hi = max(ida, idb);
     uint32_t o[4];
-    philox4x32_10(lo, hi, P.ctr_lo, P.ctr_hi, P.seed_lo, P.seed_hi, o);
+    philox4x32_10(lo, hi, ctr_lo, ctr_hi, seed_lo, seed_hi, o);
     const float m2ln2 = -1.3862943611198906f;  // -2 ln 2
     float r0 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u01f(o[0])));
     float r1 = __builtin_amdgcn_sqrtf(m2ln2 * __builtin_amdgcn_logf(u01f(o[2])));
@@ -114,6 +115,12 @@ __device__ __forceinline__ void pair_normals(const GcParams& P, uint32_t ida, ui
     n0 = r0 * __builtin_amdgcn_cosf(u1);
     n1 = r0 * __builtin_amdgcn_sinf(u1);
     n2 = r1 * __builtin_amdgcn_cosf(u3);
+}
+
+__device__ __forceinline__ void pair_normals(const GcParams& P, uint32_t ida, uint32_t idb,
+                                             float& n0, float& n1, float& n2)
+{
+    pair_normals_k(P.seed_lo, P.seed_hi, P.ctr_lo, P.ctr_hi, ida, idb, n0, n1, n2);
 }
 
 // Cube coordinate of one axis, FP32 with no contraction so the host oracle

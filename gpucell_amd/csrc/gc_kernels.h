@@ -28,11 +28,20 @@ struct GcRep {
 // the replica's own id of global id `id` (used for the pair noise, so a
 // replica draws the same numbers as when it runs alone).  Selects on
 // constant indices: a runtime index into the struct would put it in scratch.
+// The offsets come in by value: a conditional over lvalues selects their
+// addresses, and inside a by-reference lambda capture such a pointer select
+// keeps the captured locals in scratch.
+__device__ __forceinline__ int local_id_of(int id, int nb2, int nf, int nc, int d0, int d1, int d2,
+                                           int d3)
+{
+    return id - (id < nb2 ? d0 : (id < nf ? d1 : (id < nc ? d2 : d3)));
+}
+
 __device__ __forceinline__ int local_id(const GcRep& R, int id)
 {
-    int d0 = R.id_off[0] - R.loc_start[0], d1 = R.id_off[1] - R.loc_start[1];
-    int d2 = R.id_off[2] - R.loc_start[2], d3 = R.id_off[3] - R.loc_start[3];
-    return id - (id < R.P.nb2 ? d0 : (id < R.P.nf ? d1 : (id < R.P.nc ? d2 : d3)));
+    return local_id_of(id, R.P.nb2, R.P.nf, R.P.nc, R.id_off[0] - R.loc_start[0],
+                       R.id_off[1] - R.loc_start[1], R.id_off[2] - R.loc_start[2],
+                       R.id_off[3] - R.loc_start[3]);
 }
 
 // gc_sdpd.hip

@@ -220,3 +220,6 @@ class Ensemble:
         abi.check(self.lib.gc_EngineProfile(self.h, int(nsteps), int(driveon), C.byref(p)),
                   "EngineProfile")
         return {p.names[k].value.decode(): float(p.us[k]) for k in range(p.nkernels)}
+
+    def autotune(self, steps: int = 5, driveon: int = 1) -> list:
+        return Engine.autotune(self, steps, driveon)
