@@ -50,8 +50,15 @@ def make_case(name):
                                          "frozen wall layers), 100 RBCs x 642 vertices at 20% "
                                          "hematocrit, Poiseuille body force, aggregation off")
     if name == "stenosis_2m":
-        return casegen.stenosis_2m(), ("stenosis_2m_300rbc: 460x60x60 duct with 50% cosine "
-                                       "stenosis, 300 RBCs at 30% hematocrit, Morse De=0.3kT")
+        c = casegen.stenosis_2m()
+        return c, ("stenosis_2m_300rbc: 300x80x80 duct (86x86 with walls) with a 50% cosine "
+                   f"stenosis, 300 RBCs at {100 * c.hematocrit:.0f}% hematocrit (aligned lattice "
+                   "slots), Morse De=0.3kT, r0=0.3, beta=1.5")
+    if name == "suspension_10m":
+        c = casegen.suspension_10m()
+        return c, ("suspension_10m_2000rbc: 411x150x150 duct (156x156 with 3 frozen wall "
+                   f"layers), 2000 RBCs x 642 vertices at {100 * c.hematocrit:.0f}% hematocrit, "
+                   "Poiseuille body force, aggregation off")
     if name == "box32":
         return casegen.periodic_box(), "periodic 32^3 box, one RBC, reverse Poiseuille"
     if name == "ensemble64":
@@ -90,18 +97,21 @@ SURROGATE_THRESHOLD = 2e-5   # residual of the Verlet relation, in rest edge len
 
 
 def surrogate_leg(case, case_name):
-    """Learned membrane substepper on this suspension: ~1e6 training rows
-    from plain 8-substep trajectories of the 32^3 one-RBC box and of this
-    case, training with the library's BF16-MFMA kernels, then surrogate vs
-    plain substepping from the same state (wall time per step at equal dt,
-    deformation index and cell volume against plain)."""
+    """Learned membrane substepper applied to this suspension: ~1e6
+    training rows from plain 8-substep trajectories of the 32^3 one-RBC box
+    and of the 1M-particle 100-RBC channel, training with the library's
+    BF16-MFMA kernels, then surrogate vs plain substepping from the same
+    state (wall time per step at equal dt, deformation index and cell volume
+    against plain)."""
     from gpucell_amd import casegen, substepper
     import torch
     t0 = time.perf_counter()
     box = casegen.periodic_box()
+    src = case if case_name == "suspension_1m" else casegen.suspension_1m()
     X1, Y1 = substepper.collect([box], steps=150, every=1, warmup=5)
-    X2, Y2 = substepper.collect([case], steps=56, every=4, warmup=5,
-                                max_rows=14 * case.nCellPar)
+    X2, Y2 = substepper.collect([src], steps=56, every=4, warmup=5,
+                                max_rows=14 * src.nCellPar)
+    del src
     X, Y = torch.cat([X1, X2]), torch.cat([Y1, Y2])
     t_data = time.perf_counter() - t0
     log(f"[bench] surrogate: {X.shape[0]} training rows in {t_data:.1f}s")

@@ -63,6 +63,20 @@ def icosphere(subdiv: int = 3):
     n = np.cross(V[F[:, 1]] - V[F[:, 0]], V[F[:, 2]] - V[F[:, 0]])
     flip = np.einsum("ij,ij->i", n, V[F].mean(axis=1)) < 0
     F[flip] = F[flip][:, [0, 2, 1]]
+    # Mesh neighbours get nearby numbers (reverse Cuthill-McKee on the vertex
+    # graph; bandwidth 612 -> 41 at subdiv 3) and triangles are ordered by
+    # their smallest vertex, so the edges and triangles of a run of
+    # consecutive vertices lie close together in their arrays.
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+    nv = V.shape[0]
+    pairs = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
+    A = coo_matrix((np.ones(pairs.shape[0]), (pairs[:, 0], pairs[:, 1])), shape=(nv, nv)).tocsr()
+    perm = np.asarray(reverse_cuthill_mckee(A, symmetric_mode=False), dtype=np.int64)
+    inv = np.empty(nv, dtype=np.int64)
+    inv[perm] = np.arange(nv)
+    V, F = V[perm], inv[F]
+    F = F[np.argsort(F.min(axis=1), kind="stable")]
     return V, F
 
 
@@ -253,13 +267,19 @@ def _place_cells(rng, n_cells, shape, inside, gap, lo, hi, max_tries=200000, per
         return np.mod(X + shift, box)
 
     placed = []
-    tree = None
+    # one periodic KD tree per placed cell plus its centre and radius: a
+    # candidate is tested only against cells whose centres lie within the sum
+    # of the radii plus `gap` (periodic distance; by the triangle inequality
+    # no other cell can come within `gap`), the same test as one tree over
+    # every placed vertex at a cost independent of the number placed
+    trees, centres, radii = [], np.zeros((0, 3)), np.zeros(0)
+    per_axes = np.array([p is not None and p != 0 for p in (period or (None, None, None))])
     tries = 0
-    # After `random_tries` random candidates, dense suspensions continue on a
-    # lattice of slots sized to the cell's extent, with the discs nearly
-    # aligned (tilt <= 0.15 rad, random spin): random sequential addition of
-    # randomly oriented discs jams below ~25 % volume fraction.
-    random_tries = 0 if dense else min(max_tries, 20000)
+    # Random candidates for up to `max_tries`; then (or from the start with
+    # `dense`) a lattice of slots sized to the cell's extent, with the discs
+    # nearly aligned (tilt <= 0.15 rad, random spin): random sequential
+    # addition of randomly oriented discs jams below ~25 % volume fraction.
+    random_tries = 0 if dense else max_tries
     # lattice discs lie with their axis along x (the flow axis of a channel)
     R0 = _axis_rotation(np.array([0.0, 1.0, 0.0]), 0.5 * np.pi)
     ext = (shape @ R0.T).max(axis=0) - (shape @ R0.T).min(axis=0)
@@ -297,12 +317,25 @@ def _place_cells(rng, n_cells, shape, inside, gap, lo, hi, max_tries=200000, per
         X = shape @ R.T + c
         if not inside(X):
             continue
-        if tree is not None:
-            d, _ = tree.query(wrap(X), k=1, distance_upper_bound=gap)
-            if np.isfinite(d).any():
+        c = X.mean(axis=0)
+        rad = float(np.sqrt(((X - c) ** 2).sum(axis=1).max()))
+        if centres.shape[0]:
+            dc = c - centres
+            dc[:, per_axes] -= box[per_axes] * np.round(dc[:, per_axes] / box[per_axes])
+            near = np.nonzero(np.sqrt((dc ** 2).sum(axis=1)) < radii + rad + gap)[0]
+            WX = wrap(X)
+            hit = False
+            for k in near:
+                d, _ = trees[k].query(WX, k=1, distance_upper_bound=gap)
+                if np.isfinite(d).any():
+                    hit = True
+                    break
+            if hit:
                 continue
         placed.append(X)
-        tree = cKDTree(wrap(np.concatenate(placed)), boxsize=box)
+        trees.append(cKDTree(wrap(X), boxsize=box))
+        centres = np.vstack([centres, c[None, :]])
+        radii = np.append(radii, rad)
     return placed
 
 
@@ -389,14 +422,15 @@ def channel(length: int, width: int, n_rbc: int, hematocrit: float | None = None
             rbc_diameter: float | None = None, seed: int = 20261203,
             physics: Physics | None = None, jitter: float = 0.1, wall_layers: int = 3,
             subdiv: int = 3, noise: bool = True, stenosis: float = 0.0,
-            name: str = "channel") -> Case:
+            name: str = "channel", dense: bool = False) -> Case:
     """Square duct, periodic along x, `wall_layers` frozen Bound2 layers.
 
     With `hematocrit` the RBC diameter is chosen so that n_rbc cells fill
     that fraction of the fluid volume.  `stenosis` in [0, 1) narrows the
     duct with a cosine constriction: the wall sits where the signed distance
     max(|y|, |z|) - w(x) changes sign, w(x) = W/2 (1 - s/2 (1 + cos(2 pi
-    (x - L/2) / L))) within the middle half of the length."""
+    (x - L/2) / L))) within the middle half of the length.  `dense` places
+    the cells on aligned lattice slots instead of at random."""
     ph = physics or Physics()
     rng = np.random.default_rng(seed)
     W, Lx = float(width), float(length)
@@ -435,7 +469,7 @@ def channel(length: int, width: int, n_rbc: int, hematocrit: float | None = None
 
     cells = _place_cells(rng, n_rbc, shape, inside, gap, np.array([0, -half, -half]),
                          np.array([Lx, half, half]), period=(Lx, None, None),
-                         dense=hematocrit is not None and hematocrit >= 0.2) if n_rbc else []
+                         dense=dense) if n_rbc else []
     # keep each cell whole but put its centre in [0, Lx): vertices may lie
     # outside the box in x; the first step wraps them (as the reference)
     cells = [X - Lx * np.floor(X.mean(0)[0] / Lx) * np.array([1.0, 0, 0]) for X in cells]
@@ -463,9 +497,10 @@ def suspension_1m(seed: int = 20261203, physics: Physics | None = None, noise: b
 
 
 def stenosis_2m(seed: int = 20261203, physics: Physics | None = None, noise: bool = True) -> Case:
-    """~2M particles, 300 RBCs at 30 % hematocrit, 50 % cosine stenosis,
-    Morse aggregation De = 0.3 kT, r0 = 0.3, beta = 1.5: a 300 x 80 x 80
-    duct (86 x 86 with walls)."""
+    """~1.9M particles, 300 RBCs, 50 % cosine stenosis, Morse aggregation
+    De = 0.3 kT, r0 = 0.3, beta = 1.5: a 300 x 80 x 80 duct (86 x 86 with
+    walls).  The hematocrit is the highest of 30 %, 27 %, ... at which the
+    discs fit on aligned lattice slots (18 % with the default seed)."""
     ph = physics or Physics()
     ph = dataclasses.replace(ph, De=0.3 * ph.kT, r0=0.3, beta=1.5, mcut=min(1.5, ph.h))
     # 300 rigid biconcave discs pack (with a 0.8 gap, throat included) only
@@ -474,12 +509,25 @@ def stenosis_2m(seed: int = 20261203, physics: Physics | None = None, noise: boo
     for hct in (0.30, 0.27, 0.24, 0.21, 0.18, 0.15, 0.12):
         try:
             c = channel(300, 80, 300, hematocrit=hct, seed=seed, physics=ph, noise=noise,
-                        stenosis=0.5, name="stenosis_2m_300rbc")
+                        stenosis=0.5, name="stenosis_2m_300rbc", dense=True)
             c.hematocrit = hct
             return c
         except RuntimeError as e:
             last = e
     raise last
+
+
+def suspension_10m(seed: int = 20261203, physics: Physics | None = None, noise: bool = True,
+                   n_rbc: int = 2000, hematocrit: float = 0.18) -> Case:
+    """10M fluid+wall lattice sites (10,002,096; 9.37M remain around the
+    cells) and 2000 RBCs x 642 vertices at 18 % hematocrit (random
+    placement of 2000 discs jams near 19 % in this duct), 3 wall layers: a
+    411 x 150 x 150 duct (156 x 156 with walls), Poiseuille body force,
+    aggregation off."""
+    c = channel(411, 150, n_rbc, hematocrit=hematocrit, seed=seed, physics=physics,
+                noise=noise, name=f"suspension_10m_{n_rbc}rbc")
+    c.hematocrit = hematocrit
+    return c
 
 
 def ensemble64(seed: int = 20261203, nrep: int = 64, nmin: int = 20000, nmax: int = 400000,

@@ -41,6 +41,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
+#include <type_traits>
 #include <vector>
 
 using namespace gc;
@@ -147,16 +148,41 @@ __device__ __forceinline__ float ext_force_x(const GcParams& P, float y)
     return (P.force_mode == GC_FORCE_REVERSE_POISEUILLE && y >= P.force_split) ? -f : f;
 }
 
-// [1,2,3] integrate (FP64), wrap, key, histogram rank.  Slots are in the
+// [1,2] r += dt v + dt^2/(2m) F in FP64 and the periodic wrap (image
+// counters in img).  The key pass and the gather both run it on the same
+// inputs, so the gather reproduces the key pass's positions bit for bit
+// and the advanced positions are not stored in between.
+__device__ __forceinline__ void advance(const GcRep& R, const GcParams& P, const float4& v, const tfloat3& f,
+                                        double& x, double& y, double& z, int img[3])
+{
+    double dt = P.dt, a = 0.5 * (double)P.dt * (double)P.dt * (double)P.inv_m;
+    double u[3] = {x + (dt * v.x + a * f.x), y + (dt * v.y + a * f.y), z + (dt * v.z + a * f.z)};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        if (!P.per[d]) continue;
+        double lo = R.bbox[2 * d], hi = R.bbox[2 * d + 1];
+        double L = (double)(R.bbox[2 * d + 1] - R.bbox[2 * d]);
+        if (u[d] < lo) { u[d] += L; img[d] -= 1; }
+        else if (u[d] >= hi) { u[d] -= L; img[d] += 1; }
+    }
+    x = u[0]; y = u[1]; z = u[2];
+}
+
+__device__ __forceinline__ bool moves(const GcParams& P, int id, int mmove)
+{
+    return id >= P.nf && (mmove || id < P.nc);
+}
+
+// [1,2,3] key of the advanced position, histogram rank.  Slots are in the
 // previous step's (cube, id) order and few particles change cube per step,
 // so consecutive lanes mostly share a key: each run of equal keys in a wave
 // takes its ranks with one atomic (base + offset in the run), about a third
 // of the device-scope atomics of one per particle.
 template <bool ENS, int TBS>
 __global__ __launch_bounds__(TBS) void k2_integrate(int n, GcRep R0, const GcRep* __restrict__ reps,
-    const int* __restrict__ rep_of, int move, int mmove, double* __restrict__ pos64,
+    const int* __restrict__ rep_of, int move, int mmove, const double* __restrict__ pos64,
     const float4* __restrict__ vout, const tfloat3* __restrict__ force, const int* __restrict__ sp,
-    tint3* __restrict__ nthro, int* __restrict__ key, int* __restrict__ rank, int* __restrict__ cnt)
+    int* __restrict__ key, int* __restrict__ rank, int* __restrict__ cnt)
 {
     const int p = xcd_block((n + TBS - 1) / TBS) * TBS + threadIdx.x;
     const bool act = p < n;
@@ -167,25 +193,9 @@ __global__ __launch_bounds__(TBS) void k2_integrate(int n, GcRep R0, const GcRep
         const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
         const GcParams& P = R.P;
         double x = pos64[p], y = pos64[n + p], z = pos64[2 * n + p];
-        if (move && id >= P.nf && (mmove || id < P.nc)) {
-            double dt = P.dt, a = 0.5 * (double)P.dt * (double)P.dt * (double)P.inv_m;
-            float4 v = vout[p];
-            tfloat3 f = force[p];
-            double u[3] = {x + (dt * v.x + a * f.x), y + (dt * v.y + a * f.y), z + (dt * v.z + a * f.z)};
-            tint3 im = nthro[p];
-            int img[3] = {im.x, im.y, im.z};
-            bool wrapped = false;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                if (!P.per[d]) continue;
-                double lo = R.bbox[2 * d], hi = R.bbox[2 * d + 1];
-                double L = (double)(R.bbox[2 * d + 1] - R.bbox[2 * d]);
-                if (u[d] < lo) { u[d] += L; img[d] -= 1; wrapped = true; }
-                else if (u[d] >= hi) { u[d] -= L; img[d] += 1; wrapped = true; }
-            }
-            if (wrapped) nthro[p] = tint3{img[0], img[1], img[2]};
-            x = u[0]; y = u[1]; z = u[2];
-            pos64[p] = x; pos64[n + p] = y; pos64[2 * n + p] = z;
+        if (move && moves(P, id, mmove)) {
+            int img[3] = {0, 0, 0};
+            advance(R, P, vout[p], force[p], x, y, z, img);
         }
         int cx = cube_coord((float)x, R.bbox[0], P.off[0], R.invw[0], R.nc[0]);
         int cy = cube_coord((float)y, R.bbox[2], P.off[1], R.invw[1], R.nc[1]);
@@ -224,11 +234,12 @@ __global__ __launch_bounds__(BS) void k2_scatter(int n, const int* __restrict__ 
     if (r == 0) cnt[k] = 0;
 }
 
-// [4,5,6,8] final slot = cube start + number of smaller ids in the cube;
-// gather of the particle from its old slot curpos[id]; predictor.
+// [1,2,4,5,6,8] final slot = cube start + number of smaller ids in the
+// cube; gather of the particle from its old slot curpos[id], advanced as in
+// the key pass (move); predictor.
 template <bool ENS>
 __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep* __restrict__ reps,
-    const int* __restrict__ rep_of, int predict, int mpred, const int* __restrict__ tk,
+    const int* __restrict__ rep_of, int move, int mmove, const int* __restrict__ tk,
     const int* __restrict__ tv, const int* __restrict__ offs, int* __restrict__ curpos,
     const double* __restrict__ pos_old, double* __restrict__ pos_new,
     const float4* __restrict__ vout, const tfloat3* __restrict__ force,
@@ -245,18 +256,15 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     int s = b + r;
     int o = curpos[id];
     double x = pos_old[o], y = pos_old[n + o], z = pos_old[2 * n + o];
-    pos_new[s] = x; pos_new[n + s] = y; pos_new[2 * n + s] = z;
-    float fx = (float)x, fy = (float)y, fz = (float)z;
-    posid[s] = make_float4(fx, fy, fz, __int_as_float(id));
-    coor[s] = tfloat3{fx, fy, fz};
-    nt_new[s] = nt_old[o];
-    sp_new[s] = id;
-    key_new[s] = k;
+    tint3 nt = nt_old[o];
     float4 v = vout[o];
-    if (predict && id >= R0.P.nf && (mpred || id < R0.P.nc)) {
+    if (move && moves(R0.P, id, mmove)) {
         const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
         const GcParams& P = R.P;
-        tfloat3 f = force[o];
+        const tfloat3 f = force[o];
+        int img[3] = {0, 0, 0};
+        advance(R, P, v, f, x, y, z, img);
+        nt.x += img[0]; nt.y += img[1]; nt.z += img[2];
         float h = 0.5f * P.dt * P.inv_m, l = P.lam * P.dt * P.inv_m;
         velhalf[s] = tfloat3{v.x + h * f.x, v.y + h * f.y, v.z + h * f.z};
         v.x += l * f.x; v.y += l * f.y; v.z += l * f.z;
@@ -265,6 +273,13 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
         // the exported wall velocity, so reading the state changes nothing)
         velhalf[s] = id < R0.P.nf ? tfloat3{0.f, 0.f, 0.f} : tfloat3{v.x, v.y, v.z};
     }
+    pos_new[s] = x; pos_new[n + s] = y; pos_new[2 * n + s] = z;
+    float fx = (float)x, fy = (float)y, fz = (float)z;
+    posid[s] = make_float4(fx, fy, fz, __int_as_float(id));
+    coor[s] = tfloat3{fx, fy, fz};
+    nt_new[s] = nt;
+    sp_new[s] = id;
+    key_new[s] = k;
     velrho[s] = v;
     curpos[id] = s;
 }
@@ -332,7 +347,10 @@ __device__ __forceinline__ void wave_lds_sync()
 // scans its own sub-range there instead of issuing ~10 scattered global
 // loads per row.  A union longer than TSTAGE, and the periodic image cubes,
 // are read from global memory.  Scan order (image cube, then main range,
-// row by row) is the same either way.
+// row by row) is the same either way.  Rows without an image shift in any
+// lane skip the shift adds (x + 0 is x: the same bits).
+// The neighbour list of a wave is one block of 64 x NL_CAP entries, entry k
+// of lane l at 64 k + l (coalesced rows for the force pass).
 template <bool ENS, int TBS, int TSTAGE>
 __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, const float4* __restrict__ posid, float4* __restrict__ velrho,
@@ -356,21 +374,27 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
     const float ih = 1.0f / P.h;
     float d = 0.0f;  // self term added below
     float sw = 0.f, svx = 0.f, svy = 0.f, svz = 0.f;
-    int cnt = 0;
-    int woff = p;   // entry cnt of this lane's list is nl[p + cnt n]
-    auto consider = [&](int j, float4 pj, float sx, float sy, float sz, int code) {
-        float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
+    int* wl = nl + (size_t)__builtin_amdgcn_readfirstlane(p >> 6) * (NL_CAP * 64);
+    unsigned woff = (unsigned)lane;   // = lane + 64 (entries so far)
+    const unsigned wlast = (unsigned)((NL_CAP - 1) * 64 + lane);   // overflow rewrites the last entry
+    auto hit = [&](int j, float r2, int idj, int code) {
+        wl[min(woff, wlast)] = (j << 6) | code;
+        woff += 64;
+        float q = __builtin_amdgcn_sqrtf(r2) * ih, om = 1.0f - q;
+        float w = (1.0f + 3.0f * q) * om * om * om;
+        d += idj >= P.nb2 ? w : 0.0f;
+    };
+    auto consider = [&](auto SH, int j, float4 pj, float sx, float sy, float sz, int code) {
+        float rx, ry, rz;
+        if constexpr (decltype(SH)::value) {
+            rx = pi.x - (pj.x + sx); ry = pi.y - (pj.y + sy); rz = pi.z - (pj.z + sz);
+        } else {
+            rx = pi.x - pj.x; ry = pi.y - pj.y; rz = pi.z - pj.z;
+        }
         float r2 = rx * rx + ry * ry + rz * rz;
         const int idj = __float_as_int(pj.w);
         if (mov) {
-            if (r2 < P.h2 && j != p) {
-                if (cnt < NL_CAP) nl[woff] = (j << 6) | code;
-                woff += n;
-                ++cnt;
-                float q = __builtin_amdgcn_sqrtf(r2) * ih, om = 1.0f - q;
-                float w = (1.0f + 3.0f * q) * om * om * om;
-                d += idj >= P.nb2 ? w : 0.0f;
-            }
+            if (r2 < P.h2 && j != p) hit(j, r2, idj, code);
         } else if (r2 < P.h2 && idj >= P.nf) {
             float q = __builtin_amdgcn_sqrtf(r2) * ih, om = 1.0f - q;
             float w = (1.0f + 3.0f * q) * om * om * om;
@@ -384,31 +408,38 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
         const int hi = wave_max(rr.ok && rr.mb > rr.ma ? rr.mb : -1);
         if (rr.ok && rr.wb > rr.wa) {
             const int code = shift_code(rr.wsx, rr.sy, rr.sz);
-            scan_range(posid, rr.wa, rr.wb,
-                       [&](int j, float4 pj) { consider(j, pj, rr.wsx, rr.sy, rr.sz, code); });
+            scan_range(posid, rr.wa, rr.wb, [&](int j, float4 pj) {
+                consider(std::true_type{}, j, pj, rr.wsx, rr.sy, rr.sz, code);
+            });
         }
         const int code = shift_code(0.0f, rr.sy, rr.sz);
-        if (TSTAGE > 0 && hi > lo && hi - lo <= TSTAGE) {   // wave-uniform
-            for (int i = lane; i < hi - lo; i += 64) st[i] = posid[lo + i];
-            wave_lds_sync();
-            if (rr.ok) {
-                int j = rr.ma;
-                for (; j + 4 <= rr.mb; j += 4) {
-                    float4 a = st[j - lo], b = st[j + 1 - lo], c = st[j + 2 - lo], e4 = st[j + 3 - lo];
-                    consider(j, a, 0.0f, rr.sy, rr.sz, code);
-                    consider(j + 1, b, 0.0f, rr.sy, rr.sz, code);
-                    consider(j + 2, c, 0.0f, rr.sy, rr.sz, code);
-                    consider(j + 3, e4, 0.0f, rr.sy, rr.sz, code);
+        auto main_range = [&](auto SH) {
+            if (TSTAGE > 0 && hi > lo && hi - lo <= TSTAGE) {   // wave-uniform
+                for (int i = lane; i < hi - lo; i += 64) st[i] = posid[lo + i];
+                wave_lds_sync();
+                if (rr.ok) {
+                    int j = rr.ma;
+                    for (; j + 4 <= rr.mb; j += 4) {
+                        float4 a = st[j - lo], b = st[j + 1 - lo], c = st[j + 2 - lo], e4 = st[j + 3 - lo];
+                        consider(SH, j, a, 0.0f, rr.sy, rr.sz, code);
+                        consider(SH, j + 1, b, 0.0f, rr.sy, rr.sz, code);
+                        consider(SH, j + 2, c, 0.0f, rr.sy, rr.sz, code);
+                        consider(SH, j + 3, e4, 0.0f, rr.sy, rr.sz, code);
+                    }
+                    for (; j < rr.mb; ++j) consider(SH, j, st[j - lo], 0.0f, rr.sy, rr.sz, code);
                 }
-                for (; j < rr.mb; ++j) consider(j, st[j - lo], 0.0f, rr.sy, rr.sz, code);
+                wave_lds_sync();
+            } else if (rr.ok) {
+                scan_range(posid, rr.ma, rr.mb, [&](int j, float4 pj) {
+                    consider(SH, j, pj, 0.0f, rr.sy, rr.sz, code);
+                });
             }
-            wave_lds_sync();
-        } else if (rr.ok) {
-            scan_range(posid, rr.ma, rr.mb,
-                       [&](int j, float4 pj) { consider(j, pj, 0.0f, rr.sy, rr.sz, code); });
-        }
+        };
+        if (__ballot(rr.ok && (rr.sy != 0.0f || rr.sz != 0.0f))) main_range(std::true_type{});
+        else main_range(std::false_type{});
     }
     if (mov) {
+        const int cnt = (int)((woff - (unsigned)lane) >> 6);
         velrho[p].w = P.m * P.wnorm * (d + 1.0f);
         nlc[p] = cnt <= NL_CAP ? cnt : -1;
     } else if (act && id < P.nf) {
@@ -502,7 +533,8 @@ __global__ __launch_bounds__(TBS) void k2_force(int n, GcRep R0, const GcRep* __
     // order of the cubes, so the sums below run in that order either way.
     const int cnt = nlc[p];
     if (cnt >= 0) {
-        for (int k = 0; k < cnt; ++k) pair(nl[k * n + p]);
+        const int* wl = nl + (size_t)__builtin_amdgcn_readfirstlane(p >> 6) * (NL_CAP * 64) + (p & 63);
+        for (int k = 0; k < cnt; ++k) pair(wl[64 * k]);
     } else {  // more than NL_CAP neighbours: scan the cubes again
         visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
             const int code = shift_code(sx, sy, sz);
@@ -561,8 +593,9 @@ __global__ __launch_bounds__(BS) void k2_agg(int n, GcRep R0, const GcRep* __res
     };
     const int cnt = nlc[p];
     if (cnt >= 0) {
+        const int* wl = nl + (size_t)__builtin_amdgcn_readfirstlane(p >> 6) * (NL_CAP * 64) + (p & 63);
         for (int k = 0; k < cnt; ++k) {
-            const int code = nl[k * n + p];
+            const int code = wl[64 * k];
             morse(code >> 6, code_shift(code, R.L[0]), code_shift(code >> 2, R.L[1]),
                   code_shift(code >> 4, R.L[2]));
         }
@@ -656,7 +689,7 @@ void mark(Prof* pr, hipStream_t s, const char* name)
 // Neighbour build: keys from the (optionally advanced) FP64 positions,
 // counting sort by (cube, id), gather (+ predictor).
 template <bool ENS>
-int launch_build_t(gc_engine* e, int move, int predict, Prof* pr = nullptr)
+int launch_build_t(gc_engine* e, int move, Prof* pr = nullptr)
 {
     const int N = e->N, c = e->cur, nx = 1 - c;
     hipStream_t s = e->s;
@@ -668,7 +701,7 @@ int launch_build_t(gc_engine* e, int move, int predict, Prof* pr = nullptr)
     {
         auto L = [&](auto kern, int tbs) {
             kern<<<xcd_grid(blocks_for(N, tbs)), tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, mfree,
-                e->d_pos64[c], e->d_vout, e->d_force, e->d_sp[c], e->d_nt[c], e->d_key0, e->d_rank, e->d_cnt);
+                e->d_pos64[c], e->d_vout, e->d_force, e->d_sp[c], e->d_key0, e->d_rank, e->d_cnt);
         };
         switch (e->tune.v[TUNE_INTEGRATE]) {
             case 0: L(k2_integrate<ENS, 64>, 64); break;
@@ -684,7 +717,7 @@ int launch_build_t(gc_engine* e, int move, int predict, Prof* pr = nullptr)
     k2_scatter<<<gb, BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[c], e->d_offs, e->d_tk, e->d_tv,
                                  e->d_cnt);
     GC_LAUNCH_CHECK("engine scatter");
-    k2_sortgather<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, predict, mfree, e->d_tk, e->d_tv,
+    k2_sortgather<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, mfree, e->d_tk, e->d_tv,
         e->d_offs, e->d_curpos, e->d_pos64[c], e->d_pos64[nx], e->d_vout, e->d_force, e->d_nt[c],
         e->d_nt[nx], e->d_sp[nx], e->d_key, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf);
     GC_LAUNCH_CHECK("engine sort/gather");
@@ -692,9 +725,9 @@ int launch_build_t(gc_engine* e, int move, int predict, Prof* pr = nullptr)
     return GC_OK;
 }
 
-int launch_build(gc_engine* e, int move, int predict)
+int launch_build(gc_engine* e, int move)
 {
-    return e->ens() ? launch_build_t<true>(e, move, predict) : launch_build_t<false>(e, move, predict);
+    return e->ens() ? launch_build_t<true>(e, move) : launch_build_t<false>(e, move);
 }
 
 template <bool ENS>
@@ -705,7 +738,7 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
     hipStream_t s = e->s;
     const int gb = xcd_grid(blocks_for(N, BS));
     mark(pr, s, "start");
-    int st = launch_build_t<ENS>(e, 1, 1, pr);
+    int st = launch_build_t<ENS>(e, 1, pr);
     if (st) return st;
     mark(pr, s, "sort+gather+predict");
     const int c = e->cur;
@@ -759,13 +792,6 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
         mark(pr, s, "membrane substeps");
     } else if (e->ncell > 0) {
         const GcParams& P = R0.P;
-        k_geo_cell<<<e->ncell, BS, 0, s>>>(P, e->step, ENS ? e->d_reps : nullptr,
-            ENS ? e->d_cell_rep : nullptr, e->d_bbox, e->d_curpos, e->d_tri,
-            e->d_betri, e->d_nt[c], e->d_coor, e->d_trinor, e->d_triarea, e->d_reftriarea,
-            e->d_tricen, e->d_refarea_para, e->d_refvol_para, e->d_area_para, e->d_vol_para,
-            e->d_area, e->d_vol);
-        GC_LAUNCH_CHECK("engine geo");
-        mark(pr, s, "membrane geometry");
         if (e->any_morse) {
             k2_agg<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid, e->d_offs,
                                           e->d_key, e->d_nl, e->d_nlc, e->d_celltab, e->d_force,
@@ -782,15 +808,27 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
         a.es = e->d_es; a.ee = e->d_ee; a.ts = e->d_ts; a.te = e->d_te;
         a.defforce = e->d_defforce; a.force = e->d_force;
         a.velhalf = e->d_velhalf; a.velrho = e->d_velrho; a.vout = e->d_vout;
-        a.mpos = e->d_mpos; a.nm = e->nm;
+        a.nm = e->nm;
         if (ENS) { a.reps = e->d_reps; a.cell_rep = e->d_cell_rep; a.celltab = e->d_celltab; }
-        k2_mgather<<<blocks_for(e->nm, BS), BS, 0, s>>>(e->nm, R0, ENS ? e->d_reps : nullptr,
-            e->d_cell_rep, e->d_celltab, e->d_curpos, e->d_coor, e->d_nt[c], e->d_mpos);
-        GC_LAUNCH_CHECK("engine membrane gather");
-        // 64-thread groups: a few hundred RBCs give only ~Nm/256 groups of 256
-        const int dg = 64 << e->tune.v[TUNE_MEMBRANE];   // 64, 128 or 256 threads
-        k_def_force<<<xcd_grid(blocks_for(e->nm, dg)), dg, 0, s>>>(P, e->nm, e->d_bbox, a);
-        GC_LAUNCH_CHECK("engine membrane force");
+        if (use_cell_kernel(e)) {
+            // geometry, membrane force and corrector in one workgroup per cell
+            mark(pr, s, "membrane geometry");
+            k_cell_membrane<<<e->ncell, GC_CELL_BS, 0, s>>>(P, e->d_bbox, cell_args(e, e->step, 0), a);
+            GC_LAUNCH_CHECK("engine membrane (per cell)");
+        } else {
+            k_geo_cell<<<e->ncell, BS, 0, s>>>(P, e->step, ENS ? e->d_reps : nullptr,
+                ENS ? e->d_cell_rep : nullptr, e->d_bbox, e->d_curpos, e->d_tri,
+                e->d_betri, e->d_nt[c], e->d_coor, e->d_trinor, e->d_triarea, e->d_reftriarea,
+                e->d_tricen, e->d_refarea_para, e->d_refvol_para, e->d_area_para, e->d_vol_para,
+                e->d_area, e->d_vol);
+            GC_LAUNCH_CHECK("engine geo");
+            mark(pr, s, "membrane geometry");
+            k2_mgather<<<blocks_for(e->nm, BS), BS, 0, s>>>(e->nm, R0, ENS ? e->d_reps : nullptr,
+                e->d_cell_rep, e->d_celltab, e->d_curpos, e->d_coor, e->d_nt[c], e->d_mpos);
+            GC_LAUNCH_CHECK("engine membrane gather");
+            a.mpos = e->d_mpos;
+            if ((st = membrane_force(e, P, a, s))) return st;
+        }
         mark(pr, s, "membrane force+corrector");
     }
     // [18] CalBound2Vel after the corrector only feeds the output: the next
@@ -800,6 +838,32 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
     e->step += 1;
     return GC_OK;
 }
+
+}  // namespace
+
+CellArgs gc::cell_args(gc_engine* e, long step, int round_geo)
+{
+    CellArgs ca;
+    ca.range = e->d_cellr; ca.betri = e->d_betri; ca.step = step; ca.round_geo = round_geo;
+    ca.trinor = e->d_trinor; ca.triarea = e->d_triarea; ca.reftriarea = e->d_reftriarea;
+    ca.tricen = e->d_tricen; ca.refarea_para = e->d_refarea_para; ca.refvol_para = e->d_refvol_para;
+    ca.area_para = e->d_area_para; ca.vol_para = e->d_vol_para; ca.area = e->d_area; ca.vol = e->d_vol;
+    return ca;
+}
+
+int gc::membrane_force(gc_engine* e, const GcParams& P, const DefArgs& a, hipStream_t s)
+{
+    // 64-thread groups: a few hundred RBCs give only ~Nm/256 groups of 256
+    const int v = e->tune.v[TUNE_MEMBRANE];
+    const int dg = 64 << (v < 3 ? v : 0);   // 64, 128 or 256 threads
+    k_edge_force<<<xcd_grid(blocks_for(e->nedge, dg)), dg, 0, s>>>(P, e->nedge, e->d_bbox, a, e->d_ef);
+    k_tri_force<<<xcd_grid(blocks_for(e->ntri, dg)), dg, 0, s>>>(P, e->ntri, e->d_bbox, a, e->d_tf);
+    k_vert_force<<<xcd_grid(blocks_for(e->nm, dg)), dg, 0, s>>>(P, e->nm, a, e->d_ef, e->d_tf);
+    GC_LAUNCH_CHECK("engine membrane force");
+    return GC_OK;
+}
+
+namespace {
 
 int launch_step(gc_engine* e, int driveon, Prof* pr)
 {
@@ -982,7 +1046,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
         (st = e->alloc(&e->d_rank, N)) || (st = e->alloc(&e->d_cnt, K + 1)) ||
         (st = e->alloc(&e->d_offs, K + 1)) || (st = e->alloc(&e->d_tk, N)) ||
         (st = e->alloc(&e->d_tv, N)) || (st = e->alloc(&e->d_curpos, N)) ||
-        (st = e->alloc(&e->d_nl, (size_t)NL_CAP * N)) || (st = e->alloc(&e->d_nlc, N)) ||
+        (st = e->alloc(&e->d_nl, (size_t)NL_CAP * (((size_t)N + 63) & ~(size_t)63))) || (st = e->alloc(&e->d_nlc, N)) ||
         (st = e->alloc(&e->d_scanws, scan_ws_ints(e->K))) || (st = e->alloc(&e->d_ctr, 1)))
         return fail(st);
     hipStream_t s = e->s;
@@ -1023,7 +1087,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
             return fail(st);
     }
     // first neighbour build (Particle::ConCubeList at init, Particle.cu:570)
-    if ((st = launch_build(e, 0, 0))) return fail(st);
+    if ((st = launch_build(e, 0))) return fail(st);
     if ((st = reorder_carried(e, vout_id, f_id, vh_id))) return fail(st);
 
     if (e->ncell > 0) {
@@ -1041,6 +1105,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
             (st = e->alloc(&e->d_vol_para, T)) || (st = e->alloc(&e->d_area, C)) ||
             (st = e->alloc(&e->d_vol, C)) || (st = e->alloc(&e->d_pararea, M)) ||
             (st = e->alloc(&e->d_defforce, M)) || (st = e->alloc(&e->d_aggforce, M)) ||
+            (st = e->alloc(&e->d_ef, 4 * E)) || (st = e->alloc(&e->d_tf, 3 * T)) ||
             (st = e->alloc(&e->d_mpos, 3 * M)))
             return fail(st);
         // global membrane arrays: 1-based global vertex ids, global triangle ids
@@ -1070,6 +1135,12 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
         }
         std::vector<int> es, ee, ts, te2;
         if ((st = build_incidence(e->nm, e->nedge, edge.data(), e->ntri, tri.data(), es, ee, ts, te2)))
+            return fail(st);
+        std::vector<int4> cellr(C);
+        e->cell_fused = cell_ranges(e->nm, e->nedge, e->ntri, e->ncell, celltab.data(), betri.data(),
+                                    edge.data(), tri.data(), cellr.data());
+        if (e->cell_fused &&
+            ((st = e->alloc(&e->d_cellr, C)) || (st = upload(e->d_cellr, cellr.data(), C, s))))
             return fail(st);
         if ((st = e->alloc(&e->d_es, es.size())) || (st = e->alloc(&e->d_ee, ee.size())) ||
             (st = e->alloc(&e->d_ts, ts.size())) || (st = e->alloc(&e->d_te, te2.size())))
@@ -1313,7 +1384,7 @@ int restore_state(gc_engine* e, const std::vector<double>& pos, const std::vecto
         (st = upload(e->d_sp[0], idn.data(), N, s)) || (st = upload(e->d_curpos, idn.data(), N, s)) ||
         (st = upload(e->d_ctr, &ctr, 1, s)))
         return st;
-    if ((st = launch_build(e, 0, 0))) return st;
+    if ((st = launch_build(e, 0))) return st;
     if ((st = reorder_carried(e, vo, f, vh))) return st;
     e->walls_dirty = false;
     e->step = step;
@@ -1413,6 +1484,8 @@ namespace {
 
 const char* const TUNE_PHASE[TUNE_N] = {"integrate+key", "density+wall velocity", "sdpd force",
                                         "membrane force+corrector"};
+// a second phase timed with the first (the per-cell kernel also does the geometry)
+const char* const TUNE_PHASE2[TUNE_N] = {nullptr, nullptr, nullptr, "membrane geometry"};
 const char* const TUNE_FAMILY[TUNE_N] = {"integrate", "density", "sdpd force", "membrane force"};
 
 void tune_shape(int f, int v, char* out, size_t n)
@@ -1420,12 +1493,13 @@ void tune_shape(int f, int v, char* out, size_t n)
     static const int dbs[12] = {64, 64, 64, 64, 128, 128, 128, 128, 256, 256, 256, 256};
     static const int dst[12] = {0, 128, 192, 320, 0, 128, 192, 320, 0, 128, 192, 320};
     if (f == TUNE_DENSITY) snprintf(out, n, "block %d, stage %d", dbs[v], dst[v]);
-    else if (f == TUNE_MEMBRANE) snprintf(out, n, "group %d", 64 << v);
+    else if (f == TUNE_MEMBRANE && v < 3) snprintf(out, n, "3 passes, group %d", 64 << v);
+    else if (f == TUNE_MEMBRANE) snprintf(out, n, "per cell, %d threads", GC_CELL_BS);
     else snprintf(out, n, "block %d", 64 << v);
 }
 
-// Average time (us) of one phase over nsteps eager steps.
-int phase_us(gc_engine* e, int nsteps, int driveon, const char* name, float* us)
+// Average time (us) of one phase (plus name2 if given) over nsteps eager steps.
+int phase_us(gc_engine* e, int nsteps, int driveon, const char* name, const char* name2, float* us)
 {
     Prof pr;
     for (int k = 0; k < 32; ++k) (void)hipEventCreate(&pr.ev[k]);
@@ -1436,7 +1510,7 @@ int phase_us(gc_engine* e, int nsteps, int driveon, const char* name, float* us)
         if ((st = launch_step(e, driveon, &pr))) break;
         st = hip_check(hipEventSynchronize(pr.ev[pr.n - 1]), "autotune sync");
         for (int k = 1; k < pr.n && !st; ++k)
-            if (!strcmp(pr.name[k], name)) {
+            if (!strcmp(pr.name[k], name) || (name2 && !strcmp(pr.name[k], name2))) {
                 float ms = 0.f;
                 (void)hipEventElapsedTime(&ms, pr.ev[k - 1], pr.ev[k]);
                 acc += ms;
@@ -1470,10 +1544,11 @@ extern "C" int gc_EngineAutotune(gc_engine* e, int steps, int driveon, gc_tune_r
         int bestv = e->tune.v[f];
         const int row0 = row;
         for (int v = 0; v < TUNE_NVARIANT[f] && !st; ++v) {
+            if (f == TUNE_MEMBRANE && v == 3 && !e->cell_fused) continue;   // cells too large
             e->tune.v[f] = v;
             float us = 0.f;
             if ((st = launch_step(e, driveon, nullptr))) break;   // warm-up (code load)
-            if ((st = phase_us(e, steps, driveon, TUNE_PHASE[f], &us))) break;
+            if ((st = phase_us(e, steps, driveon, TUNE_PHASE[f], TUNE_PHASE2[f], &us))) break;
             if (us < best) { best = us; bestv = v; }
             if (out && row < 64) {
                 snprintf(out->family[row], sizeof(out->family[row]), "%s", TUNE_FAMILY[f]);

@@ -36,9 +36,11 @@ struct RepMeta {
 // index per kernel family; every variant computes bit-identical results.
 enum { TUNE_INTEGRATE = 0, TUNE_DENSITY = 1, TUNE_FORCE = 2, TUNE_MEMBRANE = 3, TUNE_N = 4 };
 struct TuneCfg {
-    int v[TUNE_N] = {2, 10, 2, 0};   // integrate 256, density 256 x 192, force 256, membrane 64
+    // integrate 256, density 256 x 192, force 256, membrane per cell (groups of
+    // 64 threads in three passes when a cell does not fit)
+    int v[TUNE_N] = {2, 10, 2, 3};
 };
-constexpr int TUNE_NVARIANT[TUNE_N] = {3, 12, 3, 3};
+constexpr int TUNE_NVARIANT[TUNE_N] = {3, 12, 3, 4};
 
 // Membrane substepping / surrogate state (gc_substep.hip); NULL while the
 // membrane moves with the fluid step.
@@ -74,6 +76,10 @@ struct gc_engine {
     tfloat3 *d_trinor, *d_tricen; float *d_triarea, *d_reftriarea;
     float *d_refarea_para, *d_refvol_para, *d_area_para, *d_vol_para, *d_area, *d_vol, *d_pararea;
     tfloat3 *d_defforce, *d_aggforce;
+    int4* d_cellr = nullptr;   // per-cell vertex and edge ranges (fused membrane kernel)
+    bool cell_fused = false;   // every cell fits the fused kernel
+    tfloat3* d_ef = nullptr;   // per-edge forces on the edge's four vertices
+    tfloat3* d_tf = nullptr;   // per-triangle forces on the three corners
     double* d_mpos;
     int *d_es, *d_ee, *d_ts, *d_te;
     hipGraphExec_t gexec = nullptr;
@@ -95,6 +101,13 @@ struct gc_engine {
 namespace gc {
 // One outer step of the membrane under substepping or the surrogate, after
 // the fluid forces of the step (gc_substep.hip).
+// membrane force of the engine's membrane (edge, triangle, vertex passes)
+// with the arguments of k_def_force; P is the first replica's block
+int membrane_force(gc_engine* e, const GcParams& P, const DefArgs& a, hipStream_t s);
+// the fused per-cell kernel applies (tuning and cell sizes)
+inline bool use_cell_kernel(const gc_engine* e) { return e->cell_fused && e->tune.v[TUNE_MEMBRANE] == 3; }
+// geometry outputs of the engine for k_cell_membrane
+CellArgs cell_args(gc_engine* e, long step, int round_geo);
 int ms_step(gc_engine* e);
 // Re-reads the membrane state from the engine arrays (after SetState).
 int ms_reset(gc_engine* e);

@@ -110,6 +110,38 @@ __global__ void k_mech_para(GcParams P, int nedge, const int6* __restrict__ edge
                             float* __restrict__ kbo, float* __restrict__ mo, float* __restrict__ po,
                             float* __restrict__ kpo, float* __restrict__ th0o);
 __global__ void k_def_force(GcParams P, int nm, const float* __restrict__ bbox, DefArgs a);
+// Fused per-cell membrane step (one 1024-thread workgroup per cell): the
+// vertex positions, then k_geo_cell's geometry and cell area/volume, then
+// the edge and triangle forces held in LDS and summed per vertex in the
+// incidence order, as the three kernels below do through global memory.
+// A cell fits when its vertices, edges and triangles are contiguous, its
+// records reference only its own vertices, and it has at most GC_CELL_NV
+// vertices, GC_CELL_NE edges and GC_CELL_NT triangles.
+constexpr int GC_CELL_BS = 1024;
+constexpr int GC_CELL_NV = 768;
+constexpr int GC_CELL_NE = 2048;
+constexpr int GC_CELL_NT = (GC_CELL_NE * 12) / 9;
+struct CellArgs {
+    const int4* range;    // (first vertex, vertices, first edge, edges) per cell, 0-based
+    const tint2* betri;   // triangle range per cell (inclusive)
+    long step;            // -1: also the reference geometry (k_geo_cell's stepcount)
+    int round_geo;        // geometry from FP32-rounded positions (membrane substeps)
+    tfloat3* trinor; float* triarea; float* reftriarea; tfloat3* tricen;
+    float* refarea_para; float* refvol_para; float* area_para; float* vol_para; float* area; float* vol;
+};
+__global__ void k_cell_membrane(GcParams P, const float* __restrict__ bbox, CellArgs ca, DefArgs a);
+// host check of the conditions above; fills range (C entries)
+bool cell_ranges(int nm, int nedge, int ntri, int ncell, const int* celltab, const tint2* betri,
+                 const int6* edge, const tint3* tri, int4* range);
+
+// the same force in three passes: per edge (ef[4 e + role]), per triangle
+// (tf[3 t + corner]), then the per-vertex sum in incidence order
+__global__ void k_edge_force(GcParams P, int nedge, const float* __restrict__ bbox, DefArgs a,
+                             tfloat3* __restrict__ ef);
+__global__ void k_tri_force(GcParams P, int ntri, const float* __restrict__ bbox, DefArgs a,
+                            tfloat3* __restrict__ tf);
+__global__ void k_vert_force(GcParams P, int nm, DefArgs a, const tfloat3* __restrict__ ef,
+                             const tfloat3* __restrict__ tf);
 __global__ void k_agg_force(int n, GcParams P, const tfloat3* __restrict__ coor,
                             tfloat3* __restrict__ aggforce, const tint3* __restrict__ cc,
                             const int* __restrict__ ncube, const int* __restrict__ sortpart,

@@ -374,6 +374,348 @@ __global__ __launch_bounds__(BS) void k_def_force(GcParams P, int nm, const floa
     }
 }
 
+// ---- CalCellDefForce in three passes (engine) -----------------------------
+// The gather above evaluates every edge's spring and dihedral from each of
+// its four vertices and every triangle from each of its three corners.  The
+// engine evaluates each term once instead: one thread per edge writes the
+// edge's forces on its four vertices (roles A, B, C, D), one thread per
+// triangle the area/volume forces on its three corners, and one thread per
+// vertex sums its entries in the incidence order above.  Positions are
+// relative to the edge's (triangle's) first vertex; every term depends on
+// differences only.  The volume gradient of a triangle at corner k with
+// that corner as origin is (P_n - P_k) x (P_nn - P_k), the triangle's
+// un-normalised normal n = (P1 - P0) x (P2 - P0) for every corner.
+
+__device__ __forceinline__ void rep_params(const DefArgs& a, int vtx0, GcParams& P, const float*& bbox)
+{
+    if (a.reps) {
+        const GcRep& R = a.reps[a.cell_rep[a.celltab[vtx0]]];
+        P = R.P;
+        bbox = R.bbox;
+    }
+}
+
+// origin of a record: its first vertex (1-based id v1)
+__device__ __forceinline__ void rec_origin(const DefArgs& a, const GcParams& P, int v1, tfloat3& xo,
+                                           tint3& no, double& ox, double& oy, double& oz)
+{
+    if (a.mpos) {
+        const int u = v1 - 1;
+        ox = a.mpos[u]; oy = a.mpos[a.nm + u]; oz = a.mpos[2 * a.nm + u];
+        xo = tfloat3{0.f, 0.f, 0.f}; no = tint3{0, 0, 0};
+    } else {
+        const int s = a.curpos[P.nc + v1 - 1];
+        xo = a.coor[s]; no = a.nthro[s];
+        ox = oy = oz = 0.0;
+    }
+}
+
+// Forces of edge e on its vertices A, B, C, D from B, C, D relative to A.
+__device__ __forceinline__ void edge_terms(const GcParams& P, const DefArgs& a, int e, float3 pB, float3 pC,
+                                           float3 pD, float3 out[4])
+{
+    // WLC-POW spring along d = A - B: s d on A, -s d on B
+    const float3 d = make_float3(-pB.x, -pB.y, -pB.z);
+    const float ls = sqrtf(f3dot(d, d));
+    const float x = ls / a.lmax[e];
+    const float om = 1.0f - x;
+    const float fm = -(P.kT / a.p[e]) * (0.25f / (om * om) - 0.25f + x) + a.kp[e] / powf(ls, a.mexp[e]);
+    const float s = fm / ls;
+    // bending: dihedral of the wings C (n1 = e x c) and D (n2 = d x e)
+    const float3 e3 = pB, ac = pC, ad = pD;
+    const float3 n1 = f3cross(e3, ac), n2 = f3cross(ad, e3);
+    const float l2 = f3dot(e3, e3), l = sqrtf(l2);
+    const float q1 = f3dot(n1, n1), q2 = f3dot(n2, n2);
+    const float im1 = rsqrtf(q1), im2 = rsqrtf(q2);
+    const float3 u1 = f3scale(n1, im1), u2 = f3scale(n2, im2);
+    const float3 xx = f3cross(u1, u2);
+    const float th = atan2f(f3dot(xx, e3) / l, f3dot(u1, u2));
+    const float cb = -a.kb[e] * sinf(th - a.th0[e]);
+    const float3 gC = f3scale(n1, -l / q1), gD = f3scale(n2, -l / q2);
+    const float tC = f3dot(ac, e3) / l2, tD = f3dot(ad, e3) / l2;
+    const float wCA = -(1.0f - tC), wDA = -(1.0f - tD), wCB = -tC, wDB = -tD;
+    out[0] = make_float3(s * d.x + cb * (wCA * gC.x + wDA * gD.x), s * d.y + cb * (wCA * gC.y + wDA * gD.y),
+                         s * d.z + cb * (wCA * gC.z + wDA * gD.z));
+    out[1] = make_float3(-s * d.x + cb * (wCB * gC.x + wDB * gD.x), -s * d.y + cb * (wCB * gC.y + wDB * gD.y),
+                         -s * d.z + cb * (wCB * gC.z + wDB * gD.z));
+    out[2] = make_float3(cb * gC.x, cb * gC.y, cb * gC.z);
+    out[3] = make_float3(cb * gD.x, cb * gD.y, cb * gD.z);
+}
+
+// Area and volume forces of triangle t on its corners from P1, P2 relative
+// to P0; Ac, Vc the cell's current area and volume.
+__device__ __forceinline__ void tri_terms(const DefArgs& a, int t, float3 P1, float3 P2, float Ac, float Vc,
+                                          float3 out[3])
+{
+    const float3 P0 = make_float3(0.f, 0.f, 0.f);
+    const float3 n = f3cross(P1, P2);
+    const float nn = sqrtf(f3dot(n, n));
+    const float At = 0.5f * nn;
+    const float3 un = f3scale(n, 1.0f / nn);
+    const float cA = a.ka[t] * (Ac - a.A0c[t]) / a.A0c[t] + a.kd[t] * (At - a.reftriarea[t]) / a.reftriarea[t];
+    const float cV = a.kv[t] * (Vc - a.V0c[t]) / a.V0c[t];
+    const float hv = cV * (1.0f / 6.0f);
+    const float3 cn[3] = {P1, P2, P0};   // next corner of corner k
+    const float3 cnn[3] = {P2, P0, P1};  // next-next corner
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float3 dA = f3cross(un, f3sub(cnn[k], cn[k]));   // 2 dA/dP_k
+        out[k] = make_float3(-0.5f * cA * dA.x - hv * n.x, -0.5f * cA * dA.y - hv * n.y,
+                             -0.5f * cA * dA.z - hv * n.z);
+    }
+}
+
+__global__ __launch_bounds__(BS) void k_edge_force(GcParams P, int nedge, const float* __restrict__ bbox,
+                                                   DefArgs a, tfloat3* __restrict__ ef)
+{
+    const int e = xcd_block((nedge + (int)blockDim.x - 1) / (int)blockDim.x) * blockDim.x + threadIdx.x;
+    if (e >= nedge) return;
+    const int6 E = a.edge[e];
+    const float* bb = bbox;
+    rep_params(a, E.x - 1, P, bb);
+    const PerBox b = load_box(P, bb);
+    tfloat3 xo; tint3 no; double ox, oy, oz;
+    rec_origin(a, P, E.x, xo, no, ox, oy, oz);
+    const float3 pB = vpos(a, P, b, E.y, xo, no, ox, oy, oz);
+    const float3 pC = vpos(a, P, b, E.z, xo, no, ox, oy, oz);
+    const float3 pD = vpos(a, P, b, E.k, xo, no, ox, oy, oz);
+    float3 f[4];
+    edge_terms(P, a, e, pB, pC, pD, f);
+    tfloat3* o = ef + 4 * (size_t)e;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = tfloat3{f[k].x, f[k].y, f[k].z};
+}
+
+__global__ __launch_bounds__(BS) void k_tri_force(GcParams P, int ntri, const float* __restrict__ bbox,
+                                                  DefArgs a, tfloat3* __restrict__ tf)
+{
+    const int t = xcd_block((ntri + (int)blockDim.x - 1) / (int)blockDim.x) * blockDim.x + threadIdx.x;
+    if (t >= ntri) return;
+    const tint3 T = a.tri[t];
+    const float* bb = bbox;
+    rep_params(a, T.x - 1, P, bb);
+    const PerBox b = load_box(P, bb);
+    tfloat3 xo; tint3 no; double ox, oy, oz;
+    rec_origin(a, P, T.x, xo, no, ox, oy, oz);
+    const float3 P1 = vpos(a, P, b, T.y, xo, no, ox, oy, oz);
+    const float3 P2 = vpos(a, P, b, T.z, xo, no, ox, oy, oz);
+    float3 f[3];
+    tri_terms(a, t, P1, P2, a.Ac[t], a.Vc[t], f);
+    tfloat3* o = tf + 3 * (size_t)t;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o[k] = tfloat3{f[k].x, f[k].y, f[k].z};
+}
+
+__global__ __launch_bounds__(BS) void k_vert_force(GcParams P, int nm, DefArgs a,
+                                                   const tfloat3* __restrict__ ef,
+                                                   const tfloat3* __restrict__ tf)
+{
+    const int v = xcd_block((nm + (int)blockDim.x - 1) / (int)blockDim.x) * blockDim.x + threadIdx.x;
+    if (v >= nm) return;
+    if (a.reps) P = a.reps[a.cell_rep[a.celltab[v]]].P;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    for (int q = a.es[v]; q < a.es[v + 1]; ++q) {   // code = 4 edge + role
+        const tfloat3 g = ef[a.ee[q]];
+        fx += g.x; fy += g.y; fz += g.z;
+    }
+    for (int q = a.ts[v]; q < a.ts[v + 1]; ++q) {
+        const int code = a.te[q];
+        const tfloat3 g = tf[3 * (code >> 2) + (code & 3)];
+        fx += g.x; fy += g.y; fz += g.z;
+    }
+    a.defforce[v] = tfloat3{fx, fy, fz};
+    if (!a.force) return;   // membrane-only evaluation (substepping)
+    const int sv = a.curpos[P.nc + v];
+    tfloat3 F = a.force[sv];
+    F.x += fx; F.y += fy; F.z += fz;
+    a.force[sv] = F;
+    if (a.vout) {  // this is the vertex's last force term -> corrector
+        const float h = 0.5f * P.dt * P.inv_m;
+        const tfloat3 vh = a.velhalf[sv];
+        a.vout[sv] = make_float4(vh.x + h * F.x, vh.y + h * F.y, vh.z + h * F.z, a.velrho[sv].w);
+    }
+}
+
+__global__ __launch_bounds__(GC_CELL_BS) void k_cell_membrane(GcParams P, const float* __restrict__ bbox,
+                                                              CellArgs ca, DefArgs a)
+{
+    __shared__ double sx[GC_CELL_NV], sy[GC_CELL_NV], sz[GC_CELL_NV];
+    __shared__ float buf[GC_CELL_NE * 12];
+    __shared__ double sa[BS], sv[BS];
+    const int c = blockIdx.x, tid = threadIdx.x;
+    const int4 r = ca.range[c];
+    const int v0 = r.x, nv = r.y, e0 = r.z, ne = r.w;
+    const int tb = ca.betri[c].x, te = ca.betri[c].y;
+    const float* bb = bbox;
+    if (a.reps) {
+        const GcRep& R = a.reps[a.cell_rep[c]];
+        P = R.P;
+        bb = R.bbox;
+    }
+    const PerBox b = load_box(P, bb);
+    // unwrapped FP64 positions (k2_mgather's values) of the cell's vertices
+    int slot = -1;
+    if (tid < nv) {
+        const int g = v0 + tid;
+        if (a.mpos) {
+            sx[tid] = a.mpos[g]; sy[tid] = a.mpos[a.nm + g]; sz[tid] = a.mpos[2 * a.nm + g];
+        } else {
+            slot = a.curpos[P.nc + g];
+            const tfloat3 x = a.coor[slot];
+            const tint3 n = a.nthro[slot];
+            sx[tid] = (double)x.x + (b.per[0] ? (double)n.x * (double)b.L[0] : 0.0);
+            sy[tid] = (double)x.y + (b.per[1] ? (double)n.y * (double)b.L[1] : 0.0);
+            sz[tid] = (double)x.z + (b.per[2] ? (double)n.z * (double)b.L[2] : 0.0);
+        }
+    }
+    __syncthreads();
+    // differences of unwrapped positions rounded once (what urel and vpos
+    // compute); the substepper's geometry input is the FP32 rounding of them
+    auto rel = [&](int u, int o) {
+        return make_float3((float)(sx[u] - sx[o]), (float)(sy[u] - sy[o]), (float)(sz[u] - sz[o]));
+    };
+    auto rel_geo = [&](int u, int o) {
+        if (!ca.round_geo) return rel(u, o);
+        return make_float3((float)((double)(float)sx[u] - (double)(float)sx[o]),
+                           (float)((double)(float)sy[u] - (double)(float)sy[o]),
+                           (float)((double)(float)sz[u] - (double)(float)sz[o]));
+    };
+    // geometry: k_geo_cell's arithmetic and reduction order (BS threads)
+    const int o = a.tri[tb].x - 1 - v0;
+    double A = 0.0, V = 0.0;
+    if (tid < BS) {
+        const double ox = ca.round_geo ? (double)(float)sx[o] : sx[o];
+        const double oy = ca.round_geo ? (double)(float)sy[o] : sy[o];
+        const double oz = ca.round_geo ? (double)(float)sz[o] : sz[o];
+        for (int t = tb + tid; t <= te; t += BS) {
+            const tint3 tr = a.tri[t];
+            const float3 p0 = rel_geo(tr.x - 1 - v0, o), p1 = rel_geo(tr.y - 1 - v0, o),
+                         p2 = rel_geo(tr.z - 1 - v0, o);
+            const float3 n = f3cross(f3sub(p1, p0), f3sub(p2, p0));
+            const float nn = sqrtf(f3dot(n, n));
+            const float at = 0.5f * nn;
+            const float in = 1.0f / nn;
+            ca.trinor[t] = tfloat3{n.x * in, n.y * in, n.z * in};
+            ca.triarea[t] = at;
+            if (ca.step == -1) ca.reftriarea[t] = at;
+            ca.tricen[t] = tfloat3{(float)(ox + ((double)p0.x + p1.x + p2.x) / 3.0),
+                                   (float)(oy + ((double)p0.y + p1.y + p2.y) / 3.0),
+                                   (float)(oz + ((double)p0.z + p1.z + p2.z) / 3.0)};
+            A += (double)at;
+            double ax = p0.x, ay = p0.y, az = p0.z, bx = p1.x, by = p1.y, bz = p1.z;
+            double cx = p2.x, cy = p2.y, cz = p2.z;
+            V += (ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx)) / 6.0;
+        }
+        sa[tid] = A; sv[tid] = V;
+    }
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if (tid < w) { sa[tid] += sa[tid + w]; sv[tid] += sv[tid + w]; }
+        __syncthreads();
+    }
+    const float Ac = (float)sa[0], Vc = (float)sv[0];
+    if (tid == 0) { ca.area[c] = Ac; ca.vol[c] = Vc; }
+    for (int t = tb + tid; t <= te; t += GC_CELL_BS) {
+        ca.area_para[t] = Ac; ca.vol_para[t] = Vc;
+        if (ca.step == -1) { ca.refarea_para[t] = Ac; ca.refvol_para[t] = Vc; }
+    }
+    // edges -> buf[12 (e - e0) + 3 role]
+    for (int e = e0 + tid; e < e0 + ne; e += GC_CELL_BS) {
+        const int6 E = a.edge[e];
+        const int ua = E.x - 1 - v0;
+        float3 f[4];
+        edge_terms(P, a, e, rel(E.y - 1 - v0, ua), rel(E.z - 1 - v0, ua), rel(E.k - 1 - v0, ua), f);
+        float* q = buf + 12 * (e - e0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { q[3 * k] = f[k].x; q[3 * k + 1] = f[k].y; q[3 * k + 2] = f[k].z; }
+    }
+    __syncthreads();
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    if (tid < nv) {
+        const int g = v0 + tid;
+        for (int q = a.es[g]; q < a.es[g + 1]; ++q) {
+            const int code = a.ee[q];
+            const float* w = buf + 12 * ((code >> 2) - e0) + 3 * (code & 3);
+            fx += w[0]; fy += w[1]; fz += w[2];
+        }
+    }
+    __syncthreads();
+    // triangles -> buf[9 (t - tb) + 3 corner]
+    for (int t = tb + tid; t <= te; t += GC_CELL_BS) {
+        const tint3 T = a.tri[t];
+        const int u0 = T.x - 1 - v0;
+        float3 f[3];
+        tri_terms(a, t, rel(T.y - 1 - v0, u0), rel(T.z - 1 - v0, u0), Ac, Vc, f);
+        float* q = buf + 9 * (t - tb);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { q[3 * k] = f[k].x; q[3 * k + 1] = f[k].y; q[3 * k + 2] = f[k].z; }
+    }
+    __syncthreads();
+    if (tid >= nv) return;
+    const int g = v0 + tid;
+    for (int q = a.ts[g]; q < a.ts[g + 1]; ++q) {
+        const int code = a.te[q];
+        const float* w = buf + 9 * ((code >> 2) - tb) + 3 * (code & 3);
+        fx += w[0]; fy += w[1]; fz += w[2];
+    }
+    a.defforce[g] = tfloat3{fx, fy, fz};
+    if (!a.force) return;   // membrane-only evaluation (substepping)
+    tfloat3 F = a.force[slot];
+    F.x += fx; F.y += fy; F.z += fz;
+    a.force[slot] = F;
+    if (a.vout) {  // this is the vertex's last force term -> corrector
+        const float h = 0.5f * P.dt * P.inv_m;
+        const tfloat3 vh = a.velhalf[slot];
+        a.vout[slot] = make_float4(vh.x + h * F.x, vh.y + h * F.y, vh.z + h * F.z, a.velrho[slot].w);
+    }
+}
+
+bool cell_ranges(int nm, int nedge, int ntri, int ncell, const int* celltab, const tint2* betri,
+                 const int6* edge, const tint3* tri, int4* range)
+{
+    if (ncell <= 0) return false;
+    std::vector<int> vfirst(ncell, -1), vcount(ncell, 0), efirst(ncell, -1), ecount(ncell, 0);
+    int prev = -1;
+    for (int v = 0; v < nm; ++v) {
+        const int c = celltab[v];
+        if (c < 0 || c >= ncell || c < prev) return false;
+        if (vfirst[c] < 0) vfirst[c] = v;
+        vcount[c]++;
+        prev = c;
+    }
+    prev = -1;
+    for (int e = 0; e < nedge; ++e) {
+        const int* q = &edge[e].x;
+        if (q[0] < 1 || q[0] > nm) return false;
+        const int c = celltab[q[0] - 1];
+        if (c < prev) return false;
+        for (int k = 1; k < 4; ++k)
+            if (q[k] < 1 || q[k] > nm || celltab[q[k] - 1] != c) return false;
+        if (efirst[c] < 0) efirst[c] = e;
+        ecount[c]++;
+        prev = c;
+    }
+    std::vector<char> covered(ntri, 0);
+    for (int c = 0; c < ncell; ++c) {
+        const int tb = betri[c].x, te = betri[c].y;
+        if (vcount[c] < 1 || vcount[c] > GC_CELL_NV || ecount[c] > GC_CELL_NE) return false;
+        if (tb < 0 || te < tb || te >= ntri || te - tb + 1 > GC_CELL_NT) return false;
+        for (int t = tb; t <= te; ++t) {
+            if (covered[t]) return false;
+            covered[t] = 1;
+        }
+        for (int t = tb; t <= te; ++t) {
+            const int* q = &tri[t].x;
+            for (int k = 0; k < 3; ++k)
+                if (q[k] < 1 || q[k] > nm || celltab[q[k] - 1] != c) return false;
+        }
+        range[c] = make_int4(vfirst[c], vcount[c], efirst[c] < 0 ? 0 : efirst[c], ecount[c]);
+    }
+    for (int t = 0; t < ntri; ++t)
+        if (!covered[t]) return false;
+    return true;
+}
+
 // ---- CalCellAggForce ------------------------------------------------------
 __global__ __launch_bounds__(BS) void k_agg_force(int n, GcParams P, const tfloat3* __restrict__ coor,
     tfloat3* __restrict__ aggforce, const tint3* __restrict__ cc, const int* __restrict__ ncube,

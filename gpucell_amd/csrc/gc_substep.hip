@@ -396,9 +396,6 @@ int3 per3(const gc_engine* e) { return make_int3(e->ext.periodic[0], e->ext.peri
 int ms_force(gc_engine* e, MembraneSub* m)
 {
     hipStream_t s = e->s;
-    k_geo_cell<<<e->ncell, BS, 0, s>>>(m->Pm, e->step, nullptr, nullptr, e->d_bbox, m->mid, e->d_tri,
-        e->d_betri, m->mnt, m->mcoor, e->d_trinor, e->d_triarea, e->d_reftriarea, e->d_tricen,
-        e->d_refarea_para, e->d_refvol_para, e->d_area_para, e->d_vol_para, e->d_area, e->d_vol);
     DefArgs a;
     a.edge = e->d_edge; a.tri = e->d_tri; a.coor = m->mcoor; a.nthro = m->mnt; a.curpos = m->mid;
     a.lmax = e->d_lmax; a.mexp = e->d_mexp; a.p = e->d_p; a.kp = e->d_kp; a.th0 = e->d_th0;
@@ -407,10 +404,15 @@ int ms_force(gc_engine* e, MembraneSub* m)
     a.V0c = e->d_refvol_para; a.es = e->d_es; a.ee = e->d_ee; a.ts = e->d_ts; a.te = e->d_te;
     a.defforce = m->fm; a.force = nullptr;
     a.mpos = m->mx; a.nm = m->nm;
-    const int dg = 64 << e->tune.v[TUNE_MEMBRANE];
-    k_def_force<<<xcd_grid(blocks_for(m->nm, dg)), dg, 0, s>>>(m->Pm, m->nm, e->d_bbox, a);
-    GC_LAUNCH_CHECK("membrane substep force");
-    return GC_OK;
+    if (use_cell_kernel(e)) {   // geometry from the FP32 positions, as k_geo_cell below
+        k_cell_membrane<<<e->ncell, GC_CELL_BS, 0, s>>>(m->Pm, e->d_bbox, cell_args(e, e->step, 1), a);
+        GC_LAUNCH_CHECK("membrane substep (per cell)");
+        return GC_OK;
+    }
+    k_geo_cell<<<e->ncell, BS, 0, s>>>(m->Pm, e->step, nullptr, nullptr, e->d_bbox, m->mid, e->d_tri,
+        e->d_betri, m->mnt, m->mcoor, e->d_trinor, e->d_triarea, e->d_reftriarea, e->d_tricen,
+        e->d_refarea_para, e->d_refvol_para, e->d_area_para, e->d_vol_para, e->d_area, e->d_vol);
+    return membrane_force(e, m->Pm, a, s);
 }
 
 FeatArgs feat_args(gc_engine* e, MembraneSub* m, bool normalise)

@@ -1,7 +1,8 @@
 """Autotuner (gc_EngineAutotune): every launch-shape variant of every tuned
-kernel family computes bit-identical states, and tuning in the middle of a
-run leaves the trajectory unchanged (the tuner restores the state it
-started from)."""
+kernel family computes bit-identical states (the membrane family includes
+the fused per-cell kernel against the three-pass form), and tuning in the
+middle of a run leaves the trajectory unchanged (the tuner restores the
+state it started from)."""
 import numpy as np
 import pytest
 
@@ -11,7 +12,7 @@ from tests import util
 
 pytestmark = pytest.mark.gpu
 
-NVAR = [3, 12, 3, 3]
+NVAR = [3, 12, 3, 4]
 KEYS = ("pos", "velrhop", "velhalf", "force", "nthrobound", "defforce", "area", "vol")
 
 
@@ -66,3 +67,22 @@ def test_autotune_keeps_trajectory_and_reports_each_variant(gpu):
     chosen = {r["family"]: r["variant"] for r in rows if r["chosen"]}
     t = a.tuning()
     assert t == [chosen["integrate"], chosen["density"], chosen["sdpd force"], chosen["membrane force"]]
+
+
+def test_membrane_substeps_per_cell_kernel_bit_identical(gpu):
+    """Plain membrane substepping evaluates the membrane force 8 times per
+    step: the per-cell kernel (variant 3) and the three-pass form (variant
+    0) give the same bits."""
+    c = case()
+    outs = []
+    for v in (0, 3):
+        e = Engine(c)
+        t = e.tuning()
+        t[3] = v
+        e.set_tuning(t)
+        e.set_membrane_mode(1, k=8)
+        e.step(3)
+        outs.append(e.state())
+        e.close()
+    for k in KEYS:
+        assert np.array_equal(outs[0][k], outs[1][k]), k
