@@ -38,6 +38,7 @@
 // for bit as its solo run.
 #include "gc_engine_state.h"
 
+#include <algorithm>
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -55,6 +56,7 @@ constexpr int BS = GC_BS;
 // more than NL_CAP neighbours gets count -1 and the force pass scans its
 // cubes instead.
 constexpr int NL_CAP = 64;
+constexpr int CUBE_SCAN_BS = 256, CUBE_IPT = 4, CUBE_TILE = CUBE_SCAN_BS * CUBE_IPT;
 
 // Image code of a range: 2 bits per axis, 1 = shift -L, 2 = shift +L.
 __device__ __forceinline__ int shift_code(float sx, float sy, float sz)
@@ -219,28 +221,96 @@ __global__ __launch_bounds__(TBS) void k2_integrate(int n, GcRep R0, const GcRep
     if (act) rank[p] = base + (lane - myhead);
 }
 
+// [4] cube counts per tile of CUBE_TILE cubes
+__global__ __launch_bounds__(CUBE_SCAN_BS) void k2_tile_sums(int K, const int* __restrict__ cnt,
+                                                             int* __restrict__ tsum)
+{
+    __shared__ int red[CUBE_SCAN_BS];
+    const int tid = threadIdx.x, base = blockIdx.x * CUBE_TILE + tid * CUBE_IPT;
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < CUBE_IPT; ++k) t += base + k < K ? cnt[base + k] : 0;
+    red[tid] = t;
+    __syncthreads();
+    for (int o = CUBE_SCAN_BS / 2; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) tsum[blockIdx.x] = red[0];
+}
+
+// [4] BeginEndCube (exclusive offsets of the cube counts, total at K): block
+// b sums the totals of the tiles before it and scans its own tile.  In a
+// step it also advances the noise counter (the force pass of this step
+// reads the value before the advance).
+__global__ __launch_bounds__(CUBE_SCAN_BS) void k2_cube_offsets(int K, const int* __restrict__ cnt,
+    const int* __restrict__ tsum, int* __restrict__ offs, unsigned long long* __restrict__ ctr, int bump)
+{
+    __shared__ int red[CUBE_SCAN_BS];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    int pre = 0;
+    for (int i = tid; i < b; i += CUBE_SCAN_BS) pre += tsum[i];
+    red[tid] = pre;
+    __syncthreads();
+    for (int o = CUBE_SCAN_BS / 2; o > 0; o >>= 1) {
+        if (tid < o) red[tid] += red[tid + o];
+        __syncthreads();
+    }
+    const int before = red[0];
+    __syncthreads();
+    const int base = b * CUBE_TILE + tid * CUBE_IPT;
+    int v[CUBE_IPT];
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < CUBE_IPT; ++k) {
+        v[k] = base + k < K ? cnt[base + k] : 0;
+        t += v[k];
+    }
+    red[tid] = t;
+    __syncthreads();
+    for (int o = 1; o < CUBE_SCAN_BS; o <<= 1) {
+        int a = tid >= o ? red[tid - o] : 0;
+        __syncthreads();
+        red[tid] += a;
+        __syncthreads();
+    }
+    int run = before + red[tid] - t;
+#pragma unroll
+    for (int k = 0; k < CUBE_IPT; ++k) {
+        if (base + k < K) offs[base + k] = run;
+        run += v[k];
+    }
+    if (b == (int)gridDim.x - 1 && tid == CUBE_SCAN_BS - 1) offs[K] = before + red[CUBE_SCAN_BS - 1];
+    if (bump && b == 0 && tid == 0) *ctr += 1ull;
+}
+
 // [4] (cube, id) into cube segments; clears the histogram for the next step
 // (every non-empty cube has exactly one rank-0 member).
 __global__ __launch_bounds__(BS) void k2_scatter(int n, const int* __restrict__ key,
     const int* __restrict__ rank, const int* __restrict__ sp, const int* __restrict__ offs,
-    int* __restrict__ tk, int* __restrict__ tv, int* __restrict__ cnt)
+    int* __restrict__ tk, int* __restrict__ tv, int* __restrict__ to, int* __restrict__ cnt)
 {
     int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (p >= n) return;
     int k = key[p], r = rank[p];
     int q = offs[k] + r;
+    if (q < 0 || q >= n) return;   // only if the counts were inconsistent
     tk[q] = k;
     tv[q] = sp[p];
+    to[q] = p;
     if (r == 0) cnt[k] = 0;
 }
 
 // [1,2,4,5,6,8] final slot = cube start + number of smaller ids in the
-// cube; gather of the particle from its old slot curpos[id], advanced as in
-// the key pass (move); predictor.
+// cube; gather of the particle from its old slot (carried by the scatter),
+// advanced as in the key pass (move); predictor.  CurPos is kept for the
+// membrane vertices only (the membrane kernels read it; the whole map is
+// the inverse of SortPart).
 template <bool ENS>
 __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, int move, int mmove, const int* __restrict__ tk,
-    const int* __restrict__ tv, const int* __restrict__ offs, int* __restrict__ curpos,
+    const int* __restrict__ tv, const int* __restrict__ to, const int* __restrict__ offs,
+    int* __restrict__ curpos,
     const double* __restrict__ pos_old, double* __restrict__ pos_new,
     const float4* __restrict__ vout, const tfloat3* __restrict__ force,
     const tint3* __restrict__ nt_old, tint3* __restrict__ nt_new, int* __restrict__ sp_new,
@@ -250,11 +320,11 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     int q = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (q >= n) return;
     int k = tk[q], id = tv[q];
-    int b = offs[k], e = offs[k + 1];
+    int b = max(offs[k], 0), e = min(offs[k + 1], n);
     int r = 0;
     for (int t = b; t < e; ++t) r += tv[t] < id;
     int s = b + r;
-    int o = curpos[id];
+    int o = to[q];
     double x = pos_old[o], y = pos_old[n + o], z = pos_old[2 * n + o];
     tint3 nt = nt_old[o];
     float4 v = vout[o];
@@ -281,7 +351,7 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     sp_new[s] = id;
     key_new[s] = k;
     velrho[s] = v;
-    curpos[id] = s;
+    if (id >= R0.P.nc) curpos[id] = s;
 }
 
 // [7,9] density of moving particles; velocity of Bound2 walls from the
@@ -468,7 +538,7 @@ __global__ __launch_bounds__(TBS) void k2_force(int n, GcRep R0, const GcRep* __
     if (id < R0.P.nf) { force[p] = tfloat3{0.f, 0.f, 0.f}; return; }
     const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
     const GcParams& P = R.P;   // a reference: a copy of the batch record would live in scratch
-    const unsigned long long c = *ctr;
+    const unsigned long long c = *ctr - 1ull;   // advanced by this step's offsets pass
     const uint32_t ctr_lo = (uint32_t)c, ctr_hi = (uint32_t)(c >> 32);
     const uint32_t seed_lo = P.seed_lo, seed_hi = P.seed_hi;
     const int lid = ENS ? local_id(R, id) : id;
@@ -489,9 +559,14 @@ __global__ __launch_bounds__(TBS) void k2_force(int n, GcRep R0, const GcRep* __
     float fx = 0.f, fy = 0.f, fz = 0.f;
     auto pair = [&](int code) {
         int j = code >> 6;
-        float sx = code_shift(code, L0), sy = code_shift(code >> 2, L1), sz = code_shift(code >> 4, L2);
         float4 pj = posid[j];
-        float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
+        float rx, ry, rz;
+        if (__ballot(code & 63)) {   // an image neighbour in the wave (x + 0 is x: same bits)
+            float sx = code_shift(code, L0), sy = code_shift(code >> 2, L1), sz = code_shift(code >> 4, L2);
+            rx = pi.x - (pj.x + sx); ry = pi.y - (pj.y + sy); rz = pi.z - (pj.z + sz);
+        } else {
+            rx = pi.x - pj.x; ry = pi.y - pj.y; rz = pi.z - pj.z;
+        }
         float r2 = rx * rx + ry * ry + rz * rz;
         if (r2 == 0.f) return;  // coincident particles exert no pair force
         int idj = __float_as_int(pj.w);
@@ -534,7 +609,12 @@ __global__ __launch_bounds__(TBS) void k2_force(int n, GcRep R0, const GcRep* __
     const int cnt = nlc[p];
     if (cnt >= 0) {
         const int* wl = nl + (size_t)__builtin_amdgcn_readfirstlane(p >> 6) * (NL_CAP * 64) + (p & 63);
-        for (int k = 0; k < cnt; ++k) pair(wl[64 * k]);
+        int next = cnt > 0 ? wl[0] : 0;   // the next entry is loaded during the current pair
+        for (int k = 0; k < cnt; ++k) {
+            const int code = next;
+            if (k + 1 < cnt) next = wl[64 * (k + 1)];
+            pair(code);
+        }
     } else {  // more than NL_CAP neighbours: scan the cubes again
         visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
             const int code = shift_code(sx, sy, sz);
@@ -645,7 +725,6 @@ __global__ __launch_bounds__(BS) void k2_wallvel(int n, GcRep R0, const GcRep* _
     vout[p] = make_float4(svx * s, svy * s, svz * s, P.rho0);
 }
 
-__global__ void k2_bump(unsigned long long* ctr) { *ctr += 1ull; }
 
 // Unwrapped membrane vertex positions in id order (FP64, x[nm] y[nm] z[nm]):
 // the membrane force kernel then reads a compact 24 B x Nm array instead of
@@ -711,14 +790,15 @@ int launch_build_t(gc_engine* e, int move, Prof* pr = nullptr)
     }
     GC_LAUNCH_CHECK("engine integrate");
     mark(pr, s, "integrate+key");
-    int st = exclusive_scan(e->d_cnt, e->d_offs, e->K, s, e->d_scanws);
-    if (st) return st;
+    k2_tile_sums<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum);
+    k2_cube_offsets<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, e->d_offs, e->d_ctr, move);
+    GC_LAUNCH_CHECK("engine cube offsets");
     mark(pr, s, "cube offsets");
     k2_scatter<<<gb, BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[c], e->d_offs, e->d_tk, e->d_tv,
-                                 e->d_cnt);
+                                 e->d_to, e->d_cnt);
     GC_LAUNCH_CHECK("engine scatter");
     k2_sortgather<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, mfree, e->d_tk, e->d_tv,
-        e->d_offs, e->d_curpos, e->d_pos64[c], e->d_pos64[nx], e->d_vout, e->d_force, e->d_nt[c],
+        e->d_to, e->d_offs, e->d_curpos, e->d_pos64[c], e->d_pos64[nx], e->d_vout, e->d_force, e->d_nt[c],
         e->d_nt[nx], e->d_sp[nx], e->d_key, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf);
     GC_LAUNCH_CHECK("engine sort/gather");
     e->cur = nx;
@@ -778,8 +858,6 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
     }
     GC_LAUNCH_CHECK("engine force");
     mark(pr, s, "sdpd force");
-    k2_bump<<<1, 1, 0, s>>>(e->d_ctr);
-    mark(pr, s, "noise counter");
     if (e->ncell > 0 && e->ms) {
         if (e->any_morse) {
             k2_agg<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid, e->d_offs,
@@ -1027,6 +1105,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
     }
     if (koff > (1L << 30)) return fail(set_error(GC_ERR_GRID, "EngineCreate: too many cubes"));
     e->K = (int)koff;
+    e->ntile = (e->K + CUBE_TILE - 1) / CUBE_TILE;
     if ((st = hip_check(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking), "engine stream")))
         return fail(st);
 
@@ -1045,9 +1124,10 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
         (st = e->alloc(&e->d_key0, N)) || (st = e->alloc(&e->d_key, N)) ||
         (st = e->alloc(&e->d_rank, N)) || (st = e->alloc(&e->d_cnt, K + 1)) ||
         (st = e->alloc(&e->d_offs, K + 1)) || (st = e->alloc(&e->d_tk, N)) ||
-        (st = e->alloc(&e->d_tv, N)) || (st = e->alloc(&e->d_curpos, N)) ||
+        (st = e->alloc(&e->d_tv, N)) || (st = e->alloc(&e->d_to, N)) || (st = e->alloc(&e->d_curpos, N)) ||
         (st = e->alloc(&e->d_nl, (size_t)NL_CAP * (((size_t)N + 63) & ~(size_t)63))) || (st = e->alloc(&e->d_nlc, N)) ||
-        (st = e->alloc(&e->d_scanws, scan_ws_ints(e->K))) || (st = e->alloc(&e->d_ctr, 1)))
+        (st = e->alloc(&e->d_ctr, 1)) ||
+        (st = e->alloc(&e->d_tsum, (size_t)e->ntile)))
         return fail(st);
     hipStream_t s = e->s;
     std::vector<float4> vout_id(N);
@@ -1082,7 +1162,8 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
             (st = upload(e->d_pos64[0], pos.data(), 3 * N, s)) ||
             (st = upload(e->d_vout, vout_id.data(), N, s)) || (st = upload(e->d_force, f_id.data(), N, s)) ||
             (st = upload(e->d_nt[0], nt.data(), N, s)) || (st = upload(e->d_sp[0], idn.data(), N, s)) ||
-            (st = upload(e->d_curpos, idn.data(), N, s)) || (st = upload(e->d_cnt, zero.data(), K + 1, s)) ||
+            (st = upload(e->d_curpos, idn.data(), N, s)) || (st = upload(e->d_to, idn.data(), N, s)) ||
+            (st = upload(e->d_cnt, zero.data(), K + 1, s)) ||
             (st = upload(e->d_ctr, &ctr, 1, s)))
             return fail(st);
     }
@@ -1427,12 +1508,16 @@ extern "C" int gc_EngineGetCells(gc_engine* e, gc_host_cells* o)
     auto dl = [&](void* h, const void* dv, size_t b) {
         return h ? hip_check(hipMemcpy(h, dv, b, hipMemcpyDeviceToHost), "GetCells D2H") : GC_OK;
     };
-    std::vector<int> key(N), offs(K + 1);
-    if ((st = dl(o->sortpart, e->d_sp[c], sizeof(int) * N)) ||
-        (st = dl(o->curpos, e->d_curpos, sizeof(int) * N)) ||
+    std::vector<int> key(N), offs(K + 1), sp(N);
+    if ((st = dl(sp.data(), e->d_sp[c], sizeof(int) * N)) ||
         (st = dl(key.data(), e->d_key, sizeof(int) * N)) ||
         (st = dl(offs.data(), e->d_offs, sizeof(int) * (K + 1))))
         return st;
+    if (o->sortpart) memcpy(o->sortpart, sp.data(), sizeof(int) * N);
+    // the step keeps CurPos on the device for membrane vertices only; the
+    // whole map is the inverse of SortPart
+    if (o->curpos)
+        for (int s = 0; s < N; ++s) o->curpos[sp[s]] = s;
     if (o->cubepart) memcpy(o->cubepart, key.data(), sizeof(int) * N);
     if (o->cc)
         for (int s = 0, r = 0; s < N; ++s) {  // slots of a replica are contiguous

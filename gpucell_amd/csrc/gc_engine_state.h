@@ -66,7 +66,9 @@ struct gc_engine {
     double* d_pos64[2]; tint3* d_nt[2]; int* d_sp[2];
     float4 *d_posid, *d_velrho, *d_vout;
     tfloat3 *d_coor, *d_velhalf, *d_force;
-    int *d_key0, *d_key, *d_rank, *d_cnt, *d_offs, *d_tk, *d_tv, *d_curpos, *d_scanws;
+    int *d_key0, *d_key, *d_rank, *d_cnt, *d_offs, *d_tk, *d_tv, *d_to, *d_curpos;
+    int* d_tsum = nullptr;   // cube counts per scan tile
+    int ntile = 0;
     int *d_nl, *d_nlc;   // neighbour lists of moving particles (density -> force)
     unsigned long long* d_ctr;
     // membrane
