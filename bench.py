@@ -26,17 +26,22 @@ HBM_PEAK_GBS = 8000.0
 # force+corrector 72) and per membrane vertex (440).
 STEP_BYTES_PER_PARTICLE = 286.0
 STEP_BYTES_PER_VERTEX = 440.0
-# Algorithmic bytes per particle of the single-kernel passes (neighbour
-# data from cache): integrate+wrap+key reads the FP64 position 24 B,
-# velocity 16 B and force 12 B and writes the key 4 B; density reads the
-# FP32 position 12 B and writes the density 4 B; the SDPD force pass (EOS,
-# walls, body force and corrector fused) reads position 12 B + predicted
-# velocity 12 B + density 4 B + id 4 B + half-step velocity 12 B and writes
+# Algorithmic bytes per particle of the passes (neighbour data from cache),
+# the five-pass dataflow above: integrate+wrap+key reads the FP64 position
+# 24 B, velocity 16 B and force 12 B and writes the key 4 B; the counting
+# sort and gather+predictor move 18.3 + 124 B; density reads the FP32
+# position 12 B and writes the density 4 B; the SDPD force pass (EOS, walls,
+# body force and corrector fused) reads position 12 B + predicted velocity
+# 12 B + density 4 B + id 4 B + half-step velocity 12 B and writes
 # velocity/density 16 B + force 12 B.
-KERNEL_BYTES_PER_PARTICLE = {"integrate+key": 56.0, "density+wall velocity": 16.0,
-                             "sdpd force": 72.0}
-KERNEL_NAMES = {"integrate+key": "k2_integrate", "density+wall velocity": "k2_density",
-                "sdpd force": "k2_force"}
+KERNEL_BYTES_PER_PARTICLE = {"integrate+key": 56.0, "sort+gather+predict": 142.3,
+                             "density+wall velocity": 16.0, "sdpd force": 72.0}
+KERNEL_NAMES = {"integrate+key": "k2_integrate", "sort+gather+predict": "k2_sortgather",
+                "density+wall velocity": "k2_density", "sdpd force": "k2_force"}
+# the bound of the neighbour-loop passes (instruction issue, not HBM)
+KERNEL_BOUND_NOTE = {
+    "density+wall velocity": "VALU issue in the 27-cube scan (~95 candidates per particle, lane imbalance)",
+    "sdpd force": "VALU issue: Philox-4x32-10 (20 quarter-rate 64-bit products per pair) + pair math"}
 
 
 def log(msg):
@@ -196,7 +201,7 @@ def main():
     prof = eng.profile(20)
     log("[bench] per-phase us: " + json.dumps(prof))
     # dominant single kernel of the step (HIP events on the engine stream)
-    dom = max(KERNEL_BYTES_PER_PARTICLE, key=lambda k: prof.get(k, 0.0))
+    dom = max(("integrate+key", "density+wall velocity", "sdpd force"), key=lambda k: prof.get(k, 0.0))
     dom_us = prof[dom]
     achieved = KERNEL_BYTES_PER_PARTICLE[dom] * N / (dom_us * 1e-6) / 1e9
     traffic = None
@@ -224,10 +229,17 @@ def main():
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "avg_us": dom_us,
-                     "algorithmic_bytes_per_particle": KERNEL_BYTES_PER_PARTICLE[dom]},
+                     "algorithmic_bytes_per_particle": KERNEL_BYTES_PER_PARTICLE[dom],
+                     "note": KERNEL_BOUND_NOTE.get(dom)},
         "step_roofline": {"bytes_per_step": step_bytes, "achieved": step_gbs,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
         "phases_us": prof,
+        "passes": {k: {"kernel": KERNEL_NAMES[k], "avg_us": prof.get(k),
+                       "algorithmic_bytes_per_particle": b,
+                       "achieved_gbs": (b * N / (prof[k] * 1e-6) / 1e9) if prof.get(k) else None,
+                       "frac_hbm": (b * N / (prof[k] * 1e-6) / 1e9 / HBM_PEAK_GBS) if prof.get(k) else None,
+                       "bound": KERNEL_BOUND_NOTE.get(k, "hbm")}
+                   for k, b in KERNEL_BYTES_PER_PARTICLE.items()},
         "tuning": None if tune_rows is None else {
             r["family"]: r["shape"] for r in tune_rows if r["chosen"]},
     }
