@@ -208,6 +208,20 @@ def main():
     pmc = os.path.join(ROOT, "profiles", f"traffic_{KERNEL_NAMES[dom]}_{a.case}.json")
     if os.path.exists(pmc):
         traffic = json.load(open(pmc)).get("bytes_per_launch")
+    # instruction-issue roofline of the dominant kernel: VALU instructions per
+    # wave from the committed counter profile of this case (rocprofv3 SQ
+    # counters), 4 cycles per wave64 instruction on each of 1024 SIMDs
+    valu = None
+    vp = os.path.join(ROOT, "profiles", f"valu_{a.case}.json")
+    if os.path.exists(vp):
+        vj = json.load(open(vp)).get("kernels", {}).get(KERNEL_NAMES[dom])
+        if vj:
+            waves = (N + 63) // 64
+            issue_us = vj["valu_per_wave"] * waves * 4 / (1024 * 2.4e9) * 1e6
+            valu = {"kernel": KERNEL_NAMES[dom], "valu_per_wave": vj["valu_per_wave"],
+                    "issue_bound_us": issue_us, "avg_us": dom_us, "frac": issue_us / dom_us,
+                    "peak": "1024 SIMDs x 16 lanes x 2.4 GHz (wave64 VALU instruction = 4 cycles)",
+                    "note": "quarter-rate instructions counted once: a lower bound"}
     step_bytes = STEP_BYTES_PER_PARTICLE * N + STEP_BYTES_PER_VERTEX * Nm
     step_gbs = step_bytes / (ms * 1e-3) / 1e9
     out = {
@@ -231,6 +245,7 @@ def main():
                      "traffic": traffic, "avg_us": dom_us,
                      "algorithmic_bytes_per_particle": KERNEL_BYTES_PER_PARTICLE[dom],
                      "note": KERNEL_BOUND_NOTE.get(dom)},
+        "valu_roofline": valu,
         "step_roofline": {"bytes_per_step": step_bytes, "achieved": step_gbs,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
         "phases_us": prof,
