@@ -640,16 +640,18 @@ __global__ __launch_bounds__(TBS) void k2_force(int n, GcRep R0, const GcRep* __
 // Morse partner is in it, in scan order); a vertex whose list overflowed
 // scans its cubes.
 template <bool ENS>
-__global__ __launch_bounds__(BS) void k2_agg(int n, GcRep R0, const GcRep* __restrict__ reps,
-    const int* __restrict__ rep_of, const float4* __restrict__ posid, const int* __restrict__ offs,
-    const int* __restrict__ key, const int* __restrict__ nl, const int* __restrict__ nlc,
-    const int* __restrict__ celltab, tfloat3* __restrict__ force, tfloat3* __restrict__ aggforce)
+// One thread per membrane vertex (its slot from CurPos).
+__global__ __launch_bounds__(BS) void k2_agg(int nm, GcRep R0, const GcRep* __restrict__ reps,
+    const int* __restrict__ rep_of, const int* __restrict__ curpos, const float4* __restrict__ posid,
+    const int* __restrict__ offs, const int* __restrict__ key, const int* __restrict__ nl,
+    const int* __restrict__ nlc, const int* __restrict__ celltab, tfloat3* __restrict__ force,
+    tfloat3* __restrict__ aggforce)
 {
-    int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
-    if (p >= n) return;
+    const int v = xcd_block((nm + BS - 1) / BS) * BS + threadIdx.x;
+    if (v >= nm) return;
+    const int p = curpos[R0.P.nc + v];
     float4 pi = posid[p];
     int id = __float_as_int(pi.w);
-    if (id < R0.P.nc) return;
     const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
     const GcParams& P = R.P;
     if (P.De == 0.0f) return;
@@ -673,7 +675,8 @@ __global__ __launch_bounds__(BS) void k2_agg(int n, GcRep R0, const GcRep* __res
     };
     const int cnt = nlc[p];
     if (cnt >= 0) {
-        const int* wl = nl + (size_t)__builtin_amdgcn_readfirstlane(p >> 6) * (NL_CAP * 64) + (p & 63);
+        // the lanes' slots are scattered: each lane's own list block
+        const int* wl = nl + (size_t)(p >> 6) * (NL_CAP * 64) + (p & 63);
         for (int k = 0; k < cnt; ++k) {
             const int code = wl[64 * k];
             morse(code >> 6, code_shift(code, R.L[0]), code_shift(code >> 2, R.L[1]),
@@ -860,7 +863,8 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
     mark(pr, s, "sdpd force");
     if (e->ncell > 0 && e->ms) {
         if (e->any_morse) {
-            k2_agg<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid, e->d_offs,
+            k2_agg<ENS><<<xcd_grid(blocks_for(e->nm, BS)), BS, 0, s>>>(e->nm, R0, e->d_reps, e->d_rep_of,
+                                          e->d_curpos, e->d_posid, e->d_offs,
                                           e->d_key, e->d_nl, e->d_nlc, e->d_celltab, e->d_force,
                                           e->d_aggforce);
             GC_LAUNCH_CHECK("engine morse");
@@ -871,7 +875,8 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
     } else if (e->ncell > 0) {
         const GcParams& P = R0.P;
         if (e->any_morse) {
-            k2_agg<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid, e->d_offs,
+            k2_agg<ENS><<<xcd_grid(blocks_for(e->nm, BS)), BS, 0, s>>>(e->nm, R0, e->d_reps, e->d_rep_of,
+                                          e->d_curpos, e->d_posid, e->d_offs,
                                           e->d_key, e->d_nl, e->d_nlc, e->d_celltab, e->d_force,
                                           e->d_aggforce);
             GC_LAUNCH_CHECK("engine morse");
