@@ -6,21 +6,26 @@
 // (gc_EnsembleCreate).
 //
 // Per step (reference launch numbers of UpdatePar in brackets):
-//   k2_integrate   [1,2,3]  r += dt v + dt^2/2 F/m in FP64, periodic wrap,
-//                           cube key, histogram rank (atomic)
-//   scan                    cube offsets (exclusive scan of the histogram)
-//   k2_scatter     [4]      (cube, id) pairs into cube segments
-//   k2_sortgather  [4,5,6,8] rank within the cube by id -> final slot;
-//                           gather of the state; Groot-Warren predictor
+//   k2_integrate   [1,2,3]  r + dt v + dt^2/2 F/m in FP64 and periodic wrap
+//                           (not stored), cube key, histogram rank (atomic)
+//   k2_tile_sums,  [4]      cube offsets: tile totals, then each tile's scan
+//   k2_cube_offsets         with the preceding totals; noise counter advance
+//   k2_scatter     [4]      (cube, id, old slot) into cube segments
+//   k2_sortgather  [1,2,4,5,6,8] rank within the cube by id -> final slot;
+//                           gather of the state advanced as in the key pass;
+//                           Groot-Warren predictor
 //   k2_density     [7,9]    number density (moving) and no-slip wall
 //                           velocity (Bound2) in one neighbour pass; writes
 //                           each moving particle's neighbour list
 //   k2_force       [10-13,17] EOS on the fly, SDPD pair force, Bound1
 //                           repulsion, body force over that list; corrector
 //                           for fluid
-//   k_geo_cell     [14]     membrane geometry, FP64 cell area/volume
-//   k2_agg         [16]     Morse between cells (when De != 0)
-//   k2_mgather, k_def_force [15,17] membrane forces, corrector for membrane
+//   k2_agg         [16]     Morse between cells (when De != 0), one thread
+//                           per membrane vertex
+//   k_cell_membrane [14,15,17] per cell: geometry, FP64 cell area/volume,
+//                           membrane forces, corrector for membrane (the
+//                           three-pass form k_geo_cell + k2_mgather +
+//                           k_edge/tri/vert_force for cells that do not fit)
 //   k2_wallvel     [18]     wall velocity from the corrected flow (on export)
 //
 // Layouts (slot order): posid float4 (x, y, z, id bits), velrho float4
