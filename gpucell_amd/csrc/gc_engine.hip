@@ -1298,7 +1298,10 @@ extern "C" int gc_EngineStepGraph(gc_engine* e, int nsteps, int driveon)
 {
     if (!e || nsteps < 0) return set_error(GC_ERR_INVALID, "EngineStepGraph: bad arguments");
     if (nsteps == 0) return GC_OK;
-    if (e->ms) return gc_EngineStep(e, nsteps, driveon);   // host decides fallbacks: eager
+    // membrane substepping replays from the graph when it reads nothing back
+    // on the host: no sample recording, and the surrogate's fallback decided
+    // on the device (the per-cell membrane kernel)
+    if (e->ms && !gc::ms_graph_ok(e)) return gc_EngineStep(e, nsteps, driveon);
     // A step alternates the double buffers, so one graph covers two steps
     // starting from buffer 0.
     if (e->cur != 0) {

@@ -113,6 +113,8 @@ CellArgs cell_args(gc_engine* e, long step, int round_geo);
 int ms_step(gc_engine* e);
 // Re-reads the membrane state from the engine arrays (after SetState).
 int ms_reset(gc_engine* e);
+// the membrane substep reads nothing back on the host (graph capture is safe)
+bool ms_graph_ok(const gc_engine* e);
 }  // namespace gc
 
 struct gc_mlp;

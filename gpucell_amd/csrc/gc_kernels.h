@@ -128,6 +128,7 @@ struct CellArgs {
     int round_geo;        // geometry from FP32-rounded positions (membrane substeps)
     tfloat3* trinor; float* triarea; float* reftriarea; tfloat3* tricen;
     float* refarea_para; float* refvol_para; float* area_para; float* vol_para; float* area; float* vol;
+    const int* gate = nullptr;   // non-null: run only when *gate != 0 (device-decided fallback)
 };
 __global__ void k_cell_membrane(GcParams P, const float* __restrict__ bbox, CellArgs ca, DefArgs a);
 // host check of the conditions above; fills range (C entries)

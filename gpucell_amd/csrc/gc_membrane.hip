@@ -542,6 +542,7 @@ __global__ __launch_bounds__(GC_CELL_BS) void k_cell_membrane(GcParams P, const 
     __shared__ double sx[GC_CELL_NV], sy[GC_CELL_NV], sz[GC_CELL_NV];
     __shared__ float buf[GC_CELL_NE * 12];
     __shared__ double sa[BS], sv[BS];
+    if (ca.gate && !*ca.gate) return;   // block-uniform
     const int c = blockIdx.x, tid = threadIdx.x;
     const int4 r = ca.range[c];
     const int v0 = r.x, nv = r.y, e0 = r.z, ne = r.w;

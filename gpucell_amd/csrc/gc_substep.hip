@@ -77,10 +77,10 @@ __global__ void k_ms_fext(int nm, int nc, const int* __restrict__ curpos, const 
 
 __global__ void k_ms_kick_drift(int nm, float dts, float inv_m, double* __restrict__ mx, tfloat3* __restrict__ mv,
                                 const tfloat3* __restrict__ fe, const tfloat3* __restrict__ fm,
-                                tfloat3* __restrict__ mcoor)
+                                tfloat3* __restrict__ mcoor, const int* __restrict__ gate)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nm) return;
+    if (v >= nm || (gate && !*gate)) return;
     const float h = 0.5f * dts * inv_m;
     tfloat3 u = mv[v], a = fe[v], b = fm[v];
     u.x += h * (a.x + b.x); u.y += h * (a.y + b.y); u.z += h * (a.z + b.z);
@@ -91,10 +91,10 @@ __global__ void k_ms_kick_drift(int nm, float dts, float inv_m, double* __restri
 }
 
 __global__ void k_ms_kick(int nm, float dts, float inv_m, tfloat3* __restrict__ mv, const tfloat3* __restrict__ fe,
-                          const tfloat3* __restrict__ fm)
+                          const tfloat3* __restrict__ fm, const int* __restrict__ gate)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= nm) return;
+    if (v >= nm || (gate && !*gate)) return;
     const float h = 0.5f * dts * inv_m;
     tfloat3 u = mv[v], a = fe[v], b = fm[v];
     u.x += h * (a.x + b.x); u.y += h * (a.y + b.y); u.z += h * (a.z + b.z);
@@ -107,9 +107,10 @@ __global__ void k_ms_end(int nm, int nc, int N, const int* __restrict__ curpos, 
                          const tfloat3* __restrict__ mv, const tfloat3* __restrict__ fm,
                          const float* __restrict__ bbox, int3 per, double* __restrict__ pos64,
                          tint3* __restrict__ nt, tfloat3* __restrict__ coor, float4* __restrict__ vout,
-                         tfloat3* __restrict__ defforce)
+                         tfloat3* __restrict__ defforce, int* __restrict__ flag)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v == 0) flag[2] += 1;   // steps in this mode
     if (v >= nm) return;
     int s = curpos[nc + v];
     double u[3] = {mx[v], mx[nm + v], mx[2 * nm + v]};
@@ -341,12 +342,38 @@ __global__ __launch_bounds__(BS) void k_ms_finish(int nm, int kick, float dts, f
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
-__global__ void k_ms_res_final(int nb, int nm, const float* __restrict__ part, float* __restrict__ out)
+// out[0] = rms residual; with flag: flag[0] = fallback needed (R > thr or
+// not a number), flag[1] counts the fallbacks
+__global__ void k_ms_res_final(int nb, int nm, const float* __restrict__ part, float* __restrict__ out,
+                               float thr, int* __restrict__ flag)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     float s = 0.f;
     for (int i = 0; i < nb; ++i) s += part[i];
-    out[0] = sqrtf(s / (float)nm);
+    const float r = sqrtf(s / (float)nm);
+    out[0] = r;
+    if (flag) {
+        const int fb = !(r <= thr);
+        flag[0] = fb;
+        flag[1] += fb;
+    }
+}
+
+// device copies for saving and restoring the substep state (graph-safe; with
+// a gate only when *gate != 0)
+__global__ void k_ms_copy_d(int n, const double* __restrict__ a, double* __restrict__ b,
+                            const int* __restrict__ gate)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || (gate && !*gate)) return;
+    b[i] = a[i];
+}
+__global__ void k_ms_copy_f3(int n, const tfloat3* __restrict__ a, tfloat3* __restrict__ b,
+                             const int* __restrict__ gate)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || (gate && !*gate)) return;
+    b[i] = a[i];
 }
 
 }  // namespace
@@ -364,11 +391,11 @@ struct MembraneSub {
     int* mid; tint3* mnt;
     int* ring; float* lref;
     float *frame, *feat, *pred, *norm = nullptr, *part, *res;
+    int* flag = nullptr;   // [0] fallback this step, [1] fallbacks, [2] steps (device counters)
     bool have_norm = false;
     float* rec_x = nullptr; float* rec_y = nullptr;
     long rec_cap = 0, rec_count = 0;
-    long steps = 0, fallbacks = 0;
-    float last_res = 0.f;
+    long fallbacks = 0;   // decided on the host (three-pass membrane path)
     std::vector<void*> allocs;
 
     template <class T> int alloc(T** p, size_t n)
@@ -393,7 +420,7 @@ namespace {
 int3 per3(const gc_engine* e) { return make_int3(e->ext.periodic[0], e->ext.periodic[1], e->ext.periodic[2]); }
 
 // membrane force (and geometry) at the membrane system's positions
-int ms_force(gc_engine* e, MembraneSub* m)
+int ms_force(gc_engine* e, MembraneSub* m, const int* gate = nullptr)
 {
     hipStream_t s = e->s;
     DefArgs a;
@@ -405,7 +432,9 @@ int ms_force(gc_engine* e, MembraneSub* m)
     a.defforce = m->fm; a.force = nullptr;
     a.mpos = m->mx; a.nm = m->nm;
     if (use_cell_kernel(e)) {   // geometry from the FP32 positions, as k_geo_cell below
-        k_cell_membrane<<<e->ncell, GC_CELL_BS, 0, s>>>(m->Pm, e->d_bbox, cell_args(e, e->step, 1), a);
+        CellArgs ca = cell_args(e, e->step, 1);
+        ca.gate = gate;
+        k_cell_membrane<<<e->ncell, GC_CELL_BS, 0, s>>>(m->Pm, e->d_bbox, ca, a);
         GC_LAUNCH_CHECK("membrane substep (per cell)");
         return GC_OK;
     }
@@ -430,50 +459,51 @@ FeatArgs feat_args(gc_engine* e, MembraneSub* m, bool normalise)
 int save_start(gc_engine* e, MembraneSub* m)
 {
     hipStream_t s = e->s;
-    const size_t nm = (size_t)m->nm;
-    int st;
-    if ((st = hip_check(hipMemcpyAsync(m->mx0, m->mx, sizeof(double) * 3 * nm, hipMemcpyDeviceToDevice, s), "ms save")) ||
-        (st = hip_check(hipMemcpyAsync(m->mv0, m->mv, sizeof(tfloat3) * nm, hipMemcpyDeviceToDevice, s), "ms save")) ||
-        (st = hip_check(hipMemcpyAsync(m->fm0, m->fm, sizeof(tfloat3) * nm, hipMemcpyDeviceToDevice, s), "ms save")))
-        return st;
+    const int nm = m->nm;
+    k_ms_copy_d<<<blocks_for(3 * nm, BS), BS, 0, s>>>(3 * nm, m->mx, m->mx0, nullptr);
+    k_ms_copy_f3<<<blocks_for(nm, BS), BS, 0, s>>>(nm, m->mv, m->mv0, nullptr);
+    k_ms_copy_f3<<<blocks_for(nm, BS), BS, 0, s>>>(nm, m->fm, m->fm0, nullptr);
+    GC_LAUNCH_CHECK("ms save");
     return GC_OK;
 }
 
-int restore_start(gc_engine* e, MembraneSub* m)
+// back to the saved start state (only when *gate != 0 if gate is given)
+int restore_start(gc_engine* e, MembraneSub* m, const int* gate = nullptr)
 {
     hipStream_t s = e->s;
-    const size_t nm = (size_t)m->nm;
-    int st;
-    if ((st = hip_check(hipMemcpyAsync(m->mx, m->mx0, sizeof(double) * 3 * nm, hipMemcpyDeviceToDevice, s), "ms restore")) ||
-        (st = hip_check(hipMemcpyAsync(m->mv, m->mv0, sizeof(tfloat3) * nm, hipMemcpyDeviceToDevice, s), "ms restore")) ||
-        (st = hip_check(hipMemcpyAsync(m->fm, m->fm0, sizeof(tfloat3) * nm, hipMemcpyDeviceToDevice, s), "ms restore")))
-        return st;
+    const int nm = m->nm;
+    k_ms_copy_d<<<blocks_for(3 * nm, BS), BS, 0, s>>>(3 * nm, m->mx0, m->mx, gate);
+    k_ms_copy_f3<<<blocks_for(nm, BS), BS, 0, s>>>(nm, m->mv0, m->mv, gate);
+    k_ms_copy_f3<<<blocks_for(nm, BS), BS, 0, s>>>(nm, m->fm0, m->fm, gate);
+    GC_LAUNCH_CHECK("ms restore");
     return GC_OK;
 }
 
-int plain_substeps(gc_engine* e, MembraneSub* m, bool record, long row0)
+// k substeps; with a gate (the surrogate's device-decided fallback) every
+// kernel runs only when *gate != 0 and the exact trajectory's residual is
+// not formed
+int plain_substeps(gc_engine* e, MembraneSub* m, bool record, long row0, const int* gate = nullptr)
 {
     hipStream_t s = e->s;
     const int nb = blocks_for(m->nm, BS);
     const float dts = e->hreps[0].P.dt / (float)m->k, inv_m = e->hreps[0].P.inv_m;
     int st;
     for (int q = 1; q <= m->k; ++q) {
-        k_ms_kick_drift<<<nb, BS, 0, s>>>(m->nm, dts, inv_m, m->mx, m->mv, m->fe, m->fm, m->mcoor);
-        if ((st = ms_force(e, m))) return st;
-        k_ms_kick<<<nb, BS, 0, s>>>(m->nm, dts, inv_m, m->mv, m->fe, m->fm);
+        k_ms_kick_drift<<<nb, BS, 0, s>>>(m->nm, dts, inv_m, m->mx, m->mv, m->fe, m->fm, m->mcoor, gate);
+        if ((st = ms_force(e, m, gate))) return st;
+        k_ms_kick<<<nb, BS, 0, s>>>(m->nm, dts, inv_m, m->mv, m->fe, m->fm, gate);
         if (record)
             k_ms_target<<<nb, BS, 0, s>>>(m->nm, q, dts, m->mx, m->mx0, m->mv0, m->frame, m->lref,
                                            m->rec_y + (size_t)NO * row0);
         GC_LAUNCH_CHECK("membrane substep");
-        if (m->k == KSUB && (q == KSUB - 2 || q == KSUB - 1) &&
-            (st = hip_check(hipMemcpyAsync(q == KSUB - 2 ? m->mx6 : m->mx7, m->mx, sizeof(double) * 3 * (size_t)m->nm,
-                                           hipMemcpyDeviceToDevice, s), "membrane substep copy")))
-            return st;
+        if (!gate && m->k == KSUB && (q == KSUB - 2 || q == KSUB - 1))
+            k_ms_copy_d<<<blocks_for(3 * m->nm, BS), BS, 0, s>>>(3 * m->nm, m->mx, q == KSUB - 2 ? m->mx6 : m->mx7,
+                                                                 nullptr);
     }
-    if (m->k == KSUB) {  // the residual of the exact trajectory (scale for the threshold)
+    if (!gate && m->k == KSUB) {  // the residual of the exact trajectory (scale for the threshold)
         k_ms_d2x<<<nb, BS, 0, s>>>(m->nm, m->mx, m->mx7, m->mx6, m->d2x);
         k_ms_finish<<<nb, BS, 0, s>>>(m->nm, 0, dts, inv_m, m->mv, m->fe, m->fm, m->d2x, m->lref, m->part);
-        k_ms_res_final<<<1, 1, 0, s>>>(nb, m->nm, m->part, m->res + 1);
+        k_ms_res_final<<<1, 1, 0, s>>>(nb, m->nm, m->part, m->res + 1, 0.0f, nullptr);
         GC_LAUNCH_CHECK("membrane plain residual");
     }
     return GC_OK;
@@ -491,6 +521,13 @@ int ms_reset(gc_engine* e)
                                    e->d_nt[e->cur], e->d_vout, e->d_bbox, per3(e), m->mx, m->mv, m->mcoor);
     GC_LAUNCH_CHECK("membrane substep init");
     return ms_force(e, m);
+}
+
+bool ms_graph_ok(const gc_engine* e)
+{
+    const MembraneSub* m = e->ms;
+    if (m->mode == 1) return m->rec_x == nullptr;
+    return use_cell_kernel(e);
 }
 
 int ms_step(gc_engine* e)
@@ -525,21 +562,28 @@ int ms_step(gc_engine* e)
         GC_LAUNCH_CHECK("membrane surrogate apply");
         if ((st = ms_force(e, m))) return st;
         k_ms_finish<<<nb, BS, 0, s>>>(nm, 1, dts, inv_m, m->mv, m->fe, m->fm, m->d2x, m->lref, m->part);
-        k_ms_res_final<<<1, 1, 0, s>>>(nb, nm, m->part, m->res);
-        GC_LAUNCH_CHECK("membrane residual");
-        float r = 0.f;
-        if ((st = hip_check(hipMemcpyAsync(&r, m->res, sizeof(float), hipMemcpyDeviceToHost, s), "residual D2H")) ||
-            (st = hip_check(hipStreamSynchronize(s), "residual sync")))
-            return st;
-        m->last_res = r;
-        if (!(r <= m->thr)) {   // also catches NaN
-            m->fallbacks += 1;
-            if ((st = restore_start(e, m)) || (st = plain_substeps(e, m, false, 0))) return st;
+        if (use_cell_kernel(e)) {
+            // the fallback is decided on the device: the residual kernel sets
+            // the flag that gates the restore and the plain substeps (no host
+            // read; the step can be replayed from a graph)
+            k_ms_res_final<<<1, 1, 0, s>>>(nb, nm, m->part, m->res, m->thr, m->flag);
+            GC_LAUNCH_CHECK("membrane residual");
+            if ((st = restore_start(e, m, m->flag)) || (st = plain_substeps(e, m, false, 0, m->flag))) return st;
+        } else {
+            k_ms_res_final<<<1, 1, 0, s>>>(nb, nm, m->part, m->res, 0.0f, nullptr);
+            GC_LAUNCH_CHECK("membrane residual");
+            float r = 0.f;
+            if ((st = hip_check(hipMemcpyAsync(&r, m->res, sizeof(float), hipMemcpyDeviceToHost, s), "residual D2H")) ||
+                (st = hip_check(hipStreamSynchronize(s), "residual sync")))
+                return st;
+            if (!(r <= m->thr)) {   // also catches NaN
+                m->fallbacks += 1;
+                if ((st = restore_start(e, m)) || (st = plain_substeps(e, m, false, 0))) return st;
+            }
         }
     }
-    m->steps += 1;
     k_ms_end<<<nb, BS, 0, s>>>(nm, nc, e->N, e->d_curpos, m->mx, m->mv, m->fm, e->d_bbox, per3(e),
-                               e->d_pos64[e->cur], e->d_nt[e->cur], e->d_coor, e->d_vout, e->d_defforce);
+                               e->d_pos64[e->cur], e->d_nt[e->cur], e->d_coor, e->d_vout, e->d_defforce, m->flag);
     GC_LAUNCH_CHECK("membrane write-back");
     return GC_OK;
 }
@@ -619,7 +663,7 @@ extern "C" int gc_EngineSetMembraneMode(gc_engine* e, int mode, int k, gc_mlp* m
             (st = m->alloc(&m->lref, nm)) || (st = m->alloc(&m->frame, 9 * nm)) ||
             (st = m->alloc(&m->feat, (size_t)NF * nm)) || (st = m->alloc(&m->pred, (size_t)NO * nm)) ||
             (st = m->alloc(&m->norm, 2 * (NF + NO))) || (st = m->alloc(&m->part, (size_t)blocks_for(m->nm, BS))) ||
-            (st = m->alloc(&m->res, 2))) {
+            (st = m->alloc(&m->res, 2)) || (st = m->alloc(&m->flag, 4))) {
             ms_destroy(m);
             return st;
         }
@@ -651,7 +695,8 @@ extern "C" int gc_EngineSetMembraneMode(gc_engine* e, int mode, int k, gc_mlp* m
         }
         hipStream_t s = e->s;
         if ((st = upload(m->ring, ring.data(), ring.size(), s)) || (st = upload(m->lref, lref.data(), nm, s)) ||
-            (st = upload(m->mid, mid.data(), nm, s)) || (st = upload(m->mnt, zero.data(), nm, s))) {
+            (st = upload(m->mid, mid.data(), nm, s)) || (st = upload(m->mnt, zero.data(), nm, s)) ||
+            (st = hip_check(hipMemsetAsync(m->flag, 0, 4 * sizeof(int), s), "ms flags"))) {
             ms_destroy(m);
             return st;
         }
@@ -696,13 +741,15 @@ extern "C" int gc_EngineMembraneStats(gc_engine* e, gc_membrane_stats* out)
     if (!e || !out) return set_error(GC_ERR_INVALID, "MembraneStats: null");
     memset(out, 0, sizeof(*out));
     if (!e->ms) return GC_OK;
-    out->steps = e->ms->steps;
-    out->fallbacks = e->ms->fallbacks;
     out->samples = e->ms->rec_count;
-    out->last_residual = e->ms->last_res;
     float r[2] = {0.f, 0.f};
+    int f[3] = {0, 0, 0};
     int st = hip_check(hipStreamSynchronize(e->s), "MembraneStats sync");
     if (!st) st = hip_check(hipMemcpy(r, e->ms->res, sizeof(r), hipMemcpyDeviceToHost), "MembraneStats");
+    if (!st) st = hip_check(hipMemcpy(f, e->ms->flag, sizeof(f), hipMemcpyDeviceToHost), "MembraneStats");
+    out->steps = f[2];
+    out->fallbacks = e->ms->fallbacks + f[1];
+    out->last_residual = r[0];
     out->plain_residual = r[1];
     return st;
 }

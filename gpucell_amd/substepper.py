@@ -109,9 +109,11 @@ def _cell_metrics(e, case):
     return di, S["vol"].astype(np.float64).copy()
 
 
-def compare(case, net, norm, threshold: float, steps: int, warmup: int = 2):
+def compare(case, net, norm, threshold: float, steps: int, warmup: int = 2, graph: bool = True):
     """Plain (mode 1, k = 8) and surrogate (mode 2) runs of `steps` steps
-    from the same state; returns accuracy and wall time per step."""
+    from the same state; returns accuracy and wall time per step.  With
+    `graph` both modes replay their steps from a HIP graph (the surrogate's
+    fallback is decided on the device, so neither mode reads back per step)."""
     import torch
     out = {}
     runs = {}
@@ -121,11 +123,11 @@ def compare(case, net, norm, threshold: float, steps: int, warmup: int = 2):
             e.set_membrane_mode(1, k=8)
         else:
             e.set_membrane_mode(2, k=8, mlp=net, norm=norm, threshold=threshold)
-        e.step(warmup)
+        e.step(warmup, graph=graph)
         di0, v0 = _cell_metrics(e, case)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        e.step(steps)
+        e.step(steps, graph=graph)
         torch.cuda.synchronize()
         dt = (time.perf_counter() - t0) / steps
         di, v = _cell_metrics(e, case)
@@ -143,4 +145,5 @@ def compare(case, net, norm, threshold: float, steps: int, warmup: int = 2):
     st = s["stats"]
     out["fallback_rate"] = st["fallbacks"] / max(1, st["steps"])
     out["steps"] = steps
+    out["graph"] = bool(graph)
     return out

@@ -7,7 +7,9 @@
   on every step and reproduces mode 1 bit for bit;
 * recorded training rows hold the substep displacements (checked through
   their rotation-invariant norm);
-* leaving substepping restores the reference's membrane update."""
+* leaving substepping restores the reference's membrane update;
+* replaying the substepped steps from a HIP graph gives the eager bits, in
+  mode 1 and in mode 2 with the fallback decided on the device."""
 import numpy as np
 import pytest
 
@@ -138,3 +140,42 @@ def test_leaving_substepping_restores_reference_update(gpu):
     S = e.state()
     assert np.all(np.isfinite(S["pos"])) and np.all(np.isfinite(S["velrhop"]))
     assert e.membrane_stats()["steps"] == 0
+
+
+_KEYS = ("pos", "velrhop", "force", "nthrobound", "defforce", "area", "vol")
+
+
+@pytest.mark.parametrize("k", [2, 8])
+def test_plain_substeps_graph_replay_bit_identical(gpu, k):
+    case = small_case(seed=10)
+    a, b = Engine(case), Engine(case)
+    for e in (a, b):
+        e.set_membrane_mode(1, k=k)
+    a.step(5)
+    b.step(5, graph=True)
+    sa, sb = a.state(), b.state()
+    for key in _KEYS:
+        assert np.array_equal(sa[key], sb[key]), key
+    assert a.membrane_stats()["steps"] == b.membrane_stats()["steps"] == 5
+
+
+@pytest.mark.parametrize("variant", [0, 3])
+def test_surrogate_fallback_graph_replay_reproduces_plain(gpu, variant):
+    """Variant 3 (per-cell membrane kernel) decides the fallback on the
+    device and replays from the graph; variant 0 (three passes) reads the
+    residual on the host and steps eagerly.  Both match plain substepping."""
+    case = small_case(seed=11)
+    a, b = Engine(case), Engine(case)
+    for e in (a, b):
+        t = e.tuning()
+        t[3] = variant
+        e.set_tuning(t)
+    a.set_membrane_mode(1, k=8)
+    b.set_membrane_mode(2, k=8, mlp=Mlp(seed=4), threshold=1e-30)
+    a.step(5)
+    b.step(5, graph=True)
+    sa, sb = a.state(), b.state()
+    for key in _KEYS:
+        assert np.array_equal(sa[key], sb[key]), key
+    st = b.membrane_stats()
+    assert st["steps"] == 5 and st["fallbacks"] == 5

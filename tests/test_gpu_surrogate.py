@@ -33,6 +33,19 @@ def test_trained_surrogate_tracks_plain_substepping(gpu):
     assert r["fallback_rate"] < 0.2
     assert r["di_rel_err_max"] < 1e-2          # deformation index vs plain
     assert r["vol_rel_diff_max"] < 5e-3        # cell volume vs plain
+    # the device-decided fallback replayed from a graph: the eager bits
+    from gpucell_amd.engine import Engine
+    runs = []
+    for graph in (False, True):
+        e = Engine(held)
+        e.set_membrane_mode(2, k=8, mlp=net, norm=norm, threshold=THRESHOLD)
+        e.step(9, graph=graph)
+        runs.append((e.state(), e.membrane_stats()))
+        e.close()
+    for key in ("pos", "velrhop", "defforce", "area", "vol"):
+        assert np.array_equal(runs[0][0][key], runs[1][0][key]), key
+    assert runs[0][1]["fallbacks"] == runs[1][1]["fallbacks"]
+    assert runs[0][1]["steps"] == runs[1][1]["steps"] == 9
     u = substepper.compare(held, Mlp(seed=5), norm, THRESHOLD, steps=5)
     print("untrained:", u)
     assert u["fallback_rate"] == 1.0
