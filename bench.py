@@ -148,6 +148,9 @@ def main():
                     help="skip the learned-substepper leg (training + surrogate vs plain)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (no autotuning before the timed run)")
+    ap.add_argument("--surrogate-only", action="store_true",
+                    help="only the learned-substepper leg on --case (prints its JSON and exits; "
+                         "for kernel traces of the two substep modes)")
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
@@ -170,6 +173,9 @@ def main():
     log(f"[bench] generating case {a.case}")
     t = time.perf_counter()
     case, workload = make_case(a.case)
+    if a.surrogate_only:
+        print(json.dumps(surrogate_leg(case, a.case)), flush=True)
+        return
     batch = isinstance(case, list)
     cases = case if batch else [case]
     N, Nm = sum(c.N for c in cases), sum(c.nCellPar for c in cases)

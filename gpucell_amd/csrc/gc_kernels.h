@@ -131,6 +131,28 @@ struct CellArgs {
     const int* gate = nullptr;   // non-null: run only when *gate != 0 (device-decided fallback)
 };
 __global__ void k_cell_membrane(GcParams P, const float* __restrict__ bbox, CellArgs ca, DefArgs a);
+
+// Membrane substepping (velocity Verlet of the membrane under a fixed fluid
+// force): the kick and the drift, shared by the per-substep kernels and the
+// fused per-cell kernel so both give the same bits.
+__device__ __forceinline__ void ms_kick(float3& u, const float3& fe, const float3& fm, float dts, float inv_m)
+{
+    const float h = 0.5f * dts * inv_m;
+    u.x += h * (fe.x + fm.x); u.y += h * (fe.y + fm.y); u.z += h * (fe.z + fm.z);
+}
+__device__ __forceinline__ double ms_drift(double x, float dts, float u) { return x + (double)dts * u; }
+// State of the membrane system in id order (nm = DefArgs::nm vertices):
+// positions x[nm], y[nm], z[nm] (FP64), velocity, fluid force (fixed over
+// the step), membrane force, FP32 positions; mx6/mx7 (or NULL) receive the
+// positions after substeps k-2 and k-1.
+struct SubArgs {
+    int k; float dts, inv_m;
+    double* mx; tfloat3* mv; const tfloat3* fe; tfloat3* fm; tfloat3* mcoor;
+    double* mx6; double* mx7;
+};
+// all k substeps of each cell in one workgroup (cells as k_cell_membrane;
+// ca.step must not be -1)
+__global__ void k_cell_substeps(GcParams P, CellArgs ca, DefArgs a, SubArgs sa);
 // host check of the conditions above; fills range (C entries)
 bool cell_ranges(int nm, int nedge, int ntri, int ncell, const int* celltab, const tint2* betri,
                  const int6* edge, const tint3* tri, int4* range);

@@ -536,12 +536,120 @@ __global__ __launch_bounds__(BS) void k_vert_force(GcParams P, int nm, DefArgs a
     }
 }
 
+// Geometry and membrane force of one cell from the FP64 unwrapped positions
+// of its vertices in sx/sy/sz (LDS, local index); on return thread tid < nv
+// holds the force on vertex v0 + tid in f.  With geo the geometry outputs
+// (normals, areas, centroids, cell area and volume) are written.  Ends after
+// reading buf: a caller that refills sx or buf first passes a barrier.
+__device__ __forceinline__ void cell_force(const GcParams& P, const CellArgs& ca, const DefArgs& a, int c, int tid,
+                                           int v0, int nv, int e0, int ne, int tb, int te, bool geo,
+                                           const double* sx, const double* sy, const double* sz, float* buf,
+                                           double* sa, double* sv, float3& f)
+{
+    constexpr int BS = GC_CELL_BS;
+    // differences of unwrapped positions rounded once (what urel and vpos
+    // compute); the substepper's geometry input is the FP32 rounding of them
+    auto rel = [&](int u, int o) {
+        return make_float3((float)(sx[u] - sx[o]), (float)(sy[u] - sy[o]), (float)(sz[u] - sz[o]));
+    };
+    auto rel_geo = [&](int u, int o) {
+        if (!ca.round_geo) return rel(u, o);
+        return make_float3((float)((double)(float)sx[u] - (double)(float)sx[o]),
+                           (float)((double)(float)sy[u] - (double)(float)sy[o]),
+                           (float)((double)(float)sz[u] - (double)(float)sz[o]));
+    };
+    // geometry: k_geo_cell's arithmetic and reduction order (256-thread sums)
+    const int o = a.tri[tb].x - 1 - v0;
+    double A = 0.0, V = 0.0;
+    if (tid < 256) {
+        const double ox = ca.round_geo ? (double)(float)sx[o] : sx[o];
+        const double oy = ca.round_geo ? (double)(float)sy[o] : sy[o];
+        const double oz = ca.round_geo ? (double)(float)sz[o] : sz[o];
+        for (int t = tb + tid; t <= te; t += 256) {
+            const tint3 tr = a.tri[t];
+            const float3 p0 = rel_geo(tr.x - 1 - v0, o), p1 = rel_geo(tr.y - 1 - v0, o),
+                         p2 = rel_geo(tr.z - 1 - v0, o);
+            const float3 n = f3cross(f3sub(p1, p0), f3sub(p2, p0));
+            const float nn = sqrtf(f3dot(n, n));
+            const float at = 0.5f * nn;
+            if (geo) {
+                const float in = 1.0f / nn;
+                ca.trinor[t] = tfloat3{n.x * in, n.y * in, n.z * in};
+                ca.triarea[t] = at;
+                if (ca.step == -1) ca.reftriarea[t] = at;
+                ca.tricen[t] = tfloat3{(float)(ox + ((double)p0.x + p1.x + p2.x) / 3.0),
+                                       (float)(oy + ((double)p0.y + p1.y + p2.y) / 3.0),
+                                       (float)(oz + ((double)p0.z + p1.z + p2.z) / 3.0)};
+            }
+            A += (double)at;
+            double ax = p0.x, ay = p0.y, az = p0.z, bx = p1.x, by = p1.y, bz = p1.z;
+            double cx = p2.x, cy = p2.y, cz = p2.z;
+            V += (ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx)) / 6.0;
+        }
+        sa[tid] = A; sv[tid] = V;
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) { sa[tid] += sa[tid + w]; sv[tid] += sv[tid + w]; }
+        __syncthreads();
+    }
+    const float Ac = (float)sa[0], Vc = (float)sv[0];
+    if (geo) {
+        if (tid == 0) { ca.area[c] = Ac; ca.vol[c] = Vc; }
+        for (int t = tb + tid; t <= te; t += BS) {
+            ca.area_para[t] = Ac; ca.vol_para[t] = Vc;
+            if (ca.step == -1) { ca.refarea_para[t] = Ac; ca.refvol_para[t] = Vc; }
+        }
+    }
+    // edges -> buf[12 (e - e0) + 3 role]
+    for (int e = e0 + tid; e < e0 + ne; e += BS) {
+        const int6 E = a.edge[e];
+        const int ua = E.x - 1 - v0;
+        float3 fe[4];
+        edge_terms(P, a, e, rel(E.y - 1 - v0, ua), rel(E.z - 1 - v0, ua), rel(E.k - 1 - v0, ua), fe);
+        float* q = buf + 12 * (e - e0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { q[3 * k] = fe[k].x; q[3 * k + 1] = fe[k].y; q[3 * k + 2] = fe[k].z; }
+    }
+    __syncthreads();
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    if (tid < nv) {
+        const int g = v0 + tid;
+        for (int q = a.es[g]; q < a.es[g + 1]; ++q) {
+            const int code = a.ee[q];
+            const float* w = buf + 12 * ((code >> 2) - e0) + 3 * (code & 3);
+            fx += w[0]; fy += w[1]; fz += w[2];
+        }
+    }
+    __syncthreads();
+    // triangles -> buf[9 (t - tb) + 3 corner]
+    for (int t = tb + tid; t <= te; t += BS) {
+        const tint3 T = a.tri[t];
+        const int u0 = T.x - 1 - v0;
+        float3 ft[3];
+        tri_terms(a, t, rel(T.y - 1 - v0, u0), rel(T.z - 1 - v0, u0), Ac, Vc, ft);
+        float* q = buf + 9 * (t - tb);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { q[3 * k] = ft[k].x; q[3 * k + 1] = ft[k].y; q[3 * k + 2] = ft[k].z; }
+    }
+    __syncthreads();
+    if (tid < nv) {
+        const int g = v0 + tid;
+        for (int q = a.ts[g]; q < a.ts[g + 1]; ++q) {
+            const int code = a.te[q];
+            const float* w = buf + 9 * ((code >> 2) - tb) + 3 * (code & 3);
+            fx += w[0]; fy += w[1]; fz += w[2];
+        }
+    }
+    f = make_float3(fx, fy, fz);
+}
+
 __global__ __launch_bounds__(GC_CELL_BS) void k_cell_membrane(GcParams P, const float* __restrict__ bbox,
                                                               CellArgs ca, DefArgs a)
 {
     __shared__ double sx[GC_CELL_NV], sy[GC_CELL_NV], sz[GC_CELL_NV];
     __shared__ float buf[GC_CELL_NE * 12];
-    __shared__ double sa[BS], sv[BS];
+    __shared__ double sa[256], sv[256];
     if (ca.gate && !*ca.gate) return;   // block-uniform
     const int c = blockIdx.x, tid = threadIdx.x;
     const int4 r = ca.range[c];
@@ -570,105 +678,69 @@ __global__ __launch_bounds__(GC_CELL_BS) void k_cell_membrane(GcParams P, const 
         }
     }
     __syncthreads();
-    // differences of unwrapped positions rounded once (what urel and vpos
-    // compute); the substepper's geometry input is the FP32 rounding of them
-    auto rel = [&](int u, int o) {
-        return make_float3((float)(sx[u] - sx[o]), (float)(sy[u] - sy[o]), (float)(sz[u] - sz[o]));
-    };
-    auto rel_geo = [&](int u, int o) {
-        if (!ca.round_geo) return rel(u, o);
-        return make_float3((float)((double)(float)sx[u] - (double)(float)sx[o]),
-                           (float)((double)(float)sy[u] - (double)(float)sy[o]),
-                           (float)((double)(float)sz[u] - (double)(float)sz[o]));
-    };
-    // geometry: k_geo_cell's arithmetic and reduction order (BS threads)
-    const int o = a.tri[tb].x - 1 - v0;
-    double A = 0.0, V = 0.0;
-    if (tid < BS) {
-        const double ox = ca.round_geo ? (double)(float)sx[o] : sx[o];
-        const double oy = ca.round_geo ? (double)(float)sy[o] : sy[o];
-        const double oz = ca.round_geo ? (double)(float)sz[o] : sz[o];
-        for (int t = tb + tid; t <= te; t += BS) {
-            const tint3 tr = a.tri[t];
-            const float3 p0 = rel_geo(tr.x - 1 - v0, o), p1 = rel_geo(tr.y - 1 - v0, o),
-                         p2 = rel_geo(tr.z - 1 - v0, o);
-            const float3 n = f3cross(f3sub(p1, p0), f3sub(p2, p0));
-            const float nn = sqrtf(f3dot(n, n));
-            const float at = 0.5f * nn;
-            const float in = 1.0f / nn;
-            ca.trinor[t] = tfloat3{n.x * in, n.y * in, n.z * in};
-            ca.triarea[t] = at;
-            if (ca.step == -1) ca.reftriarea[t] = at;
-            ca.tricen[t] = tfloat3{(float)(ox + ((double)p0.x + p1.x + p2.x) / 3.0),
-                                   (float)(oy + ((double)p0.y + p1.y + p2.y) / 3.0),
-                                   (float)(oz + ((double)p0.z + p1.z + p2.z) / 3.0)};
-            A += (double)at;
-            double ax = p0.x, ay = p0.y, az = p0.z, bx = p1.x, by = p1.y, bz = p1.z;
-            double cx = p2.x, cy = p2.y, cz = p2.z;
-            V += (ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx)) / 6.0;
-        }
-        sa[tid] = A; sv[tid] = V;
-    }
-    __syncthreads();
-    for (int w = BS / 2; w > 0; w >>= 1) {
-        if (tid < w) { sa[tid] += sa[tid + w]; sv[tid] += sv[tid + w]; }
-        __syncthreads();
-    }
-    const float Ac = (float)sa[0], Vc = (float)sv[0];
-    if (tid == 0) { ca.area[c] = Ac; ca.vol[c] = Vc; }
-    for (int t = tb + tid; t <= te; t += GC_CELL_BS) {
-        ca.area_para[t] = Ac; ca.vol_para[t] = Vc;
-        if (ca.step == -1) { ca.refarea_para[t] = Ac; ca.refvol_para[t] = Vc; }
-    }
-    // edges -> buf[12 (e - e0) + 3 role]
-    for (int e = e0 + tid; e < e0 + ne; e += GC_CELL_BS) {
-        const int6 E = a.edge[e];
-        const int ua = E.x - 1 - v0;
-        float3 f[4];
-        edge_terms(P, a, e, rel(E.y - 1 - v0, ua), rel(E.z - 1 - v0, ua), rel(E.k - 1 - v0, ua), f);
-        float* q = buf + 12 * (e - e0);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) { q[3 * k] = f[k].x; q[3 * k + 1] = f[k].y; q[3 * k + 2] = f[k].z; }
-    }
-    __syncthreads();
-    float fx = 0.f, fy = 0.f, fz = 0.f;
-    if (tid < nv) {
-        const int g = v0 + tid;
-        for (int q = a.es[g]; q < a.es[g + 1]; ++q) {
-            const int code = a.ee[q];
-            const float* w = buf + 12 * ((code >> 2) - e0) + 3 * (code & 3);
-            fx += w[0]; fy += w[1]; fz += w[2];
-        }
-    }
-    __syncthreads();
-    // triangles -> buf[9 (t - tb) + 3 corner]
-    for (int t = tb + tid; t <= te; t += GC_CELL_BS) {
-        const tint3 T = a.tri[t];
-        const int u0 = T.x - 1 - v0;
-        float3 f[3];
-        tri_terms(a, t, rel(T.y - 1 - v0, u0), rel(T.z - 1 - v0, u0), Ac, Vc, f);
-        float* q = buf + 9 * (t - tb);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) { q[3 * k] = f[k].x; q[3 * k + 1] = f[k].y; q[3 * k + 2] = f[k].z; }
-    }
-    __syncthreads();
+    float3 f;
+    cell_force(P, ca, a, c, tid, v0, nv, e0, ne, tb, te, true, sx, sy, sz, buf, sa, sv, f);
     if (tid >= nv) return;
     const int g = v0 + tid;
-    for (int q = a.ts[g]; q < a.ts[g + 1]; ++q) {
-        const int code = a.te[q];
-        const float* w = buf + 9 * ((code >> 2) - tb) + 3 * (code & 3);
-        fx += w[0]; fy += w[1]; fz += w[2];
-    }
-    a.defforce[g] = tfloat3{fx, fy, fz};
+    a.defforce[g] = tfloat3{f.x, f.y, f.z};
     if (!a.force) return;   // membrane-only evaluation (substepping)
     tfloat3 F = a.force[slot];
-    F.x += fx; F.y += fy; F.z += fz;
+    F.x += f.x; F.y += f.y; F.z += f.z;
     a.force[slot] = F;
     if (a.vout) {  // this is the vertex's last force term -> corrector
         const float h = 0.5f * P.dt * P.inv_m;
         const tfloat3 vh = a.velhalf[slot];
         a.vout[slot] = make_float4(vh.x + h * F.x, vh.y + h * F.y, vh.z + h * F.z, a.velrho[slot].w);
     }
+}
+
+// k velocity-Verlet substeps of one cell's membrane under the fixed fluid
+// force fe: per substep the kick and drift of k_ms_kick_drift, the membrane
+// force of k_cell_membrane (geometry from the FP32-rounded positions) and
+// the kick of k_ms_kick, with the same arithmetic, so the same bits.  The
+// positions stay in LDS and each thread keeps its vertex's velocity and
+// forces in registers between substeps.
+__global__ __launch_bounds__(GC_CELL_BS) void k_cell_substeps(GcParams P, CellArgs ca, DefArgs a, SubArgs sa_)
+{
+    __shared__ double sx[GC_CELL_NV], sy[GC_CELL_NV], sz[GC_CELL_NV];
+    __shared__ float buf[GC_CELL_NE * 12];
+    __shared__ double sa[256], sv[256];
+    if (ca.gate && !*ca.gate) return;   // block-uniform
+    const int c = blockIdx.x, tid = threadIdx.x;
+    const int4 r = ca.range[c];
+    const int v0 = r.x, nv = r.y, e0 = r.z, ne = r.w;
+    const int tb = ca.betri[c].x, te = ca.betri[c].y;
+    const int nm = a.nm, g = v0 + tid;
+    const bool own = tid < nv;
+    float3 u = make_float3(0.f, 0.f, 0.f), fe = u, fm = u;
+    if (own) {
+        sx[tid] = sa_.mx[g]; sy[tid] = sa_.mx[nm + g]; sz[tid] = sa_.mx[2 * nm + g];
+        const tfloat3 u0 = sa_.mv[g], a0 = sa_.fe[g], b0 = sa_.fm[g];
+        u = make_float3(u0.x, u0.y, u0.z);
+        fe = make_float3(a0.x, a0.y, a0.z);
+        fm = make_float3(b0.x, b0.y, b0.z);
+    }
+    const float dts = sa_.dts;
+    for (int q = 1; q <= sa_.k; ++q) {
+        if (own) {
+            ms_kick(u, fe, fm, dts, sa_.inv_m);
+            sx[tid] = ms_drift(sx[tid], dts, u.x);
+            sy[tid] = ms_drift(sy[tid], dts, u.y);
+            sz[tid] = ms_drift(sz[tid], dts, u.z);
+            double* keep = q == sa_.k - 2 ? sa_.mx6 : q == sa_.k - 1 ? sa_.mx7 : nullptr;
+            if (keep) { keep[g] = sx[tid]; keep[nm + g] = sy[tid]; keep[2 * nm + g] = sz[tid]; }
+        }
+        __syncthreads();
+        cell_force(P, ca, a, c, tid, v0, nv, e0, ne, tb, te, q == sa_.k, sx, sy, sz, buf, sa, sv, fm);
+        if (own) ms_kick(u, fe, fm, dts, sa_.inv_m);
+        __syncthreads();
+    }
+    if (!own) return;
+    const double x = sx[tid], y = sy[tid], z = sz[tid];
+    sa_.mx[g] = x; sa_.mx[nm + g] = y; sa_.mx[2 * nm + g] = z;
+    sa_.mcoor[g] = tfloat3{(float)x, (float)y, (float)z};
+    sa_.mv[g] = tfloat3{u.x, u.y, u.z};
+    sa_.fm[g] = tfloat3{fm.x, fm.y, fm.z};
 }
 
 bool cell_ranges(int nm, int nedge, int ntri, int ncell, const int* celltab, const tint2* betri,

@@ -44,8 +44,11 @@ constexpr int MLP_BS = 256;
 // fused forward: 8 waves (two per SIMD) share the weights in LDS; each wave
 // keeps its 16 rows' activations in one buffer, overwritten layer by layer
 constexpr int FWD_BS = 512;
-constexpr size_t FUSED_LDS = sizeof(uint16_t) * ((size_t)H * LDI + 2 * (size_t)H * LDW +
-                                                 (size_t)OUTP * LDW + (FWD_BS / 64) * 16 * (size_t)LDW);
+constexpr size_t FUSED_LDS_W = sizeof(uint16_t) * ((size_t)H * LDI + 2 * (size_t)H * LDW +
+                                                   (size_t)OUTP * LDW + (FWD_BS / 64) * 16 * (size_t)LDW);
+constexpr int NBIAS = 3 * H + OUTP;   // b1, b2, b3, b4 (padded), FP32, after the BF16 blocks
+constexpr size_t FUSED_LDS = FUSED_LDS_W + sizeof(float) * NBIAS;
+static_assert(FUSED_LDS_W % 16 == 0, "bias block alignment");
 
 // Offsets of the blocks of the flat parameter vector.
 constexpr int OW1 = 0, OB1 = OW1 + H * IN, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 + H,
@@ -148,8 +151,9 @@ __device__ inline uint2 pack4(float a, float b, float c, float d)
     return make_uint2(__builtin_bit_cast(uint32_t, lo), __builtin_bit_cast(uint32_t, hi));
 }
 
-// bias + tanh, BF16 into out[16][LDW] (and optionally global [row][H])
-__device__ inline void wave_tanh_out(const f32x4* acc, const float* __restrict__ bias, uint16_t* out,
+// bias (LDS) + tanh, BF16 into out[16][LDW] (training: also global [row][H])
+template <bool TRAIN>
+__device__ inline void wave_tanh_out(const f32x4* acc, const float* bias, uint16_t* out,
                                      uint16_t* __restrict__ gout, int row0, int nrows)
 {
     const int lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
@@ -161,14 +165,15 @@ __device__ inline void wave_tanh_out(const f32x4* acc, const float* __restrict__
         uint2 v = pack4(tanh_hw(acc[t][0] + b.x), tanh_hw(acc[t][1] + b.y), tanh_hw(acc[t][2] + b.z),
                         tanh_hw(acc[t][3] + b.w));
         *(uint2*)(out + li * LDW + n0) = v;
-        if (gout && row < nrows) *(uint2*)(gout + (size_t)row * H + n0) = v;
+        if (TRAIN && row < nrows) *(uint2*)(gout + (size_t)row * H + n0) = v;
     }
 }
 
 // Fused forward.  Inference: y = out[B][ycols] (ycols = 24, ldy = 24).
 // Training: also xb [Bp][INP], h1..h3 [Bp][H] (BF16) and y as [Bp][OUTP].
 // Waves run their tiles independently (no block barrier after the weights
-// are staged).
+// are staged).  Weights and biases are read from LDS only.
+template <bool TRAIN>
 __global__ __launch_bounds__(FWD_BS) void k_mlp_fused(int B, const float* __restrict__ X, MlpDev w,
     float* __restrict__ Y, int ycols, int ldy, uint16_t* __restrict__ xb, uint16_t* __restrict__ h1,
     uint16_t* __restrict__ h2, uint16_t* __restrict__ h3)
@@ -183,40 +188,53 @@ __global__ __launch_bounds__(FWD_BS) void k_mlp_fused(int B, const float* __rest
     stage(s2, LDW, w.w2, H, H);
     stage(s3, LDW, w.w3, H, H);
     stage(s4, LDW, w.w4, OUTP, H);
+    float* sb = (float*)(lds + FUSED_LDS_W / sizeof(uint16_t));
+    for (int i = threadIdx.x; i < NBIAS; i += FWD_BS)
+        sb[i] = i < H ? w.b1[i] : i < 2 * H ? w.b2[i - H] : i < 3 * H ? w.b3[i - 2 * H] : w.b4[i - 3 * H];
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, li = lane & 15, lg = lane >> 4;
     const int nwaves = FWD_BS / 64;
     uint16_t* buf = sa + wave * 16 * LDW;
-    const int nrows = xb ? ((B + 255) / 256) * 256 : B;   // training rows are padded
+    const int nrows = TRAIN ? ((B + 255) / 256) * 256 : B;   // training rows are padded
     const int nchunks = (nrows + 15) / 16;                // 16 rows per wave task
     f32x4 acc[8];
-    for (int task = blockIdx.x * nwaves + wave; task < nchunks; task += gridDim.x * nwaves) {
-        const int r0 = task * 16;
-        {   // the wave's 16 input rows (lane = column): loads issued first
-            const float* xp = X + (size_t)r0 * IN + lane;
-            const bool col = lane < IN;
-            float v[16];
+    // The 16 input rows of a task (lane = column), loaded from clamped
+    // addresses so the sixteen loads issue back to back (a load under a
+    // branch waits out its own latency); rows past B and lanes past IN
+    // read as zero.  The next task's rows are loaded while this task runs.
+    float v[16];
+    auto load_rows = [&](int r0) {
+        const float* xp = X + min(lane, IN - 1);
 #pragma unroll
-            for (int i = 0; i < 16; ++i) v[i] = (col && r0 + i < B) ? xp[i * IN] : 0.0f;
+        for (int i = 0; i < 16; ++i) v[i] = xp[(size_t)min(r0 + i, B - 1) * IN];
+    };
+    const int stride = gridDim.x * nwaves;
+    int task = blockIdx.x * nwaves + wave;
+    if (task < nchunks) load_rows(task * 16);
+    for (; task < nchunks; task += stride) {
+        const int r0 = task * 16;
+        {
+            const bool col = lane < IN;
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
-                const uint16_t q = __builtin_bit_cast(uint16_t, (__bf16)v[i]);
+                const uint16_t q = __builtin_bit_cast(uint16_t, (__bf16)((col && r0 + i < B) ? v[i] : 0.0f));
                 buf[i * LDW + lane] = q;
-                if (xb && r0 + i < nrows) xb[(size_t)(r0 + i) * INP + lane] = q;
+                if (TRAIN && r0 + i < nrows) xb[(size_t)(r0 + i) * INP + lane] = q;
             }
         }
+        if (task + stride < nchunks) load_rows((task + stride) * 16);
         wave_lds_sync();
         wave_layer<INP, 8>(buf, LDW, s1, LDI, acc);
         wave_lds_sync();
-        wave_tanh_out(acc, w.b1, buf, h1, r0, nrows);
+        wave_tanh_out<TRAIN>(acc, sb, buf, h1, r0, nrows);
         wave_lds_sync();
         wave_layer<H, 8>(buf, LDW, s2, LDW, acc);
         wave_lds_sync();
-        wave_tanh_out(acc, w.b2, buf, h2, r0, nrows);
+        wave_tanh_out<TRAIN>(acc, sb + H, buf, h2, r0, nrows);
         wave_lds_sync();
         wave_layer<H, 8>(buf, LDW, s3, LDW, acc);
         wave_lds_sync();
-        wave_tanh_out(acc, w.b3, buf, h3, r0, nrows);
+        wave_tanh_out<TRAIN>(acc, sb + 2 * H, buf, h3, r0, nrows);
         wave_lds_sync();
         wave_layer<H, 2>(buf, LDW, s4, LDW, acc);
         wave_lds_sync();
@@ -225,7 +243,7 @@ __global__ __launch_bounds__(FWD_BS) void k_mlp_fused(int B, const float* __rest
         for (int t = 0; t < 2; ++t) {
             const int j0 = t * 16 + 4 * lg;
             if (j0 >= ycols || row >= nrows) continue;   // ycols is a multiple of 4
-            const float4 b = *(const float4*)(w.b4 + j0);
+            const float4 b = *(const float4*)(sb + 3 * H + j0);
             float* yp = Y + (size_t)row * ldy + j0;
             yp[0] = acc[t][0] + b.x; yp[1] = acc[t][1] + b.y;
             yp[2] = acc[t][2] + b.z; yp[3] = acc[t][3] + b.w;
@@ -533,8 +551,12 @@ int launch_fused(gc_mlp* m, int B, const float* x, float* y, int ycols, int ldy,
     int nrows = train ? ((B + 255) / 256) * 256 : B;
     int grid = std::min(fused_grid(), (nrows + 16 * (FWD_BS / 64) - 1) / (16 * (FWD_BS / 64)));
     if (grid < 1) return GC_OK;
-    k_mlp_fused<<<grid, FWD_BS, FUSED_LDS, s>>>(B, x, m->dev(), y, ycols, ldy, train ? m->xb : nullptr,
-        train ? m->hh[0] : nullptr, train ? m->hh[1] : nullptr, train ? m->hh[2] : nullptr);
+    if (train)
+        k_mlp_fused<true><<<grid, FWD_BS, FUSED_LDS, s>>>(B, x, m->dev(), y, ycols, ldy, m->xb, m->hh[0],
+                                                          m->hh[1], m->hh[2]);
+    else
+        k_mlp_fused<false><<<grid, FWD_BS, FUSED_LDS, s>>>(B, x, m->dev(), y, ycols, ldy, nullptr, nullptr,
+                                                           nullptr, nullptr);
     GC_LAUNCH_CHECK("mlp forward");
     return GC_OK;
 }
@@ -594,7 +616,10 @@ extern "C" gc_mlp* gc_MlpCreate(uint64_t seed, int* status)
     if (!status) status = &dummy;
     *status = GC_OK;
     if (!fused_attr_set) {
-        if ((*status = hip_check(hipFuncSetAttribute((const void*)k_mlp_fused,
+        if ((*status = hip_check(hipFuncSetAttribute((const void*)k_mlp_fused<true>,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                     (int)FUSED_LDS), "mlp LDS attribute")) ||
+            (*status = hip_check(hipFuncSetAttribute((const void*)k_mlp_fused<false>,
                                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                                      (int)FUSED_LDS), "mlp LDS attribute")))
             return nullptr;

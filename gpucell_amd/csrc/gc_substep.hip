@@ -81,11 +81,10 @@ __global__ void k_ms_kick_drift(int nm, float dts, float inv_m, double* __restri
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nm || (gate && !*gate)) return;
-    const float h = 0.5f * dts * inv_m;
-    tfloat3 u = mv[v], a = fe[v], b = fm[v];
-    u.x += h * (a.x + b.x); u.y += h * (a.y + b.y); u.z += h * (a.z + b.z);
-    mv[v] = u;
-    double x = mx[v] + (double)dts * u.x, y = mx[nm + v] + (double)dts * u.y, z = mx[2 * nm + v] + (double)dts * u.z;
+    float3 u = ld3(mv[v]);
+    ms_kick(u, ld3(fe[v]), ld3(fm[v]), dts, inv_m);
+    mv[v] = tfloat3{u.x, u.y, u.z};
+    const double x = ms_drift(mx[v], dts, u.x), y = ms_drift(mx[nm + v], dts, u.y), z = ms_drift(mx[2 * nm + v], dts, u.z);
     mx[v] = x; mx[nm + v] = y; mx[2 * nm + v] = z;
     mcoor[v] = tfloat3{(float)x, (float)y, (float)z};
 }
@@ -95,10 +94,9 @@ __global__ void k_ms_kick(int nm, float dts, float inv_m, tfloat3* __restrict__ 
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nm || (gate && !*gate)) return;
-    const float h = 0.5f * dts * inv_m;
-    tfloat3 u = mv[v], a = fe[v], b = fm[v];
-    u.x += h * (a.x + b.x); u.y += h * (a.y + b.y); u.z += h * (a.z + b.z);
-    mv[v] = u;
+    float3 u = ld3(mv[v]);
+    ms_kick(u, ld3(fe[v]), ld3(fm[v]), dts, inv_m);
+    mv[v] = tfloat3{u.x, u.y, u.z};
 }
 
 // membrane state -> engine slots: wrapped FP64 position, image counters,
@@ -162,6 +160,32 @@ __device__ inline float3 mrel(const double* mx, int nm, int j, double ox, double
     return make_float3((float)(mx[j] - ox), (float)(mx[nm + j] - oy), (float)(mx[2 * nm + j] - oz));
 }
 
+// rows of w floats formed per thread in st[thread (w + 1) + c] -> out as
+// contiguous runs (the rows of consecutive vertices are adjacent)
+__device__ __forceinline__ void store_rows(const float* st, int w, float* __restrict__ out, int rows)
+{
+    __syncthreads();
+    for (int i = threadIdx.x; i < rows * w; i += BS) out[i] = st[(i / w) * (w + 1) + i % w];
+    __syncthreads();
+}
+
+// rows of W floats from in (contiguous) -> st[row (W + 1) + c]; the caller
+// synchronises before reading st.  All W loads of a thread are issued before
+// the first store (clamped addresses, no branch around a load).
+template <int W>
+__device__ __forceinline__ void load_rows(const float* __restrict__ in, float* st, int rows)
+{
+    const int n = rows * W;
+    float t[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) t[k] = in[min((int)threadIdx.x + k * BS, n - 1)];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        const int i = threadIdx.x + k * BS;
+        if (i < n) st[(i / W) * (W + 1) + i % W] = t[k];
+    }
+}
+
 // 36 features of vertex v in its frame (e1, e2, n):
 //  0-17 1-ring edge vectors / l (CCW from the neighbour that defines e1;
 //       zeros for a missing sixth neighbour)
@@ -169,71 +193,81 @@ __device__ inline float3 mrel(const double* mx, int nm, int j, double ox, double
 //  27-29 (mean ring velocity - v) dt / l;
 //  30 A/A0 - 1 (cell), 31 V/V0 - 1 (cell), 32 local area / rest - 1,
 //  33 1 for a five-neighbour vertex, 34 mean ring edge / l - 1, 35 zero.
-__global__ void k_ms_features(FeatArgs a)
+// Frame rows (e1, e2, n) go to frame[9 v].
+__global__ __launch_bounds__(BS) void k_ms_features(FeatArgs a)
 {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= a.nm) return;
+    __shared__ float st[BS * (NF + 1)];
     const int nm = a.nm;
-    const double ox = a.mx[v], oy = a.mx[nm + v], oz = a.mx[2 * nm + v];
-    int r[6];
-    int deg = 0;
+    const int v0 = blockIdx.x * BS, v = v0 + (int)threadIdx.x;
+    const int rows = min(BS, nm - v0);
+    float f[NF], fr[9];
+    if (v < nm) {
+        const double ox = a.mx[v], oy = a.mx[nm + v], oz = a.mx[2 * nm + v];
+        int r[6];
+        int deg = 0;
 #pragma unroll
-    for (int k = 0; k < 6; ++k) { r[k] = a.ring[6 * v + k]; deg += r[k] >= 0; }
-    float3 d[6];
+        for (int k = 0; k < 6; ++k) { r[k] = a.ring[6 * v + k]; deg += r[k] >= 0; }
+        float3 d[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) d[k] = r[k] >= 0 ? mrel(a.mx, nm, r[k], ox, oy, oz) : make_float3(0.f, 0.f, 0.f);
-    float3 n = make_float3(0.f, 0.f, 0.f);
+        for (int k = 0; k < 6; ++k) d[k] = r[k] >= 0 ? mrel(a.mx, nm, r[k], ox, oy, oz) : make_float3(0.f, 0.f, 0.f);
+        float3 n = make_float3(0.f, 0.f, 0.f);
 #pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        if (k >= deg) break;
-        float3 c = cross3(d[k], d[(k + 1) % deg]);
-        n.x += c.x; n.y += c.y; n.z += c.z;
-    }
-    n = scl3(n, 1.0f / fmaxf(norm3(n), 1e-30f));
-    float3 e1 = sub3(d[0], scl3(n, dot3(d[0], n)));
-    e1 = scl3(e1, 1.0f / fmaxf(norm3(e1), 1e-30f));
-    float3 e2 = cross3(n, e1);
-    float* fr = a.frame + 9 * v;
-    fr[0] = e1.x; fr[1] = e1.y; fr[2] = e1.z; fr[3] = e2.x; fr[4] = e2.y; fr[5] = e2.z;
-    fr[6] = n.x; fr[7] = n.y; fr[8] = n.z;
-    const float il = 1.0f / a.lref[v];
-    float f[NF];
-    float mean_len = 0.f;
-    float3 vr = make_float3(0.f, 0.f, 0.f);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        f[3 * k] = dot3(d[k], e1) * il; f[3 * k + 1] = dot3(d[k], e2) * il; f[3 * k + 2] = dot3(d[k], n) * il;
-        if (r[k] >= 0) {
-            mean_len += norm3(d[k]);
-            float3 w = ld3(a.mv[r[k]]);
-            vr.x += w.x; vr.y += w.y; vr.z += w.z;
+        for (int k = 0; k < 6; ++k) {
+            if (k >= deg) break;
+            float3 c = cross3(d[k], d[(k + 1) % deg]);
+            n.x += c.x; n.y += c.y; n.z += c.z;
         }
-    }
-    const float3 vv = ld3(a.mv[v]);
-    vr = scl3(vr, 1.0f / (float)deg);
-    vr = sub3(vr, vv);
-    const float sv = a.dt * il, sf = a.dt * a.dt * a.inv_m * il;
-    const float3 fe = ld3(a.fe[v]), fm = ld3(a.fm[v]);
-    f[18] = dot3(vv, e1) * sv; f[19] = dot3(vv, e2) * sv; f[20] = dot3(vv, n) * sv;
-    f[21] = dot3(fe, e1) * sf; f[22] = dot3(fe, e2) * sf; f[23] = dot3(fe, n) * sf;
-    f[24] = dot3(fm, e1) * sf; f[25] = dot3(fm, e2) * sf; f[26] = dot3(fm, n) * sf;
-    f[27] = dot3(vr, e1) * sv; f[28] = dot3(vr, e2) * sv; f[29] = dot3(vr, n) * sv;
-    const int t0 = a.te[a.ts[v]] >> 2;
-    f[30] = a.Ac[t0] / a.A0c[t0] - 1.0f;
-    f[31] = a.Vc[t0] / a.V0c[t0] - 1.0f;
-    float la = 0.f, lr = 0.f;
-    for (int q = a.ts[v]; q < a.ts[v + 1]; ++q) {
-        int t = a.te[q] >> 2;
-        la += a.triarea[t];
-        lr += a.reftriarea[t];
-    }
-    f[32] = la / lr - 1.0f;
-    f[33] = deg == 5 ? 1.0f : 0.0f;
-    f[34] = mean_len / (float)deg * il - 1.0f;
-    f[35] = 0.0f;
-    float* out = a.feat + (size_t)NF * v;
+        n = scl3(n, 1.0f / fmaxf(norm3(n), 1e-30f));
+        float3 e1 = sub3(d[0], scl3(n, dot3(d[0], n)));
+        e1 = scl3(e1, 1.0f / fmaxf(norm3(e1), 1e-30f));
+        float3 e2 = cross3(n, e1);
+        fr[0] = e1.x; fr[1] = e1.y; fr[2] = e1.z; fr[3] = e2.x; fr[4] = e2.y; fr[5] = e2.z;
+        fr[6] = n.x; fr[7] = n.y; fr[8] = n.z;
+        const float il = 1.0f / a.lref[v];
+        float mean_len = 0.f;
+        float3 vr = make_float3(0.f, 0.f, 0.f);
 #pragma unroll
-    for (int c = 0; c < NF; ++c) out[c] = a.norm ? (f[c] - a.norm[c]) / a.norm[NF + c] : f[c];
+        for (int k = 0; k < 6; ++k) {
+            f[3 * k] = dot3(d[k], e1) * il; f[3 * k + 1] = dot3(d[k], e2) * il; f[3 * k + 2] = dot3(d[k], n) * il;
+            if (r[k] >= 0) {
+                mean_len += norm3(d[k]);
+                float3 w = ld3(a.mv[r[k]]);
+                vr.x += w.x; vr.y += w.y; vr.z += w.z;
+            }
+        }
+        const float3 vv = ld3(a.mv[v]);
+        vr = scl3(vr, 1.0f / (float)deg);
+        vr = sub3(vr, vv);
+        const float sv = a.dt * il, sf = a.dt * a.dt * a.inv_m * il;
+        const float3 fe = ld3(a.fe[v]), fm = ld3(a.fm[v]);
+        f[18] = dot3(vv, e1) * sv; f[19] = dot3(vv, e2) * sv; f[20] = dot3(vv, n) * sv;
+        f[21] = dot3(fe, e1) * sf; f[22] = dot3(fe, e2) * sf; f[23] = dot3(fe, n) * sf;
+        f[24] = dot3(fm, e1) * sf; f[25] = dot3(fm, e2) * sf; f[26] = dot3(fm, n) * sf;
+        f[27] = dot3(vr, e1) * sv; f[28] = dot3(vr, e2) * sv; f[29] = dot3(vr, n) * sv;
+        const int t0 = a.te[a.ts[v]] >> 2;
+        f[30] = a.Ac[t0] / a.A0c[t0] - 1.0f;
+        f[31] = a.Vc[t0] / a.V0c[t0] - 1.0f;
+        float la = 0.f, lr = 0.f;
+        for (int q = a.ts[v]; q < a.ts[v + 1]; ++q) {
+            int t = a.te[q] >> 2;
+            la += a.triarea[t];
+            lr += a.reftriarea[t];
+        }
+        f[32] = la / lr - 1.0f;
+        f[33] = deg == 5 ? 1.0f : 0.0f;
+        f[34] = mean_len / (float)deg * il - 1.0f;
+        f[35] = 0.0f;
+        float* my = st + 10 * threadIdx.x;
+#pragma unroll
+        for (int c = 0; c < 9; ++c) my[c] = fr[c];
+    }
+    store_rows(st, 9, a.frame + 9 * (size_t)v0, rows);
+    if (v < nm) {
+        float* my = st + (NF + 1) * threadIdx.x;
+#pragma unroll
+        for (int c = 0; c < NF; ++c) my[c] = a.norm ? (f[c] - a.norm[c]) / a.norm[NF + c] : f[c];
+    }
+    store_rows(st, NF, a.feat + (size_t)NF * v0, rows);
 }
 
 // recording: target block of substep s (1-based) of every vertex
@@ -258,15 +292,23 @@ __global__ void k_ms_target(int nm, int s, float dts, const double* __restrict__
 
 // surrogate: predicted substep displacements -> x8 (into mx), d2x, the
 // half-step velocity (x8 - x7)/dts into mv
-__global__ void k_ms_apply(int nm, float dts, const float* __restrict__ pred, const float* __restrict__ norm,
+__global__ __launch_bounds__(BS) void k_ms_apply(int nm, float dts, const float* __restrict__ pred,
+                           const float* __restrict__ norm,
                            const float* __restrict__ frame, const float* __restrict__ lref,
                            const double* __restrict__ mx0, const tfloat3* __restrict__ mv0,
                            double* __restrict__ mx, tfloat3* __restrict__ mv, tfloat3* __restrict__ mcoor,
                            tfloat3* __restrict__ d2x)
 {
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    // the block's prediction and frame rows, read as contiguous runs
+    __shared__ float sp[BS * (NO + 1)], sf[BS * 10];
+    const int v0 = blockIdx.x * BS, v = v0 + (int)threadIdx.x;
+    const int rows = min(BS, nm - v0);
+    load_rows<NO>(pred + (size_t)NO * v0, sp, rows);
+    load_rows<9>(frame + 9 * (size_t)v0, sf, rows);
+    __syncthreads();
     if (v >= nm) return;
-    const float* fr = frame + 9 * v;
+    const float* fr = sf + 10 * threadIdx.x;
+    const float* pv = sp + (NO + 1) * threadIdx.x;
     const float L = lref[v];
     float3 del[3];   // displacements of substeps 6, 7, 8 (minus the ballistic part)
 #pragma unroll
@@ -276,7 +318,7 @@ __global__ void k_ms_apply(int nm, float dts, const float* __restrict__ pred, co
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             int o = 3 * s + c;
-            float p = pred[(size_t)NO * v + o];
+            float p = pv[o];
             y[c] = norm ? p * norm[2 * NF + NO + o] + norm[2 * NF + o] : p;
         }
         del[q] = make_float3(L * (y[0] * fr[0] + y[1] * fr[3] + y[2] * fr[6]),
@@ -343,14 +385,24 @@ __global__ __launch_bounds__(BS) void k_ms_finish(int nm, int kick, float dts, f
 }
 
 // out[0] = rms residual; with flag: flag[0] = fallback needed (R > thr or
-// not a number), flag[1] counts the fallbacks
-__global__ void k_ms_res_final(int nb, int nm, const float* __restrict__ part, float* __restrict__ out,
-                               float thr, int* __restrict__ flag)
+// not a number), flag[1] counts the fallbacks.  One block: strided sums,
+// then a tree (fixed order).
+constexpr int RES_BS = 1024;
+__global__ __launch_bounds__(RES_BS) void k_ms_res_final(int nb, int nm, const float* __restrict__ part,
+                                                         float* __restrict__ out, float thr, int* __restrict__ flag)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    float s = 0.f;
-    for (int i = 0; i < nb; ++i) s += part[i];
-    const float r = sqrtf(s / (float)nm);
+    __shared__ float sh[RES_BS];
+    const int tid = threadIdx.x;
+    float q = 0.f;
+    for (int i = tid; i < nb; i += RES_BS) q += part[i];
+    sh[tid] = q;
+    __syncthreads();
+    for (int o = RES_BS / 2; o > 0; o >>= 1) {
+        if (tid < o) sh[tid] += sh[tid + o];
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    const float r = sqrtf(sh[0] / (float)nm);
     out[0] = r;
     if (flag) {
         const int fb = !(r <= thr);
@@ -359,20 +411,24 @@ __global__ void k_ms_res_final(int nb, int nm, const float* __restrict__ part, f
     }
 }
 
-// device copies for saving and restoring the substep state (graph-safe; with
-// a gate only when *gate != 0)
-__global__ void k_ms_copy_d(int n, const double* __restrict__ a, double* __restrict__ b,
-                            const int* __restrict__ gate)
+// saves or restores the substep state (positions, velocity, membrane
+// force); with a gate only when *gate != 0
+__global__ void k_ms_copy_state(int nm, const double* __restrict__ mx, const tfloat3* __restrict__ mv,
+                                const tfloat3* __restrict__ fm, double* __restrict__ mx_to, tfloat3* __restrict__ mv_to,
+                                tfloat3* __restrict__ fm_to, const int* __restrict__ gate)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || (gate && !*gate)) return;
-    b[i] = a[i];
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nm || (gate && !*gate)) return;
+    mx_to[v] = mx[v]; mx_to[nm + v] = mx[nm + v]; mx_to[2 * nm + v] = mx[2 * nm + v];
+    mv_to[v] = mv[v];
+    fm_to[v] = fm[v];
 }
-__global__ void k_ms_copy_f3(int n, const tfloat3* __restrict__ a, tfloat3* __restrict__ b,
-                             const int* __restrict__ gate)
+
+// device copy (graph-safe)
+__global__ void k_ms_copy_d(int n, const double* __restrict__ a, double* __restrict__ b)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || (gate && !*gate)) return;
+    if (i >= n) return;
     b[i] = a[i];
 }
 
@@ -419,10 +475,9 @@ namespace {
 
 int3 per3(const gc_engine* e) { return make_int3(e->ext.periodic[0], e->ext.periodic[1], e->ext.periodic[2]); }
 
-// membrane force (and geometry) at the membrane system's positions
-int ms_force(gc_engine* e, MembraneSub* m, const int* gate = nullptr)
+// the membrane system's arguments of the membrane force kernels
+DefArgs ms_defargs(gc_engine* e, MembraneSub* m)
 {
-    hipStream_t s = e->s;
     DefArgs a;
     a.edge = e->d_edge; a.tri = e->d_tri; a.coor = m->mcoor; a.nthro = m->mnt; a.curpos = m->mid;
     a.lmax = e->d_lmax; a.mexp = e->d_mexp; a.p = e->d_p; a.kp = e->d_kp; a.th0 = e->d_th0;
@@ -431,6 +486,14 @@ int ms_force(gc_engine* e, MembraneSub* m, const int* gate = nullptr)
     a.V0c = e->d_refvol_para; a.es = e->d_es; a.ee = e->d_ee; a.ts = e->d_ts; a.te = e->d_te;
     a.defforce = m->fm; a.force = nullptr;
     a.mpos = m->mx; a.nm = m->nm;
+    return a;
+}
+
+// membrane force (and geometry) at the membrane system's positions
+int ms_force(gc_engine* e, MembraneSub* m, const int* gate = nullptr)
+{
+    hipStream_t s = e->s;
+    const DefArgs a = ms_defargs(e, m);
     if (use_cell_kernel(e)) {   // geometry from the FP32 positions, as k_geo_cell below
         CellArgs ca = cell_args(e, e->step, 1);
         ca.gate = gate;
@@ -458,11 +521,8 @@ FeatArgs feat_args(gc_engine* e, MembraneSub* m, bool normalise)
 
 int save_start(gc_engine* e, MembraneSub* m)
 {
-    hipStream_t s = e->s;
-    const int nm = m->nm;
-    k_ms_copy_d<<<blocks_for(3 * nm, BS), BS, 0, s>>>(3 * nm, m->mx, m->mx0, nullptr);
-    k_ms_copy_f3<<<blocks_for(nm, BS), BS, 0, s>>>(nm, m->mv, m->mv0, nullptr);
-    k_ms_copy_f3<<<blocks_for(nm, BS), BS, 0, s>>>(nm, m->fm, m->fm0, nullptr);
+    k_ms_copy_state<<<blocks_for(m->nm, BS), BS, 0, e->s>>>(m->nm, m->mx, m->mv, m->fm, m->mx0, m->mv0, m->fm0,
+                                                             nullptr);
     GC_LAUNCH_CHECK("ms save");
     return GC_OK;
 }
@@ -470,11 +530,8 @@ int save_start(gc_engine* e, MembraneSub* m)
 // back to the saved start state (only when *gate != 0 if gate is given)
 int restore_start(gc_engine* e, MembraneSub* m, const int* gate = nullptr)
 {
-    hipStream_t s = e->s;
-    const int nm = m->nm;
-    k_ms_copy_d<<<blocks_for(3 * nm, BS), BS, 0, s>>>(3 * nm, m->mx0, m->mx, gate);
-    k_ms_copy_f3<<<blocks_for(nm, BS), BS, 0, s>>>(nm, m->mv0, m->mv, gate);
-    k_ms_copy_f3<<<blocks_for(nm, BS), BS, 0, s>>>(nm, m->fm0, m->fm, gate);
+    k_ms_copy_state<<<blocks_for(m->nm, BS), BS, 0, e->s>>>(m->nm, m->mx0, m->mv0, m->fm0, m->mx, m->mv, m->fm,
+                                                             gate);
     GC_LAUNCH_CHECK("ms restore");
     return GC_OK;
 }
@@ -487,23 +544,34 @@ int plain_substeps(gc_engine* e, MembraneSub* m, bool record, long row0, const i
     hipStream_t s = e->s;
     const int nb = blocks_for(m->nm, BS);
     const float dts = e->hreps[0].P.dt / (float)m->k, inv_m = e->hreps[0].P.inv_m;
+    const bool keep = !gate && m->k == KSUB;   // positions after substeps 6 and 7 for the residual
     int st;
-    for (int q = 1; q <= m->k; ++q) {
-        k_ms_kick_drift<<<nb, BS, 0, s>>>(m->nm, dts, inv_m, m->mx, m->mv, m->fe, m->fm, m->mcoor, gate);
-        if ((st = ms_force(e, m, gate))) return st;
-        k_ms_kick<<<nb, BS, 0, s>>>(m->nm, dts, inv_m, m->mv, m->fe, m->fm, gate);
-        if (record)
-            k_ms_target<<<nb, BS, 0, s>>>(m->nm, q, dts, m->mx, m->mx0, m->mv0, m->frame, m->lref,
-                                           m->rec_y + (size_t)NO * row0);
-        GC_LAUNCH_CHECK("membrane substep");
-        if (!gate && m->k == KSUB && (q == KSUB - 2 || q == KSUB - 1))
-            k_ms_copy_d<<<blocks_for(3 * m->nm, BS), BS, 0, s>>>(3 * m->nm, m->mx, q == KSUB - 2 ? m->mx6 : m->mx7,
-                                                                 nullptr);
+    if (use_cell_kernel(e) && !record && e->step != -1) {
+        // all substeps of a cell in one workgroup
+        CellArgs ca = cell_args(e, e->step, 1);
+        ca.gate = gate;
+        SubArgs sa{m->k, dts, inv_m, m->mx, m->mv, m->fe, m->fm, m->mcoor,
+                   keep ? m->mx6 : nullptr, keep ? m->mx7 : nullptr};
+        k_cell_substeps<<<e->ncell, GC_CELL_BS, 0, s>>>(m->Pm, ca, ms_defargs(e, m), sa);
+        GC_LAUNCH_CHECK("membrane substeps (per cell)");
+    } else {
+        for (int q = 1; q <= m->k; ++q) {
+            k_ms_kick_drift<<<nb, BS, 0, s>>>(m->nm, dts, inv_m, m->mx, m->mv, m->fe, m->fm, m->mcoor, gate);
+            if ((st = ms_force(e, m, gate))) return st;
+            k_ms_kick<<<nb, BS, 0, s>>>(m->nm, dts, inv_m, m->mv, m->fe, m->fm, gate);
+            if (record)
+                k_ms_target<<<nb, BS, 0, s>>>(m->nm, q, dts, m->mx, m->mx0, m->mv0, m->frame, m->lref,
+                                               m->rec_y + (size_t)NO * row0);
+            GC_LAUNCH_CHECK("membrane substep");
+            if (keep && (q == KSUB - 2 || q == KSUB - 1))
+                k_ms_copy_d<<<blocks_for(3 * m->nm, BS), BS, 0, s>>>(3 * m->nm, m->mx,
+                                                                     q == KSUB - 2 ? m->mx6 : m->mx7);
+        }
     }
-    if (!gate && m->k == KSUB) {  // the residual of the exact trajectory (scale for the threshold)
+    if (keep) {  // the residual of the exact trajectory (scale for the threshold)
         k_ms_d2x<<<nb, BS, 0, s>>>(m->nm, m->mx, m->mx7, m->mx6, m->d2x);
         k_ms_finish<<<nb, BS, 0, s>>>(m->nm, 0, dts, inv_m, m->mv, m->fe, m->fm, m->d2x, m->lref, m->part);
-        k_ms_res_final<<<1, 1, 0, s>>>(nb, m->nm, m->part, m->res + 1, 0.0f, nullptr);
+        k_ms_res_final<<<1, RES_BS, 0, s>>>(nb, m->nm, m->part, m->res + 1, 0.0f, nullptr);
         GC_LAUNCH_CHECK("membrane plain residual");
     }
     return GC_OK;
@@ -566,11 +634,11 @@ int ms_step(gc_engine* e)
             // the fallback is decided on the device: the residual kernel sets
             // the flag that gates the restore and the plain substeps (no host
             // read; the step can be replayed from a graph)
-            k_ms_res_final<<<1, 1, 0, s>>>(nb, nm, m->part, m->res, m->thr, m->flag);
+            k_ms_res_final<<<1, RES_BS, 0, s>>>(nb, nm, m->part, m->res, m->thr, m->flag);
             GC_LAUNCH_CHECK("membrane residual");
             if ((st = restore_start(e, m, m->flag)) || (st = plain_substeps(e, m, false, 0, m->flag))) return st;
         } else {
-            k_ms_res_final<<<1, 1, 0, s>>>(nb, nm, m->part, m->res, 0.0f, nullptr);
+            k_ms_res_final<<<1, RES_BS, 0, s>>>(nb, nm, m->part, m->res, 0.0f, nullptr);
             GC_LAUNCH_CHECK("membrane residual");
             float r = 0.f;
             if ((st = hip_check(hipMemcpyAsync(&r, m->res, sizeof(float), hipMemcpyDeviceToHost, s), "residual D2H")) ||
