@@ -890,7 +890,8 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
         DefArgs a;
         a.edge = e->d_edge; a.tri = e->d_tri; a.coor = e->d_coor; a.nthro = e->d_nt[c];
         a.curpos = e->d_curpos; a.lmax = e->d_lmax; a.mexp = e->d_mexp; a.p = e->d_p;
-        a.kp = e->d_kp; a.th0 = e->d_th0; a.kb = e->d_kb; a.reftriarea = e->d_reftriarea;
+        a.kp = e->d_kp; a.th0 = e->d_th0; a.cth0 = e->d_cth0; a.sth0 = e->d_sth0;
+        a.kb = e->d_kb; a.reftriarea = e->d_reftriarea;
         a.kd = e->d_kd; a.ka = e->d_ka; a.Ac = e->d_area_para; a.A0c = e->d_refarea_para;
         a.kv = e->d_kv; a.Vc = e->d_vol_para; a.V0c = e->d_refvol_para;
         a.es = e->d_es; a.ee = e->d_ee; a.ts = e->d_ts; a.te = e->d_te;
@@ -904,7 +905,7 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
             k_cell_membrane<<<e->ncell, GC_CELL_BS, 0, s>>>(P, e->d_bbox, cell_args(e, e->step, 0), a);
             GC_LAUNCH_CHECK("engine membrane (per cell)");
         } else {
-            k_geo_cell<<<e->ncell, BS, 0, s>>>(P, e->step, ENS ? e->d_reps : nullptr,
+            k_geo_cell<<<e->ncell, GC_GEO_BS, 0, s>>>(P, e->step, ENS ? e->d_reps : nullptr,
                 ENS ? e->d_cell_rep : nullptr, e->d_bbox, e->d_curpos, e->d_tri,
                 e->d_betri, e->d_nt[c], e->d_coor, e->d_trinor, e->d_triarea, e->d_reftriarea,
                 e->d_tricen, e->d_refarea_para, e->d_refvol_para, e->d_area_para, e->d_vol_para,
@@ -1189,7 +1190,8 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
             (st = e->alloc(&e->d_kd, T)) || (st = e->alloc(&e->d_ka, T)) || (st = e->alloc(&e->d_kv, T)) ||
             (st = e->alloc(&e->d_l0, E)) || (st = e->alloc(&e->d_lmax, E)) || (st = e->alloc(&e->d_kb, E)) ||
             (st = e->alloc(&e->d_mexp, E)) || (st = e->alloc(&e->d_p, E)) || (st = e->alloc(&e->d_kp, E)) ||
-            (st = e->alloc(&e->d_th0, E)) || (st = e->alloc(&e->d_trinor, T)) ||
+            (st = e->alloc(&e->d_th0, E)) || (st = e->alloc(&e->d_cth0, E)) || (st = e->alloc(&e->d_sth0, E)) ||
+            (st = e->alloc(&e->d_trinor, T)) ||
             (st = e->alloc(&e->d_tricen, T)) || (st = e->alloc(&e->d_triarea, T)) ||
             (st = e->alloc(&e->d_reftriarea, T)) || (st = e->alloc(&e->d_refarea_para, T)) ||
             (st = e->alloc(&e->d_refvol_para, T)) || (st = e->alloc(&e->d_area_para, T)) ||
@@ -1249,7 +1251,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
             return fail(st);
         const int c = e->cur;
         // reference geometry and mechanics (Particle::SetPar, Particle.cu:476-479)
-        k_geo_cell<<<e->ncell, BS, 0, s>>>(e->hreps[0].P, -1L, e->ens() ? e->d_reps : nullptr,
+        k_geo_cell<<<e->ncell, GC_GEO_BS, 0, s>>>(e->hreps[0].P, -1L, e->ens() ? e->d_reps : nullptr,
             e->ens() ? e->d_cell_rep : nullptr, e->d_bbox, e->d_curpos, e->d_tri,
             e->d_betri, e->d_nt[c], e->d_coor, e->d_trinor, e->d_triarea, e->d_reftriarea,
             e->d_tricen, e->d_refarea_para, e->d_refvol_para, e->d_area_para, e->d_vol_para,
@@ -1263,6 +1265,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
                 e->d_curpos, e->d_coor, e->d_l0 + o, e->d_lmax + o, e->d_kc + o, e->d_mu0 + o,
                 e->d_kb + o, e->d_mexp + o, e->d_p + o, e->d_kp + o, e->d_th0 + o);
         }
+        k_sincos<<<blocks_for(E, BS), BS, 0, s>>>(E, e->d_th0, e->d_cth0, e->d_sth0);
         if ((st = hip_check(hipGetLastError(), "engine membrane init"))) return fail(st);
     }
     if ((st = hip_check(hipStreamSynchronize(s), "engine init sync"))) return fail(st);

@@ -75,6 +75,7 @@ struct gc_engine {
     tint3* d_tri; int6* d_edge; int* d_celltab; tint2* d_betri;
     float *d_mu0, *d_kc, *d_kd, *d_ka, *d_kv;
     float *d_l0, *d_lmax, *d_kb, *d_mexp, *d_p, *d_kp, *d_th0;
+    float *d_cth0, *d_sth0;   // cos and sin of th0
     tfloat3 *d_trinor, *d_tricen; float *d_triarea, *d_reftriarea;
     float *d_refarea_para, *d_refvol_para, *d_area_para, *d_vol_para, *d_area, *d_vol, *d_pararea;
     tfloat3 *d_defforce, *d_aggforce;

@@ -79,6 +79,8 @@ struct DefArgs {
     const float* A0c; const float* kv; const float* Vc; const float* V0c;
     const int* es; const int* ee; const int* ts; const int* te;
     tfloat3* defforce; tfloat3* force;
+    // engine: cos and sin of th0 per edge (NULL: computed from th0)
+    const float* cth0 = nullptr; const float* sth0 = nullptr;
     // engine only (NULL in the entry point): Groot-Warren corrector after
     // the membrane force, vout = (velhalf + dt/2 F/m, rho of velrho)
     const tfloat3* velhalf = nullptr; const float4* velrho = nullptr; float4* vout = nullptr;
@@ -121,6 +123,8 @@ constexpr int GC_CELL_BS = 1024;
 constexpr int GC_CELL_NV = 768;
 constexpr int GC_CELL_NE = 2048;
 constexpr int GC_CELL_NT = (GC_CELL_NE * 12) / 9;
+constexpr int GC_GEO_BS = GC_CELL_BS;   // threads of k_geo_cell (sets its FP64 reduction order)
+constexpr int GC_CELL_INC = 12;   // incidences per vertex read together (more are read one by one)
 struct CellArgs {
     const int4* range;    // (first vertex, vertices, first edge, edges) per cell, 0-based
     const tint2* betri;   // triangle range per cell (inclusive)
@@ -131,6 +135,8 @@ struct CellArgs {
     const int* gate = nullptr;   // non-null: run only when *gate != 0 (device-decided fallback)
 };
 __global__ void k_cell_membrane(GcParams P, const float* __restrict__ bbox, CellArgs ca, DefArgs a);
+// c[i], s[i] = cos, sin of th[i] (per-edge reference dihedral angle)
+__global__ void k_sincos(int n, const float* __restrict__ th, float* __restrict__ c, float* __restrict__ s);
 
 // Membrane substepping (velocity Verlet of the membrane under a fixed fluid
 // force): the kick and the drift, shared by the per-substep kernels and the

@@ -153,7 +153,7 @@ __device__ __forceinline__ float3 f3scale(float3 a, float s) { return make_float
 // One block per cell: per-triangle geometry, FP64 block reduction of the
 // cell's area and volume (signed tetrahedra against the cell's first
 // triangle's first vertex), then the per-triangle broadcasts.
-__global__ __launch_bounds__(BS) void k_geo_cell(GcParams P, long stepcount,
+__global__ __launch_bounds__(GC_GEO_BS) void k_geo_cell(GcParams P, long stepcount,
     const GcRep* __restrict__ reps, const int* __restrict__ cell_rep,
     const float* __restrict__ bbox, const int* __restrict__ curpos, const tint3* __restrict__ tri,
     const tint2* __restrict__ betri, const tint3* __restrict__ nthro, const tfloat3* __restrict__ coor,
@@ -162,6 +162,7 @@ __global__ __launch_bounds__(BS) void k_geo_cell(GcParams P, long stepcount,
     float* __restrict__ area_para, float* __restrict__ vol_para, float* __restrict__ area,
     float* __restrict__ vol)
 {
+    constexpr int BS = GC_GEO_BS;
     __shared__ double sa[BS], sv[BS];
     const int c = blockIdx.x;
     const int tb = betri[c].x, te = betri[c].y;
@@ -196,10 +197,10 @@ __global__ __launch_bounds__(BS) void k_geo_cell(GcParams P, long stepcount,
                             (float)(oy + ((double)p0.y + p1.y + p2.y) / 3.0),
                             (float)(oz + ((double)p0.z + p1.z + p2.z) / 3.0)};
         A += (double)at;
-        // signed tetrahedron (O, p0, p1, p2) in FP64
+        // signed tetrahedron (O, p0, p1, p2) in FP64: six times its volume
         double ax = p0.x, ay = p0.y, az = p0.z, bx = p1.x, by = p1.y, bz = p1.z;
         double cx = p2.x, cy = p2.y, cz = p2.z;
-        V += (ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx)) / 6.0;
+        V += ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
     }
     sa[threadIdx.x] = A; sv[threadIdx.x] = V;
     __syncthreads();
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(BS) void k_geo_cell(GcParams P, long stepcount,
         if (threadIdx.x < o) { sa[threadIdx.x] += sa[threadIdx.x + o]; sv[threadIdx.x] += sv[threadIdx.x + o]; }
         __syncthreads();
     }
-    float Ac = (float)sa[0], Vc = (float)sv[0];
+    float Ac = (float)sa[0], Vc = (float)(sv[0] / 6.0);
     if (threadIdx.x == 0) { area[c] = Ac; vol[c] = Vc; }
     for (int t = tb + threadIdx.x; t <= te; t += BS) {
         area_para[t] = Ac; vol_para[t] = Vc;
@@ -410,16 +411,37 @@ __device__ __forceinline__ void rec_origin(const DefArgs& a, const GcParams& P, 
     }
 }
 
-// Forces of edge e on its vertices A, B, C, D from B, C, D relative to A.
-__device__ __forceinline__ void edge_terms(const GcParams& P, const DefArgs& a, int e, float3 pB, float3 pC,
-                                           float3 pD, float3 out[4])
+// per-edge and per-triangle parameters (loaded ahead of their use)
+struct EdgePar { float lmax, p, kp, mexp, kb, cth0, sth0; };
+struct TriPar { float ka, A0c, kd, ref, kv, V0c; };
+__device__ __forceinline__ EdgePar edge_par(const DefArgs& a, int e)
+{
+    float c, sn;
+    if (a.cth0) {   // precomputed (engine)
+        c = a.cth0[e];
+        sn = a.sth0[e];
+    } else {
+        sincosf(a.th0[e], &sn, &c);
+    }
+    return EdgePar{a.lmax[e], a.p[e], a.kp[e], a.mexp[e], a.kb[e], c, sn};
+}
+__device__ __forceinline__ TriPar tri_par(const DefArgs& a, int t)
+{
+    return TriPar{a.ka[t], a.A0c[t], a.kd[t], a.reftriarea[t], a.kv[t], a.V0c[t]};
+}
+
+// Forces of an edge on its vertices A, B, C, D from B, C, D relative to A.
+__device__ __forceinline__ void edge_terms(const GcParams& P, const EdgePar& q, float3 pB, float3 pC, float3 pD,
+                                           float3 out[4])
 {
     // WLC-POW spring along d = A - B: s d on A, -s d on B
     const float3 d = make_float3(-pB.x, -pB.y, -pB.z);
     const float ls = sqrtf(f3dot(d, d));
-    const float x = ls / a.lmax[e];
+    const float x = ls / q.lmax;
     const float om = 1.0f - x;
-    const float fm = -(P.kT / a.p[e]) * (0.25f / (om * om) - 0.25f + x) + a.kp[e] / powf(ls, a.mexp[e]);
+    // kp / ls^m with the hardware base-2 logarithm and exponential (~1 ulp each)
+    const float fm = -(P.kT / q.p) * (0.25f / (om * om) - 0.25f + x) +
+                     q.kp * __builtin_amdgcn_exp2f(-q.mexp * __builtin_amdgcn_logf(ls));
     const float s = fm / ls;
     // bending: dihedral of the wings C (n1 = e x c) and D (n2 = d x e)
     const float3 e3 = pB, ac = pC, ad = pD;
@@ -428,9 +450,12 @@ __device__ __forceinline__ void edge_terms(const GcParams& P, const DefArgs& a, 
     const float q1 = f3dot(n1, n1), q2 = f3dot(n2, n2);
     const float im1 = rsqrtf(q1), im2 = rsqrtf(q2);
     const float3 u1 = f3scale(n1, im1), u2 = f3scale(n2, im2);
+    // dihedral angle th from (cos th, sin th) = (u1.u2, (u1 x u2).e3 / l) / r,
+    // sin(th - th0) = sin th cos th0 - cos th sin th0
     const float3 xx = f3cross(u1, u2);
-    const float th = atan2f(f3dot(xx, e3) / l, f3dot(u1, u2));
-    const float cb = -a.kb[e] * sinf(th - a.th0[e]);
+    const float sy = f3dot(xx, e3) / l, cx = f3dot(u1, u2);
+    const float ir = rsqrtf(cx * cx + sy * sy);
+    const float cb = -q.kb * ((sy * q.cth0 - cx * q.sth0) * ir);
     const float3 gC = f3scale(n1, -l / q1), gD = f3scale(n2, -l / q2);
     const float tC = f3dot(ac, e3) / l2, tD = f3dot(ad, e3) / l2;
     const float wCA = -(1.0f - tC), wDA = -(1.0f - tD), wCB = -tC, wDB = -tD;
@@ -444,16 +469,15 @@ __device__ __forceinline__ void edge_terms(const GcParams& P, const DefArgs& a, 
 
 // Area and volume forces of triangle t on its corners from P1, P2 relative
 // to P0; Ac, Vc the cell's current area and volume.
-__device__ __forceinline__ void tri_terms(const DefArgs& a, int t, float3 P1, float3 P2, float Ac, float Vc,
-                                          float3 out[3])
+__device__ __forceinline__ void tri_terms(const TriPar& q, float3 P1, float3 P2, float Ac, float Vc, float3 out[3])
 {
     const float3 P0 = make_float3(0.f, 0.f, 0.f);
     const float3 n = f3cross(P1, P2);
     const float nn = sqrtf(f3dot(n, n));
     const float At = 0.5f * nn;
     const float3 un = f3scale(n, 1.0f / nn);
-    const float cA = a.ka[t] * (Ac - a.A0c[t]) / a.A0c[t] + a.kd[t] * (At - a.reftriarea[t]) / a.reftriarea[t];
-    const float cV = a.kv[t] * (Vc - a.V0c[t]) / a.V0c[t];
+    const float cA = q.ka * (Ac - q.A0c) / q.A0c + q.kd * (At - q.ref) / q.ref;
+    const float cV = q.kv * (Vc - q.V0c) / q.V0c;
     const float hv = cV * (1.0f / 6.0f);
     const float3 cn[3] = {P1, P2, P0};   // next corner of corner k
     const float3 cnn[3] = {P2, P0, P1};  // next-next corner
@@ -480,7 +504,7 @@ __global__ __launch_bounds__(BS) void k_edge_force(GcParams P, int nedge, const 
     const float3 pC = vpos(a, P, b, E.z, xo, no, ox, oy, oz);
     const float3 pD = vpos(a, P, b, E.k, xo, no, ox, oy, oz);
     float3 f[4];
-    edge_terms(P, a, e, pB, pC, pD, f);
+    edge_terms(P, edge_par(a, e), pB, pC, pD, f);
     tfloat3* o = ef + 4 * (size_t)e;
 #pragma unroll
     for (int k = 0; k < 4; ++k) o[k] = tfloat3{f[k].x, f[k].y, f[k].z};
@@ -500,7 +524,7 @@ __global__ __launch_bounds__(BS) void k_tri_force(GcParams P, int ntri, const fl
     const float3 P1 = vpos(a, P, b, T.y, xo, no, ox, oy, oz);
     const float3 P2 = vpos(a, P, b, T.z, xo, no, ox, oy, oz);
     float3 f[3];
-    tri_terms(a, t, P1, P2, a.Ac[t], a.Vc[t], f);
+    tri_terms(tri_par(a, t), P1, P2, a.Ac[t], a.Vc[t], f);
     tfloat3* o = tf + 3 * (size_t)t;
 #pragma unroll
     for (int k = 0; k < 3; ++k) o[k] = tfloat3{f[k].x, f[k].y, f[k].z};
@@ -558,15 +582,27 @@ __device__ __forceinline__ void cell_force(const GcParams& P, const CellArgs& ca
                            (float)((double)(float)sy[u] - (double)(float)sy[o]),
                            (float)((double)(float)sz[u] - (double)(float)sz[o]));
     };
-    // geometry: k_geo_cell's arithmetic and reduction order (256-thread sums)
+    // geometry: k_geo_cell's arithmetic and reduction order (GC_GEO_BS =
+    // GC_CELL_BS strided sums, then a tree).
+    // Global loads are issued in groups from clamped indices before their
+    // results are used (a load under a branch would wait out its latency
+    // alone); the arithmetic and its order are unchanged.
     const int o = a.tri[tb].x - 1 - v0;
     double A = 0.0, V = 0.0;
-    if (tid < 256) {
+    static_assert(GC_GEO_BS == GC_CELL_BS, "geometry reduction order");
+    {
         const double ox = ca.round_geo ? (double)(float)sx[o] : sx[o];
         const double oy = ca.round_geo ? (double)(float)sy[o] : sy[o];
         const double oz = ca.round_geo ? (double)(float)sz[o] : sz[o];
-        for (int t = tb + tid; t <= te; t += 256) {
-            const tint3 tr = a.tri[t];
+        for (int t0 = tb + tid; t0 <= te; t0 += 2 * BS) {
+        tint3 trs[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) trs[u] = a.tri[min(t0 + BS * u, te)];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int t = t0 + BS * u;
+            if (t > te) break;
+            const tint3 tr = trs[u];
             const float3 p0 = rel_geo(tr.x - 1 - v0, o), p1 = rel_geo(tr.y - 1 - v0, o),
                          p2 = rel_geo(tr.z - 1 - v0, o);
             const float3 n = f3cross(f3sub(p1, p0), f3sub(p2, p0));
@@ -584,16 +620,17 @@ __device__ __forceinline__ void cell_force(const GcParams& P, const CellArgs& ca
             A += (double)at;
             double ax = p0.x, ay = p0.y, az = p0.z, bx = p1.x, by = p1.y, bz = p1.z;
             double cx = p2.x, cy = p2.y, cz = p2.z;
-            V += (ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx)) / 6.0;
+            V += ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
+        }
         }
         sa[tid] = A; sv[tid] = V;
     }
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
+    for (int w = BS / 2; w > 0; w >>= 1) {
         if (tid < w) { sa[tid] += sa[tid + w]; sv[tid] += sv[tid + w]; }
         __syncthreads();
     }
-    const float Ac = (float)sa[0], Vc = (float)sv[0];
+    const float Ac = (float)sa[0], Vc = (float)(sv[0] / 6.0);
     if (geo) {
         if (tid == 0) { ca.area[c] = Ac; ca.vol[c] = Vc; }
         for (int t = tb + tid; t <= te; t += BS) {
@@ -602,45 +639,78 @@ __device__ __forceinline__ void cell_force(const GcParams& P, const CellArgs& ca
         }
     }
     // edges -> buf[12 (e - e0) + 3 role]
-    for (int e = e0 + tid; e < e0 + ne; e += BS) {
-        const int6 E = a.edge[e];
-        const int ua = E.x - 1 - v0;
-        float3 fe[4];
-        edge_terms(P, a, e, rel(E.y - 1 - v0, ua), rel(E.z - 1 - v0, ua), rel(E.k - 1 - v0, ua), fe);
-        float* q = buf + 12 * (e - e0);
+    for (int i0 = tid; i0 < ne; i0 += 2 * BS) {
+        int6 Es[2];
+        EdgePar qs[2];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) { q[3 * k] = fe[k].x; q[3 * k + 1] = fe[k].y; q[3 * k + 2] = fe[k].z; }
+        for (int u = 0; u < 2; ++u) {
+            const int e = e0 + min(i0 + BS * u, ne - 1);
+            Es[u] = a.edge[e];
+            qs[u] = edge_par(a, e);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int i = i0 + BS * u;
+            if (i >= ne) break;
+            const int6 E = Es[u];
+            const int ua = E.x - 1 - v0;
+            float3 fe[4];
+            edge_terms(P, qs[u], rel(E.y - 1 - v0, ua), rel(E.z - 1 - v0, ua), rel(E.k - 1 - v0, ua), fe);
+            float* q = buf + 12 * i;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) { q[3 * k] = fe[k].x; q[3 * k + 1] = fe[k].y; q[3 * k + 2] = fe[k].z; }
+        }
     }
     __syncthreads();
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    if (tid < nv) {
+    // incidences of the vertex in list order: the first GC_CELL_INC codes
+    // loaded together, any further ones one by one
+    auto sum_inc = [&](const int* __restrict__ off, const int* __restrict__ code, int base, int w) {
         const int g = v0 + tid;
-        for (int q = a.es[g]; q < a.es[g + 1]; ++q) {
-            const int code = a.ee[q];
-            const float* w = buf + 12 * ((code >> 2) - e0) + 3 * (code & 3);
-            fx += w[0]; fy += w[1]; fz += w[2];
+        const int q0 = off[g], q1 = off[g + 1];
+        int cd[GC_CELL_INC];
+#pragma unroll
+        for (int k = 0; k < GC_CELL_INC; ++k) cd[k] = code[max(min(q0 + k, q1 - 1), 0)];
+#pragma unroll
+        for (int k = 0; k < GC_CELL_INC; ++k) {
+            if (q0 + k >= q1) break;
+            const float* p = buf + w * ((cd[k] >> 2) - base) + 3 * (cd[k] & 3);
+            fx += p[0]; fy += p[1]; fz += p[2];
         }
-    }
+        for (int q = q0 + GC_CELL_INC; q < q1; ++q) {
+            const int c = code[q];
+            const float* p = buf + w * ((c >> 2) - base) + 3 * (c & 3);
+            fx += p[0]; fy += p[1]; fz += p[2];
+        }
+    };
+    if (tid < nv) sum_inc(a.es, a.ee, e0, 12);
     __syncthreads();
     // triangles -> buf[9 (t - tb) + 3 corner]
-    for (int t = tb + tid; t <= te; t += BS) {
-        const tint3 T = a.tri[t];
-        const int u0 = T.x - 1 - v0;
-        float3 ft[3];
-        tri_terms(a, t, rel(T.y - 1 - v0, u0), rel(T.z - 1 - v0, u0), Ac, Vc, ft);
-        float* q = buf + 9 * (t - tb);
+    const int nt = te - tb + 1;
+    for (int i0 = tid; i0 < nt; i0 += 2 * BS) {
+        tint3 Ts[2];
+        TriPar qs[2];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) { q[3 * k] = ft[k].x; q[3 * k + 1] = ft[k].y; q[3 * k + 2] = ft[k].z; }
-    }
-    __syncthreads();
-    if (tid < nv) {
-        const int g = v0 + tid;
-        for (int q = a.ts[g]; q < a.ts[g + 1]; ++q) {
-            const int code = a.te[q];
-            const float* w = buf + 9 * ((code >> 2) - tb) + 3 * (code & 3);
-            fx += w[0]; fy += w[1]; fz += w[2];
+        for (int u = 0; u < 2; ++u) {
+            const int t = tb + min(i0 + BS * u, nt - 1);
+            Ts[u] = a.tri[t];
+            qs[u] = tri_par(a, t);
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int i = i0 + BS * u;
+            if (i >= nt) break;
+            const tint3 T = Ts[u];
+            const int u0 = T.x - 1 - v0;
+            float3 ft[3];
+            tri_terms(qs[u], rel(T.y - 1 - v0, u0), rel(T.z - 1 - v0, u0), Ac, Vc, ft);
+            float* q = buf + 9 * i;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) { q[3 * k] = ft[k].x; q[3 * k + 1] = ft[k].y; q[3 * k + 2] = ft[k].z; }
         }
     }
+    __syncthreads();
+    if (tid < nv) sum_inc(a.ts, a.te, tb, 9);
     f = make_float3(fx, fy, fz);
 }
 
@@ -649,7 +719,7 @@ __global__ __launch_bounds__(GC_CELL_BS) void k_cell_membrane(GcParams P, const 
 {
     __shared__ double sx[GC_CELL_NV], sy[GC_CELL_NV], sz[GC_CELL_NV];
     __shared__ float buf[GC_CELL_NE * 12];
-    __shared__ double sa[256], sv[256];
+    __shared__ double sa[GC_CELL_BS], sv[GC_CELL_BS];
     if (ca.gate && !*ca.gate) return;   // block-uniform
     const int c = blockIdx.x, tid = threadIdx.x;
     const int4 r = ca.range[c];
@@ -694,6 +764,14 @@ __global__ __launch_bounds__(GC_CELL_BS) void k_cell_membrane(GcParams P, const 
     }
 }
 
+// a pointer the compiler cannot see through (keeps loads inside a loop)
+template <class T>
+__device__ __forceinline__ T* opaque(T* p)
+{
+    asm volatile("" : "+s"(p));
+    return p;
+}
+
 // k velocity-Verlet substeps of one cell's membrane under the fixed fluid
 // force fe: per substep the kick and drift of k_ms_kick_drift, the membrane
 // force of k_cell_membrane (geometry from the FP32-rounded positions) and
@@ -704,7 +782,7 @@ __global__ __launch_bounds__(GC_CELL_BS) void k_cell_substeps(GcParams P, CellAr
 {
     __shared__ double sx[GC_CELL_NV], sy[GC_CELL_NV], sz[GC_CELL_NV];
     __shared__ float buf[GC_CELL_NE * 12];
-    __shared__ double sa[256], sv[256];
+    __shared__ double sa[GC_CELL_BS], sv[GC_CELL_BS];
     if (ca.gate && !*ca.gate) return;   // block-uniform
     const int c = blockIdx.x, tid = threadIdx.x;
     const int4 r = ca.range[c];
@@ -731,7 +809,16 @@ __global__ __launch_bounds__(GC_CELL_BS) void k_cell_substeps(GcParams P, CellAr
             if (keep) { keep[g] = sx[tid]; keep[nm + g] = sy[tid]; keep[2 * nm + g] = sz[tid]; }
         }
         __syncthreads();
-        cell_force(P, ca, a, c, tid, v0, nv, e0, ne, tb, te, q == sa_.k, sx, sy, sz, buf, sa, sv, fm);
+        // the topology is re-read every substep (grouped loads, mostly cache
+        // hits) rather than held in registers across the loop
+        DefArgs al = a;
+        al.tri = opaque(al.tri); al.edge = opaque(al.edge);
+        al.es = opaque(al.es); al.ee = opaque(al.ee); al.ts = opaque(al.ts); al.te = opaque(al.te);
+        al.lmax = opaque(al.lmax); al.p = opaque(al.p); al.kp = opaque(al.kp); al.mexp = opaque(al.mexp);
+        al.kb = opaque(al.kb); al.th0 = opaque(al.th0); al.ka = opaque(al.ka); al.A0c = opaque(al.A0c);
+        al.kd = opaque(al.kd); al.reftriarea = opaque(al.reftriarea); al.kv = opaque(al.kv);
+        al.V0c = opaque(al.V0c);
+        cell_force(P, ca, al, c, tid, v0, nv, e0, ne, tb, te, q == sa_.k, sx, sy, sz, buf, sa, sv, fm);
         if (own) ms_kick(u, fe, fm, dts, sa_.inv_m);
         __syncthreads();
     }
@@ -741,6 +828,13 @@ __global__ __launch_bounds__(GC_CELL_BS) void k_cell_substeps(GcParams P, CellAr
     sa_.mcoor[g] = tfloat3{(float)x, (float)y, (float)z};
     sa_.mv[g] = tfloat3{u.x, u.y, u.z};
     sa_.fm[g] = tfloat3{fm.x, fm.y, fm.z};
+}
+
+__global__ void k_sincos(int n, const float* __restrict__ th, float* __restrict__ c, float* __restrict__ s)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    sincosf(th[i], &s[i], &c[i]);
 }
 
 bool cell_ranges(int nm, int nedge, int ntri, int ncell, const int* celltab, const tint2* betri,
@@ -856,7 +950,7 @@ extern "C" int gc_CalGeoPara(long stepcount, int ntri, int ncell, float* d_bbox,
         !d_refvol_para || !d_area_para || !d_vol_para || !d_area || !d_vol)
         return set_error(GC_ERR_INVALID, "CalGeoPara: null pointer");
     hipStream_t s = stream();
-    k_geo_cell<<<ncell, BS, 0, s>>>(P, stepcount, nullptr, nullptr, d_bbox, d_curpos, d_tri, d_beginendtri,
+    k_geo_cell<<<ncell, GC_GEO_BS, 0, s>>>(P, stepcount, nullptr, nullptr, d_bbox, d_curpos, d_tri, d_beginendtri,
                                     d_nthrobound, d_totalcoor, d_trinordir, d_triarea,
                                     d_reftriarea, d_tricenter, d_refarea_para, d_refvol_para,
                                     d_area_para, d_vol_para, d_area, d_vol);

@@ -481,6 +481,7 @@ DefArgs ms_defargs(gc_engine* e, MembraneSub* m)
     DefArgs a;
     a.edge = e->d_edge; a.tri = e->d_tri; a.coor = m->mcoor; a.nthro = m->mnt; a.curpos = m->mid;
     a.lmax = e->d_lmax; a.mexp = e->d_mexp; a.p = e->d_p; a.kp = e->d_kp; a.th0 = e->d_th0;
+    a.cth0 = e->d_cth0; a.sth0 = e->d_sth0;
     a.kb = e->d_kb; a.reftriarea = e->d_reftriarea; a.kd = e->d_kd; a.ka = e->d_ka;
     a.Ac = e->d_area_para; a.A0c = e->d_refarea_para; a.kv = e->d_kv; a.Vc = e->d_vol_para;
     a.V0c = e->d_refvol_para; a.es = e->d_es; a.ee = e->d_ee; a.ts = e->d_ts; a.te = e->d_te;
@@ -501,7 +502,7 @@ int ms_force(gc_engine* e, MembraneSub* m, const int* gate = nullptr)
         GC_LAUNCH_CHECK("membrane substep (per cell)");
         return GC_OK;
     }
-    k_geo_cell<<<e->ncell, BS, 0, s>>>(m->Pm, e->step, nullptr, nullptr, e->d_bbox, m->mid, e->d_tri,
+    k_geo_cell<<<e->ncell, GC_GEO_BS, 0, s>>>(m->Pm, e->step, nullptr, nullptr, e->d_bbox, m->mid, e->d_tri,
         e->d_betri, m->mnt, m->mcoor, e->d_trinor, e->d_triarea, e->d_reftriarea, e->d_tricen,
         e->d_refarea_para, e->d_refvol_para, e->d_area_para, e->d_vol_para, e->d_area, e->d_vol);
     return membrane_force(e, m->Pm, a, s);
