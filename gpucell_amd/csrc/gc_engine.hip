@@ -96,6 +96,25 @@ __device__ __forceinline__ void key_to_cc(int k, const GcRep& R, int& cx, int& c
     cz = t / R.nc[1];
 }
 
+// Runs body(R) with the lane's replica record.  A batch wave whose lanes
+// all belong to one replica (all but the few waves at replica boundaries)
+// passes that replica's record at a wave-uniform address, so its fields
+// are read with scalar loads once per wave instead of vector loads per lane;
+// body is inlined for both cases.  live: lanes with a valid id.
+template <bool ENS, class Body>
+__device__ __forceinline__ void with_rep(const GcRep& R0, const GcRep* __restrict__ reps,
+                                         const int* __restrict__ rep_of, bool live, int id, Body&& body)
+{
+    if constexpr (ENS) {
+        const int r = live ? rep_of[id] : -1;
+        const int r0 = __builtin_amdgcn_readfirstlane(r);
+        if (__ballot(live && r != r0) == 0) body(reps[r0]);
+        else body(reps[r < 0 ? 0 : r]);
+    } else {
+        body(R0);
+    }
+}
+
 // The replica of global id `id`: the kernel argument for one suspension,
 // the table entry for a batch (a reference either way: a by-value copy of
 // the whole record costs registers or scratch).
@@ -195,9 +214,9 @@ __global__ __launch_bounds__(TBS) void k2_integrate(int n, GcRep R0, const GcRep
     const bool act = p < n;
     if (__ballot(act) == 0) return;   // wave-uniform
     int k = -1;
+    const int id = act ? sp[p] : 0;
+    with_rep<ENS>(R0, reps, rep_of, act, id, [&](const GcRep& R) __attribute__((always_inline)) {
     if (act) {
-        int id = sp[p];
-        const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
         const GcParams& P = R.P;
         double x = pos64[p], y = pos64[n + p], z = pos64[2 * n + p];
         if (move && moves(P, id, mmove)) {
@@ -210,6 +229,7 @@ __global__ __launch_bounds__(TBS) void k2_integrate(int n, GcRep R0, const GcRep
         k = R.cube_off + cx + R.nc[0] * (cy + R.nc[1] * cz);
         key[p] = k;
     }
+    });
     // runs of equal keys among the active lanes (active lanes are a prefix)
     const int lane = threadIdx.x & 63;
     const int prev = __shfl_up(k, 1);
@@ -333,8 +353,9 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     double x = pos_old[o], y = pos_old[n + o], z = pos_old[2 * n + o];
     tint3 nt = nt_old[o];
     float4 v = vout[o];
-    if (move && moves(R0.P, id, mmove)) {
-        const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
+    const bool mv = move && moves(R0.P, id, mmove);
+    with_rep<ENS>(R0, reps, rep_of, mv, id, [&](const GcRep& R) __attribute__((always_inline)) {
+    if (mv) {
         const GcParams& P = R.P;
         const tfloat3 f = force[o];
         int img[3] = {0, 0, 0};
@@ -348,6 +369,7 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
         // the exported wall velocity, so reading the state changes nothing)
         velhalf[s] = id < R0.P.nf ? tfloat3{0.f, 0.f, 0.f} : tfloat3{v.x, v.y, v.z};
     }
+    });
     pos_new[s] = x; pos_new[n + s] = y; pos_new[2 * n + s] = z;
     float fx = (float)x, fy = (float)y, fz = (float)z;
     posid[s] = make_float4(fx, fy, fz, __int_as_float(id));
@@ -441,7 +463,7 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
     const int id = inb ? __float_as_int(pi.w) : 0;
     const bool act = inb && id >= R0.P.nb2;
     if (__ballot(act) == 0) return;   // wave-uniform
-    const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, act ? id : 0);
+    with_rep<ENS>(R0, reps, rep_of, inb, id, [&](const GcRep& R) __attribute__((always_inline)) {
     const GcParams& P = R.P;
     int cx = 0, cy = 0, cz = 0;
     if (act) key_to_cc(key[p], R, cx, cy, cz);
@@ -523,6 +545,7 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
         v.x = svx * s; v.y = svy * s; v.z = svz * s;
         velrho[p] = v;
     }
+    });
 }
 
 // [10-13,17] SDPD pair force with the pressure from the EOS on the fly,
@@ -541,7 +564,7 @@ __global__ __launch_bounds__(TBS) void k2_force(int n, GcRep R0, const GcRep* __
     float4 pi = posid[p];
     int id = __float_as_int(pi.w);
     if (id < R0.P.nf) { force[p] = tfloat3{0.f, 0.f, 0.f}; return; }
-    const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
+    with_rep<ENS>(R0, reps, rep_of, true, id, [&](const GcRep& R) __attribute__((always_inline)) {
     const GcParams& P = R.P;   // a reference: a copy of the batch record would live in scratch
     const unsigned long long c = *ctr - 1ull;   // advanced by this step's offsets pass
     const uint32_t ctr_lo = (uint32_t)c, ctr_hi = (uint32_t)(c >> 32);
@@ -638,6 +661,7 @@ __global__ __launch_bounds__(TBS) void k2_force(int n, GcRep R0, const GcRep* __
         tfloat3 vh = velhalf[p];
         vout[p] = make_float4(vh.x + h * fx, vh.y + h * fy, vh.z + h * fz, vi.w);
     }
+    });
 }
 
 // [16] Morse between membrane vertices of different cells
@@ -657,7 +681,7 @@ __global__ __launch_bounds__(BS) void k2_agg(int nm, GcRep R0, const GcRep* __re
     const int p = curpos[R0.P.nc + v];
     float4 pi = posid[p];
     int id = __float_as_int(pi.w);
-    const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
+    with_rep<ENS>(R0, reps, rep_of, true, id, [&](const GcRep& R) __attribute__((always_inline)) {
     const GcParams& P = R.P;
     if (P.De == 0.0f) return;
     int cell = celltab[id - P.nc];
@@ -696,6 +720,7 @@ __global__ __launch_bounds__(BS) void k2_agg(int nm, GcRep R0, const GcRep* __re
     tfloat3 F = force[p];
     F.x += fx; F.y += fy; F.z += fz;
     force[p] = F;
+    });
 }
 
 // [18] wall velocity from the corrected velocities; walls' entry of vout
