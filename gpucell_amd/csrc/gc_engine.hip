@@ -340,7 +340,8 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     const float4* __restrict__ vout, const tfloat3* __restrict__ force,
     const tint3* __restrict__ nt_old, tint3* __restrict__ nt_new, int* __restrict__ sp_new,
     int* __restrict__ key_new, float4* __restrict__ posid, tfloat3* __restrict__ coor,
-    float4* __restrict__ velrho, tfloat3* __restrict__ velhalf)
+    float4* __restrict__ velrho, tfloat3* __restrict__ velhalf, const int* __restrict__ wincl,
+    int* __restrict__ act)
 {
     int q = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (q >= n) return;
@@ -379,6 +380,12 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     key_new[s] = k;
     velrho[s] = v;
     if (id >= R0.P.nc) curpos[id] = s;
+    if (id >= R0.P.nf) {
+        // a cube holds its walls (the smallest ids) first, so the moving
+        // particles before this one are s minus the walls of cubes <= k
+        const int ai = s - wincl[k];
+        if ((unsigned)ai < (unsigned)n) act[ai] = s;
+    }
 }
 
 // [7,9] density of moving particles; velocity of Bound2 walls from the
@@ -552,18 +559,20 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
 // Bound1 repulsion, body force; corrector for fluid (membrane vertices get
 // it after their membrane force).
 template <bool ENS, int TBS>
-__global__ __launch_bounds__(TBS) void k2_force(int n, GcRep R0, const GcRep* __restrict__ reps,
-    const int* __restrict__ rep_of, int driveon, const float4* __restrict__ posid,
+__global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep* __restrict__ reps,
+    const int* __restrict__ rep_of, int driveon, const int* __restrict__ act, const float4* __restrict__ posid,
     const float4* __restrict__ velrho, const int* __restrict__ offs, const int* __restrict__ key,
     const int* __restrict__ nl, const int* __restrict__ nlc,
     const tfloat3* __restrict__ velhalf, tfloat3* __restrict__ force, float4* __restrict__ vout,
     const unsigned long long* __restrict__ ctr)
 {
-    int p = xcd_block((n + TBS - 1) / TBS) * TBS + threadIdx.x;
-    if (p >= n) return;
+    // one lane per moving particle (walls get no force: no lane idles on a
+    // wall slot); act lists their slots in slot order
+    const int t = xcd_block((nmov + TBS - 1) / TBS) * TBS + threadIdx.x;
+    if (t >= nmov) return;
+    const int p = act[t];
     float4 pi = posid[p];
     int id = __float_as_int(pi.w);
-    if (id < R0.P.nf) { force[p] = tfloat3{0.f, 0.f, 0.f}; return; }
     with_rep<ENS>(R0, reps, rep_of, true, id, [&](const GcRep& R) __attribute__((always_inline)) {
     const GcParams& P = R.P;   // a reference: a copy of the batch record would live in scratch
     const unsigned long long c = *ctr - 1ull;   // advanced by this step's offsets pass
@@ -636,7 +645,7 @@ __global__ __launch_bounds__(TBS) void k2_force(int n, GcRep R0, const GcRep* __
     // order of the cubes, so the sums below run in that order either way.
     const int cnt = nlc[p];
     if (cnt >= 0) {
-        const int* wl = nl + (size_t)__builtin_amdgcn_readfirstlane(p >> 6) * (NL_CAP * 64) + (p & 63);
+        const int* wl = nl + (size_t)(p >> 6) * (NL_CAP * 64) + (p & 63);   // the density lane's block
         int next = cnt > 0 ? wl[0] : 0;   // the next entry is loaded during the current pair
         for (int k = 0; k < cnt; ++k) {
             const int code = next;
@@ -832,7 +841,8 @@ int launch_build_t(gc_engine* e, int move, Prof* pr = nullptr)
     GC_LAUNCH_CHECK("engine scatter");
     k2_sortgather<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, mfree, e->d_tk, e->d_tv,
         e->d_to, e->d_offs, e->d_curpos, e->d_pos64[c], e->d_pos64[nx], e->d_vout, e->d_force, e->d_nt[c],
-        e->d_nt[nx], e->d_sp[nx], e->d_key, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf);
+        e->d_nt[nx], e->d_sp[nx], e->d_key, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf, e->d_wincl,
+        e->d_act);
     GC_LAUNCH_CHECK("engine sort/gather");
     e->cur = nx;
     return GC_OK;
@@ -879,9 +889,10 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
     mark(pr, s, "density+wall velocity");
     {
         auto L = [&](auto kern, int tbs) {
-            kern<<<xcd_grid(blocks_for(N, tbs)), tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, driveon,
-                e->d_posid, e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc, e->d_velhalf, e->d_force,
-                e->d_vout, e->d_ctr);
+            if (e->nmov > 0)
+                kern<<<xcd_grid(blocks_for(e->nmov, tbs)), tbs, 0, s>>>(e->nmov, R0, e->d_reps, e->d_rep_of,
+                    driveon, e->d_act, e->d_posid, e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc,
+                    e->d_velhalf, e->d_force, e->d_vout, e->d_ctr);
         };
         switch (e->tune.v[TUNE_FORCE]) {
             case 0: L(k2_force<ENS, 64>, 64); break;
@@ -1033,6 +1044,29 @@ int reorder_carried(gc_engine* e, const std::vector<float4>& vout_id,
     return hip_check(hipStreamSynchronize(e->s), "reorder sync2");
 }
 
+// walls per cube from the current keys (walls never move), the inclusive
+// prefix over cubes, and the number of moving particles
+int wall_tables(gc_engine* e)
+{
+    const int N = e->N, K = e->K, nf = e->hreps[0].P.nf;
+    std::vector<int> sp(N), key(N), w(K + 1, 0);
+    int st = hip_check(hipStreamSynchronize(e->s), "wall table sync");
+    if (!st) st = hip_check(hipMemcpy(sp.data(), e->d_sp[e->cur], sizeof(int) * N, hipMemcpyDeviceToHost), "wall table");
+    if (!st) st = hip_check(hipMemcpy(key.data(), e->d_key, sizeof(int) * N, hipMemcpyDeviceToHost), "wall table");
+    if (st) return st;
+    int nwall = 0;
+    for (int s = 0; s < N; ++s)
+        if (sp[s] < nf) {
+            if (key[s] < 0 || key[s] >= K) return set_error(GC_ERR_GRID, "wall table: key out of range");
+            w[key[s]] += 1;
+            ++nwall;
+        }
+    for (int k = 1; k < K; ++k) w[k] += w[k - 1];
+    e->nmov = N - nwall;
+    if ((st = upload(e->d_wincl, w.data(), (size_t)K, e->s))) return st;
+    return hip_check(hipStreamSynchronize(e->s), "wall table sync");   // w is freed on return
+}
+
 int validate_desc(const gc_case_desc* d)
 {
     if (!d || !d->coor || d->nBound1Par < 0 || d->nBound2Par < 0 || d->nFluidPar < 0 ||
@@ -1163,7 +1197,10 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
         (st = e->alloc(&e->d_tv, N)) || (st = e->alloc(&e->d_to, N)) || (st = e->alloc(&e->d_curpos, N)) ||
         (st = e->alloc(&e->d_nl, (size_t)NL_CAP * (((size_t)N + 63) & ~(size_t)63))) || (st = e->alloc(&e->d_nlc, N)) ||
         (st = e->alloc(&e->d_ctr, 1)) ||
-        (st = e->alloc(&e->d_tsum, (size_t)e->ntile)))
+        (st = e->alloc(&e->d_tsum, (size_t)e->ntile)) || (st = e->alloc(&e->d_act, N)) ||
+        (st = e->alloc(&e->d_wincl, K + 1)) ||
+        (st = hip_check(hipMemset(e->d_wincl, 0, sizeof(int) * (K + 1)), "engine wall table")) ||
+        (st = hip_check(hipMemset(e->d_act, 0, sizeof(int) * N), "engine moving slots")))
         return fail(st);
     hipStream_t s = e->s;
     std::vector<float4> vout_id(N);
@@ -1206,6 +1243,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
     // first neighbour build (Particle::ConCubeList at init, Particle.cu:570)
     if ((st = launch_build(e, 0))) return fail(st);
     if ((st = reorder_carried(e, vout_id, f_id, vh_id))) return fail(st);
+    if ((st = wall_tables(e))) return fail(st);
 
     if (e->ncell > 0) {
         const size_t T = (size_t)e->ntri, E = (size_t)e->nedge, C = (size_t)e->ncell, M = (size_t)e->nm;
@@ -1423,7 +1461,9 @@ int snapshot(gc_engine* e, Snapshot& S)
     for (int slot = 0; slot < N; ++slot) {
         int id = sp[slot];
         for (int d = 0; d < 3; ++d) S.pos[3 * (size_t)id + d] = pos[(size_t)d * N + slot];
-        S.vo[id] = vo[slot]; S.vh[id] = vh[slot]; S.f[id] = f[slot]; S.co[id] = co[slot];
+        // the force pass leaves wall slots alone: walls carry no force
+        S.vo[id] = vo[slot]; S.vh[id] = vh[slot]; S.co[id] = co[slot];
+        S.f[id] = id < e->hreps[0].P.nf ? tfloat3{0.f, 0.f, 0.f} : f[slot];
         S.nt[id] = nt[slot];
     }
     return GC_OK;
@@ -1508,6 +1548,7 @@ int restore_state(gc_engine* e, const std::vector<double>& pos, const std::vecto
         return st;
     if ((st = launch_build(e, 0))) return st;
     if ((st = reorder_carried(e, vo, f, vh))) return st;
+    if ((st = wall_tables(e))) return st;
     e->walls_dirty = false;
     e->step = step;
     if (e->ms && (st = ms_reset(e))) return st;

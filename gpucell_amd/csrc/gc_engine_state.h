@@ -67,6 +67,11 @@ struct gc_engine {
     float4 *d_posid, *d_velrho, *d_vout;
     tfloat3 *d_coor, *d_velhalf, *d_force;
     int *d_key0, *d_key, *d_rank, *d_cnt, *d_offs, *d_tk, *d_tv, *d_to, *d_curpos;
+    // moving particles: walls do not move, so the number of walls per cube is
+    // fixed; wincl = its inclusive prefix over cubes; act[i] = slot of the
+    // i-th moving particle in slot order (written by the gather), nmov of them
+    int *d_wincl = nullptr, *d_act = nullptr;
+    int nmov = 0;
     int* d_tsum = nullptr;   // cube counts per scan tile
     int ntile = 0;
     int *d_nl, *d_nlc;   // neighbour lists of moving particles (density -> force)

@@ -37,8 +37,14 @@ def bound1_case():
     return util.outer_walls_as_bound1(c, ph, 6.0)
 
 
+def big_cell_case():
+    """One 2562-vertex cell (icosphere subdivided 4x): more vertices than a
+    workgroup's LDS holds, so the engine takes the three-pass membrane form."""
+    return casegen.periodic_box(n=20, rbc_diameter=11.0, subdiv=4, seed=12)
+
+
 CASES = {"channel": lambda: channel_case(), "channel_morse": lambda: channel_case(De=0.02),
-         "box": box_case, "channel_bound1": bound1_case}
+         "box": box_case, "channel_bound1": bound1_case, "box_big_cell": big_cell_case}
 
 
 @pytest.mark.parametrize("name", ["channel", "box"])
@@ -73,7 +79,7 @@ def _oracle_state(o, case, S, info):
     return st
 
 
-@pytest.mark.parametrize("name", ["channel", "channel_morse", "box", "channel_bound1"])
+@pytest.mark.parametrize("name", ["channel", "channel_morse", "box", "channel_bound1", "box_big_cell"])
 def test_engine_step_parity(gpu, name):
     case = CASES[name]()
     e = Engine(case)
