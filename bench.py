@@ -222,7 +222,9 @@ def main():
     if os.path.exists(vp):
         vj = json.load(open(vp)).get("kernels", {}).get(KERNEL_NAMES[dom])
         if vj:
-            waves = (N + 63) // 64
+            # waves the kernel launches on this case (the force pass runs one
+            # lane per moving particle), as recorded with the counters
+            waves = vj.get("waves", (N + 63) // 64)
             issue_us = vj["valu_per_wave"] * waves * 4 / (1024 * 2.4e9) * 1e6
             valu = {"kernel": KERNEL_NAMES[dom], "valu_per_wave": vj["valu_per_wave"],
                     "issue_bound_us": issue_us, "avg_us": dom_us, "frac": issue_us / dom_us,
