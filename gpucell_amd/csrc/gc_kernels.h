@@ -124,7 +124,8 @@ constexpr int GC_CELL_NV = 768;
 constexpr int GC_CELL_NE = 2048;
 constexpr int GC_CELL_NT = (GC_CELL_NE * 12) / 9;
 constexpr int GC_GEO_BS = GC_CELL_BS;   // threads of k_geo_cell (sets its FP64 reduction order)
-constexpr int GC_CELL_INC = 12;   // incidences per vertex read together (more are read one by one)
+constexpr int GC_CELL_INC = 12;   // edge incidences per vertex read together (more are read one by one)
+constexpr int GC_CELL_TINC = 8;   // triangle incidences per vertex read together
 struct CellArgs {
     const int4* range;    // (first vertex, vertices, first edge, edges) per cell, 0-based
     const tint2* betri;   // triangle range per cell (inclusive)

@@ -560,15 +560,52 @@ __global__ __launch_bounds__(BS) void k_vert_force(GcParams P, int nm, DefArgs a
     }
 }
 
+// A vertex's incidence list [q0, q1) of (record << 2 | role) codes: the
+// first NC codes loaded together from clamped indices, and the sum of the
+// matching LDS terms (w floats per record) in list order.
+template <int NC>
+struct IncList {
+    int q0 = 0, q1 = 0;
+    int cd[NC];
+    __device__ __forceinline__ void load(const int* __restrict__ off, const int* __restrict__ code, int g)
+    {
+        q0 = off[g];
+        q1 = off[g + 1];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) cd[k] = code[max(min(q0 + k, q1 - 1), 0)];
+    }
+    __device__ __forceinline__ void sum(const float* buf, int w, int base, const int* __restrict__ code, float& fx,
+                                        float& fy, float& fz) const
+    {
+#pragma unroll
+        for (int k = 0; k < NC; ++k) {
+            if (q0 + k >= q1) break;
+            const float* p = buf + w * ((cd[k] >> 2) - base) + 3 * (cd[k] & 3);
+            fx += p[0]; fy += p[1]; fz += p[2];
+        }
+        for (int q = q0 + NC; q < q1; ++q) {   // more incidences than NC: one by one
+            const int c = code[q];
+            const float* p = buf + w * ((c >> 2) - base) + 3 * (c & 3);
+            fx += p[0]; fy += p[1]; fz += p[2];
+        }
+    }
+};
+using EdgeInc = IncList<GC_CELL_INC>;
+using TriInc = IncList<GC_CELL_TINC>;
+
 // Geometry and membrane force of one cell from the FP64 unwrapped positions
 // of its vertices in sx/sy/sz (LDS, local index); on return thread tid < nv
 // holds the force on vertex v0 + tid in f.  With geo the geometry outputs
 // (normals, areas, centroids, cell area and volume) are written.  Ends after
 // reading buf: a caller that refills sx or buf first passes a barrier.
+// With PRE the vertex's incidence lists (ei, ti) were loaded by the caller
+// (ahead of the phases that need them); otherwise they are loaded here.
+template <bool PRE>
 __device__ __forceinline__ void cell_force(const GcParams& P, const CellArgs& ca, const DefArgs& a, int c, int tid,
                                            int v0, int nv, int e0, int ne, int tb, int te, bool geo,
                                            const double* sx, const double* sy, const double* sz, float* buf,
-                                           double* sa, double* sv, float3& f)
+                                           double* sa, double* sv, float3& f, const EdgeInc& ei_pre,
+                                           const TriInc& ti_pre)
 {
     constexpr int BS = GC_CELL_BS;
     // differences of unwrapped positions rounded once (what urel and vpos
@@ -663,27 +700,15 @@ __device__ __forceinline__ void cell_force(const GcParams& P, const CellArgs& ca
     }
     __syncthreads();
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    // incidences of the vertex in list order: the first GC_CELL_INC codes
-    // loaded together, any further ones one by one
-    auto sum_inc = [&](const int* __restrict__ off, const int* __restrict__ code, int base, int w) {
-        const int g = v0 + tid;
-        const int q0 = off[g], q1 = off[g + 1];
-        int cd[GC_CELL_INC];
-#pragma unroll
-        for (int k = 0; k < GC_CELL_INC; ++k) cd[k] = code[max(min(q0 + k, q1 - 1), 0)];
-#pragma unroll
-        for (int k = 0; k < GC_CELL_INC; ++k) {
-            if (q0 + k >= q1) break;
-            const float* p = buf + w * ((cd[k] >> 2) - base) + 3 * (cd[k] & 3);
-            fx += p[0]; fy += p[1]; fz += p[2];
+    if (tid < nv) {
+        if constexpr (PRE) {
+            ei_pre.sum(buf, 12, e0, a.ee, fx, fy, fz);
+        } else {
+            EdgeInc ei;
+            ei.load(a.es, a.ee, v0 + tid);
+            ei.sum(buf, 12, e0, a.ee, fx, fy, fz);
         }
-        for (int q = q0 + GC_CELL_INC; q < q1; ++q) {
-            const int c = code[q];
-            const float* p = buf + w * ((c >> 2) - base) + 3 * (c & 3);
-            fx += p[0]; fy += p[1]; fz += p[2];
-        }
-    };
-    if (tid < nv) sum_inc(a.es, a.ee, e0, 12);
+    }
     __syncthreads();
     // triangles -> buf[9 (t - tb) + 3 corner]
     const int nt = te - tb + 1;
@@ -710,7 +735,15 @@ __device__ __forceinline__ void cell_force(const GcParams& P, const CellArgs& ca
         }
     }
     __syncthreads();
-    if (tid < nv) sum_inc(a.ts, a.te, tb, 9);
+    if (tid < nv) {
+        if constexpr (PRE) {
+            ti_pre.sum(buf, 9, tb, a.te, fx, fy, fz);
+        } else {
+            TriInc ti;
+            ti.load(a.ts, a.te, v0 + tid);
+            ti.sum(buf, 9, tb, a.te, fx, fy, fz);
+        }
+    }
     f = make_float3(fx, fy, fz);
 }
 
@@ -732,12 +765,23 @@ __global__ __launch_bounds__(GC_CELL_BS) void k_cell_membrane(GcParams P, const 
         bb = R.bbox;
     }
     const PerBox b = load_box(P, bb);
-    // unwrapped FP64 positions (k2_mgather's values) of the cell's vertices
+    // Everything a vertex's thread reads from global memory besides the
+    // records of the phases is loaded here, ahead of its use: the
+    // incidence lists for the two per-vertex sums and the inputs of the
+    // force update and corrector (written by the preceding passes).
+    EdgeInc ei;
+    TriInc ti;
     int slot = -1;
+    tfloat3 F0 = tfloat3{0.f, 0.f, 0.f}, vh = F0;
+    float rw = 0.f;
+    // unwrapped FP64 positions (k2_mgather's values) of the cell's vertices
     if (tid < nv) {
         const int g = v0 + tid;
+        ei.load(a.es, a.ee, g);
+        ti.load(a.ts, a.te, g);
         if (a.mpos) {
             sx[tid] = a.mpos[g]; sy[tid] = a.mpos[a.nm + g]; sz[tid] = a.mpos[2 * a.nm + g];
+            if (a.force) slot = a.curpos[P.nc + g];
         } else {
             slot = a.curpos[P.nc + g];
             const tfloat3 x = a.coor[slot];
@@ -746,21 +790,24 @@ __global__ __launch_bounds__(GC_CELL_BS) void k_cell_membrane(GcParams P, const 
             sy[tid] = (double)x.y + (b.per[1] ? (double)n.y * (double)b.L[1] : 0.0);
             sz[tid] = (double)x.z + (b.per[2] ? (double)n.z * (double)b.L[2] : 0.0);
         }
+        if (a.force) {
+            F0 = a.force[slot];
+            if (a.vout) { vh = a.velhalf[slot]; rw = a.velrho[slot].w; }
+        }
     }
     __syncthreads();
     float3 f;
-    cell_force(P, ca, a, c, tid, v0, nv, e0, ne, tb, te, true, sx, sy, sz, buf, sa, sv, f);
+    cell_force<true>(P, ca, a, c, tid, v0, nv, e0, ne, tb, te, true, sx, sy, sz, buf, sa, sv, f, ei, ti);
     if (tid >= nv) return;
     const int g = v0 + tid;
     a.defforce[g] = tfloat3{f.x, f.y, f.z};
     if (!a.force) return;   // membrane-only evaluation (substepping)
-    tfloat3 F = a.force[slot];
+    tfloat3 F = F0;
     F.x += f.x; F.y += f.y; F.z += f.z;
     a.force[slot] = F;
     if (a.vout) {  // this is the vertex's last force term -> corrector
         const float h = 0.5f * P.dt * P.inv_m;
-        const tfloat3 vh = a.velhalf[slot];
-        a.vout[slot] = make_float4(vh.x + h * F.x, vh.y + h * F.y, vh.z + h * F.z, a.velrho[slot].w);
+        a.vout[slot] = make_float4(vh.x + h * F.x, vh.y + h * F.y, vh.z + h * F.z, rw);
     }
 }
 
@@ -818,7 +865,8 @@ __global__ __launch_bounds__(GC_CELL_BS) void k_cell_substeps(GcParams P, CellAr
         al.kb = opaque(al.kb); al.th0 = opaque(al.th0); al.ka = opaque(al.ka); al.A0c = opaque(al.A0c);
         al.kd = opaque(al.kd); al.reftriarea = opaque(al.reftriarea); al.kv = opaque(al.kv);
         al.V0c = opaque(al.V0c);
-        cell_force(P, ca, al, c, tid, v0, nv, e0, ne, tb, te, q == sa_.k, sx, sy, sz, buf, sa, sv, fm);
+        cell_force<false>(P, ca, al, c, tid, v0, nv, e0, ne, tb, te, q == sa_.k, sx, sy, sz, buf, sa, sv, fm,
+                          EdgeInc{}, TriInc{});
         if (own) ms_kick(u, fe, fm, dts, sa_.inv_m);
         __syncthreads();
     }
