@@ -52,7 +52,8 @@ class gc_ext_consts(C.Structure):
     _fields_ = [
         ("periodic", C.c_int * 3), ("force_mode", C.c_int), ("force_split", C.c_float),
         ("noise_on", C.c_int), ("wlc_ratio", C.c_float), ("pow_index", C.c_float),
-        ("rep_scale", C.c_float), ("pad0", C.c_int), ("noise_counter", C.c_uint64),
+        ("rep_scale", C.c_float), ("wall_split", C.c_float), ("wall_u_lo", C.c_float),
+        ("wall_u_hi", C.c_float), ("noise_counter", C.c_uint64),
     ]
 
 
@@ -101,7 +102,7 @@ class gc_engine_profile(C.Structure):
 
 
 assert C.sizeof(Structconst) == 96
-assert C.sizeof(gc_ext_consts) == 48
+assert C.sizeof(gc_ext_consts) == 56
 
 P = C.c_void_p
 I = C.c_int

@@ -418,6 +418,45 @@ def periodic_box(n: int = 32, n_rbc: int = 1, rbc_diameter: float = 12.0, seed: 
                      cells, F0, E0, rng, jitter, 1.0)
 
 
+def couette(n: int = 24, n_rbc: int = 0, rbc_diameter: float = 8.0, shear_rate: float = 0.005,
+            seed: int = 20261203, physics: Physics | None = None, jitter: float = 0.1, subdiv: int = 2,
+            wall_layers: int = 3, noise: bool = True) -> Case:
+    """Plane Couette shear: an n^3 box of unit-spaced fluid, periodic in x
+    and z, between `wall_layers` frozen Bound2 layers below y = 0 and above
+    y = n that move along x at -U and +U (U = shear_rate n / 2; positions
+    stay fixed, the wall velocity enters through the no-slip image), no body
+    force, `n_rbc` RBCs placed at random."""
+    ph = dataclasses.replace(physics or Physics(), fext=0.0)
+    rng = np.random.default_rng(seed)
+    L = float(n)
+    bbox = np.array([0, L, 0, L, 0, L], dtype=np.float32)
+    ext = abi.default_ext()
+    ext.periodic[0], ext.periodic[1], ext.periodic[2] = 1, 0, 1
+    ext.force_mode = 0
+    ext.noise_on = 1 if noise else 0
+    U = 0.5 * shear_rate * L
+    ext.wall_split, ext.wall_u_lo, ext.wall_u_hi = 0.5 * L, -U, U
+    nl = wall_layers
+    pts = _lattice([0, -nl, 0], [L, L + nl, L], 1.0)
+    fluid = pts[(pts[:, 1] >= 0) & (pts[:, 1] < L)]
+    walls = pts[(pts[:, 1] < 0) | (pts[:, 1] >= L)]
+    V0, F0 = icosphere(subdiv)
+    shape = biconcave(V0, rbc_diameter)
+    gap = 1.0
+
+    def inside(X):
+        return bool(np.all((X[:, 1] > gap) & (X[:, 1] < L - 1 - gap)))
+
+    cells = _place_cells(rng, n_rbc, shape, inside, gap, np.zeros(3), np.full(3, L),
+                         period=(L, None, L)) if n_rbc else []
+    cells = [X - L * np.floor(X.mean(0) / L) * np.array([1.0, 0, 1.0]) for X in cells]
+    E0 = edge_records(F0)
+    case = _assemble(f"couette{n}", ph, ext, bbox, np.zeros((0, 3)), walls, fluid, cells, F0, E0, rng,
+                     jitter, 1.0)
+    case.shear_rate = shear_rate
+    return case
+
+
 def channel(length: int, width: int, n_rbc: int, hematocrit: float | None = None,
             rbc_diameter: float | None = None, seed: int = 20261203,
             physics: Physics | None = None, jitter: float = 0.1, wall_layers: int = 3,

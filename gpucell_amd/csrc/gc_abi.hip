@@ -59,6 +59,7 @@ void make_params(const Structconst& c, const gc_ext_consts& ext, GcParams* P)
     P->De = c.surEnergy; P->mcut2 = c.MaxCut * c.MaxCut; P->r0 = c.zeroLen; P->beta = c.scaFac;
     P->kT = c.Temp;
     P->force_split = ext.force_split;
+    P->wall_split = ext.wall_split; P->wall_u_lo = ext.wall_u_lo; P->wall_u_hi = ext.wall_u_hi;
     for (int d = 0; d < 3; ++d) {
         P->per[d] = ext.periodic[d];
         P->off[d] = ext.periodic[d] ? 0.0f : c.CutRadius;

@@ -199,6 +199,17 @@ __device__ __forceinline__ bool moves(const GcParams& P, int id, int mmove)
     return id >= P.nf && (mmove || id < P.nc);
 }
 
+// Moving Bound2 walls (plane Couette shear) translate along x at their
+// wall velocity; with wmove = 0 (no wall of any suspension moves) they stay.
+__device__ __forceinline__ bool wall_moves(const GcParams& P, int id, int wmove)
+{
+    return wmove && id >= P.nb2 && id < P.nf;
+}
+__device__ __forceinline__ float4 wall_v4(const GcParams& P, double y)
+{
+    return make_float4((float)y >= P.wall_split ? P.wall_u_hi : P.wall_u_lo, 0.f, 0.f, 0.f);
+}
+
 // [1,2,3] key of the advanced position, histogram rank.  Slots are in the
 // previous step's (cube, id) order and few particles change cube per step,
 // so consecutive lanes mostly share a key: each run of equal keys in a wave
@@ -206,7 +217,7 @@ __device__ __forceinline__ bool moves(const GcParams& P, int id, int mmove)
 // of the device-scope atomics of one per particle.
 template <bool ENS, int TBS>
 __global__ __launch_bounds__(TBS) void k2_integrate(int n, GcRep R0, const GcRep* __restrict__ reps,
-    const int* __restrict__ rep_of, int move, int mmove, const double* __restrict__ pos64,
+    const int* __restrict__ rep_of, int move, int mmove, int wmove, const double* __restrict__ pos64,
     const float4* __restrict__ vout, const tfloat3* __restrict__ force, const int* __restrict__ sp,
     int* __restrict__ key, int* __restrict__ rank, int* __restrict__ cnt)
 {
@@ -222,6 +233,9 @@ __global__ __launch_bounds__(TBS) void k2_integrate(int n, GcRep R0, const GcRep
         if (move && moves(P, id, mmove)) {
             int img[3] = {0, 0, 0};
             advance(R, P, vout[p], force[p], x, y, z, img);
+        } else if (move && wall_moves(P, id, wmove)) {
+            int img[3] = {0, 0, 0};
+            advance(R, P, wall_v4(P, y), tfloat3{0.f, 0.f, 0.f}, x, y, z, img);
         }
         int cx = cube_coord((float)x, R.bbox[0], P.off[0], R.invw[0], R.nc[0]);
         int cy = cube_coord((float)y, R.bbox[2], P.off[1], R.invw[1], R.nc[1]);
@@ -333,7 +347,7 @@ __global__ __launch_bounds__(BS) void k2_scatter(int n, const int* __restrict__ 
 // the inverse of SortPart).
 template <bool ENS>
 __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep* __restrict__ reps,
-    const int* __restrict__ rep_of, int move, int mmove, const int* __restrict__ tk,
+    const int* __restrict__ rep_of, int move, int mmove, int wmove, const int* __restrict__ tk,
     const int* __restrict__ tv, const int* __restrict__ to, const int* __restrict__ offs,
     int* __restrict__ curpos,
     const double* __restrict__ pos_old, double* __restrict__ pos_new,
@@ -355,7 +369,8 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     tint3 nt = nt_old[o];
     float4 v = vout[o];
     const bool mv = move && moves(R0.P, id, mmove);
-    with_rep<ENS>(R0, reps, rep_of, mv, id, [&](const GcRep& R) __attribute__((always_inline)) {
+    const bool wmv = move && wall_moves(R0.P, id, wmove);
+    with_rep<ENS>(R0, reps, rep_of, mv || wmv, id, [&](const GcRep& R) __attribute__((always_inline)) {
     if (mv) {
         const GcParams& P = R.P;
         const tfloat3 f = force[o];
@@ -366,9 +381,14 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
         velhalf[s] = tfloat3{v.x + h * f.x, v.y + h * f.y, v.z + h * f.z};
         v.x += l * f.x; v.y += l * f.y; v.z += l * f.z;
     } else {
-        // walls do not move: their half-step velocity is zero (independent of
-        // the exported wall velocity, so reading the state changes nothing)
+        // walls: half-step velocity zero (independent of the exported wall
+        // velocity, so reading the state changes nothing)
         velhalf[s] = id < R0.P.nf ? tfloat3{0.f, 0.f, 0.f} : tfloat3{v.x, v.y, v.z};
+        if (wmv) {   // a moving wall: the key pass's advance, the same bits
+            int img[3] = {0, 0, 0};
+            advance(R, R.P, wall_v4(R.P, y), tfloat3{0.f, 0.f, 0.f}, x, y, z, img);
+            nt.x += img[0]; nt.y += img[1]; nt.z += img[2];
+        }
     }
     });
     pos_new[s] = x; pos_new[n + s] = y; pos_new[2 * n + s] = z;
@@ -380,7 +400,9 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     key_new[s] = k;
     velrho[s] = v;
     if (id >= R0.P.nc) curpos[id] = s;
-    if (id >= R0.P.nf) {
+    if (!wincl) {
+        act[s] = s;   // moving walls: the force pass runs over every slot
+    } else if (id >= R0.P.nf) {
         // a cube holds its walls (the smallest ids) first, so the moving
         // particles before this one are s minus the walls of cubes <= k
         const int ai = s - wincl[k];
@@ -547,9 +569,9 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
         velrho[p].w = P.m * P.wnorm * (d + 1.0f);
         nlc[p] = cnt <= NL_CAP ? cnt : -1;
     } else if (act && id < P.nf) {
-        float s = sw > 0.f ? -1.0f / sw : 0.0f;
+        const float3 u = wall_velocity(P, pi.y, sw, svx, svy, svz);
         float4 v = velrho[p];
-        v.x = svx * s; v.y = svy * s; v.z = svz * s;
+        v.x = u.x; v.y = u.y; v.z = u.z;
         velrho[p] = v;
     }
     });
@@ -573,6 +595,7 @@ __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep*
     const int p = act[t];
     float4 pi = posid[p];
     int id = __float_as_int(pi.w);
+    if (id < R0.P.nf) return;   // a wall slot (only with moving walls): no force
     with_rep<ENS>(R0, reps, rep_of, true, id, [&](const GcRep& R) __attribute__((always_inline)) {
     const GcParams& P = R.P;   // a reference: a copy of the batch record would live in scratch
     const unsigned long long c = *ctr - 1ull;   // advanced by this step's offsets pass
@@ -763,8 +786,8 @@ __global__ __launch_bounds__(BS) void k2_wallvel(int n, GcRep R0, const GcRep* _
             }
         }
     });
-    float s = sw > 0.f ? -1.0f / sw : 0.0f;
-    vout[p] = make_float4(svx * s, svy * s, svz * s, P.rho0);
+    const float3 u = wall_velocity(P, pi.y, sw, svx, svy, svz);
+    vout[p] = make_float4(u.x, u.y, u.z, P.rho0);
 }
 
 
@@ -822,6 +845,7 @@ int launch_build_t(gc_engine* e, int move, Prof* pr = nullptr)
     {
         auto L = [&](auto kern, int tbs) {
             kern<<<xcd_grid(blocks_for(N, tbs)), tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, mfree,
+                e->moving_walls,
                 e->d_pos64[c], e->d_vout, e->d_force, e->d_sp[c], e->d_key0, e->d_rank, e->d_cnt);
         };
         switch (e->tune.v[TUNE_INTEGRATE]) {
@@ -839,10 +863,11 @@ int launch_build_t(gc_engine* e, int move, Prof* pr = nullptr)
     k2_scatter<<<gb, BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[c], e->d_offs, e->d_tk, e->d_tv,
                                  e->d_to, e->d_cnt);
     GC_LAUNCH_CHECK("engine scatter");
-    k2_sortgather<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, mfree, e->d_tk, e->d_tv,
+    k2_sortgather<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, mfree, e->moving_walls,
+        e->d_tk, e->d_tv,
         e->d_to, e->d_offs, e->d_curpos, e->d_pos64[c], e->d_pos64[nx], e->d_vout, e->d_force, e->d_nt[c],
-        e->d_nt[nx], e->d_sp[nx], e->d_key, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf, e->d_wincl,
-        e->d_act);
+        e->d_nt[nx], e->d_sp[nx], e->d_key, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf,
+        e->moving_walls ? nullptr : e->d_wincl, e->d_act);
     GC_LAUNCH_CHECK("engine sort/gather");
     e->cur = nx;
     return GC_OK;
@@ -1062,7 +1087,8 @@ int wall_tables(gc_engine* e)
             ++nwall;
         }
     for (int k = 1; k < K; ++k) w[k] += w[k - 1];
-    e->nmov = N - nwall;
+    // moving walls change cubes: the force pass then runs over every slot
+    e->nmov = e->moving_walls ? N : N - nwall;
     if ((st = upload(e->d_wincl, w.data(), (size_t)K, e->s))) return st;
     return hip_check(hipStreamSynchronize(e->s), "wall table sync");   // w is freed on return
 }
@@ -1157,6 +1183,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
         cg.nCellStart = e->consts.nCellStart; cg.nCellPar = e->consts.nCellPar;
         cg.nTotalPar = e->consts.nTotalPar;
         make_params(cg, d.ext, &R.P);
+        if (d.ext.wall_u_lo != 0.0f || d.ext.wall_u_hi != 0.0f) e->moving_walls = 1;
         int Kr = 0;
         float outer[3];
         if ((st = gc_ConfigDomain(d.bbox, d.consts.CutRadius, d.ext.periodic, R.nc, outer, R.invw, &Kr)))

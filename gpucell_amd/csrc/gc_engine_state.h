@@ -72,6 +72,7 @@ struct gc_engine {
     // i-th moving particle in slot order (written by the gather), nmov of them
     int *d_wincl = nullptr, *d_act = nullptr;
     int nmov = 0;
+    int moving_walls = 0;   // some suspension's Bound2 walls translate (Couette shear)
     int* d_tsum = nullptr;   // cube counts per scan tile
     int ntile = 0;
     int *d_nl, *d_nlc;   // neighbour lists of moving particles (density -> force)

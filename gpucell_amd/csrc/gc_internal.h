@@ -41,6 +41,7 @@ struct GcParams {
     float krep;                 // Bound1 repulsion coefficient
     float De, mcut2, r0, beta, kT;
     float force_split;
+    float wall_split, wall_u_lo, wall_u_hi;   // moving Bound2 walls (x velocity below / above the split)
     float off[3];               // key padding offset per axis
     int per[3];
     int force_mode;
@@ -63,6 +64,21 @@ __device__ __forceinline__ float lucy_w(float r, const GcParams& P)
     float q = r * (1.0f / P.h);
     float om = 1.0f - q;
     return q < 1.0f ? P.wnorm * (1.0f + 3.0f * q) * om * om * om : 0.0f;
+}
+
+// Velocity of a Bound2 wall particle at height y from the Lucy-weighted
+// sums of its moving neighbours' velocities (sw, svx, svy, svz): the no-slip
+// image 2 u_w - sum(W v) / sum(W) about the wall's own velocity u_w (along
+// x: wall_u_lo below wall_split, wall_u_hi above); u_w with no moving
+// neighbour.  For walls at rest this is -sum(W v) / sum(W).
+__device__ __forceinline__ float3 wall_velocity(const GcParams& P, float y, float sw, float svx, float svy,
+                                                float svz)
+{
+    const float uw = y >= P.wall_split ? P.wall_u_hi : P.wall_u_lo;
+    const float s = sw > 0.f ? -1.0f / sw : 0.0f;
+    float3 v = make_float3(svx * s, svy * s, svz * s);
+    if (uw != 0.0f) v.x += sw > 0.f ? 2.0f * uw : uw;
+    return v;
 }
 
 // F(r) = -W'(r)/r

@@ -408,8 +408,6 @@ extern "C" int gc_EngineWriteTecplot(gc_engine* e, const char* dir_c, long stepd
         if ((st = gc_EngineGetTriangles(e, nor.data(), cen.data()))) return st;
     }
     // id ranges from the engine's parameter block
-    gc_ext_consts ext;
-    (void)ext;
     const int nCellStart = N - M;
     int nB1 = 0, nB12 = 0;
     if ((st = gc_EngineRanges(e, &nB1, &nB12))) return st;

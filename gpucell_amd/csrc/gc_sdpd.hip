@@ -125,9 +125,9 @@ __global__ __launch_bounds__(BS) void k_bound2_vel(int n, GcParams P, const tflo
             }
         }
     });
-    float s = sw > 0.f ? -1.0f / sw : 0.0f;
+    const float3 u = wall_velocity(P, xi.y, sw, svx, svy, svz);
     tfloat4 v = velrhop[p];
-    v.x = svx * s; v.y = svy * s; v.z = svz * s;
+    v.x = u.x; v.y = u.y; v.z = u.z;
     velrhop[p] = v;
 }
 

@@ -85,7 +85,9 @@ typedef struct {
     float wlc_ratio;        /* l_max / l0 of the WLC springs (x0 = 1/ratio) */
     float pow_index;        /* POW exponent m */
     float rep_scale;        /* Bound1 repulsion: multiple of c1^2 rho0 */
-    int pad0;
+    float wall_split;       /* moving walls (plane Couette shear): Bound2 walls with */
+    float wall_u_lo;        /* y < wall_split move along +x at wall_u_lo, the others */
+    float wall_u_hi;        /* at wall_u_hi (positions stay fixed; 0 = at rest)     */
     uint64_t noise_counter; /* counter of the next SDPD force evaluation */
 } gc_ext_consts;
 
@@ -113,6 +115,7 @@ GC_STATIC_ASSERT(offsetof(Structconst, nCellPar) == 16, "Structconst.nCellPar");
 GC_STATIC_ASSERT(offsetof(Structconst, iSeed) == 24, "Structconst.iSeed");
 GC_STATIC_ASSERT(offsetof(Structconst, CutRadius) == 32, "Structconst.CutRadius");
 GC_STATIC_ASSERT(offsetof(Structconst, Temp) == 88, "Structconst.Temp");
-GC_STATIC_ASSERT(sizeof(gc_ext_consts) == 48, "gc_ext_consts layout");
+GC_STATIC_ASSERT(sizeof(gc_ext_consts) == 56, "gc_ext_consts layout");
+GC_STATIC_ASSERT(offsetof(gc_ext_consts, noise_counter) == 48, "gc_ext_consts.noise_counter");
 
 #endif /* GPUCELL_TYPES_H */

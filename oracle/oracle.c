@@ -158,6 +158,7 @@ struct orc_sys {
     double lo[3], hi[3], L[3];
     int per[3];
     int force_mode; double force_split;
+    double wall_split, wall_u_lo, wall_u_hi;
     int noise_on;
     double wlc_ratio, pow_index, rep_scale;
     /* membrane topology (0-based) */
@@ -207,6 +208,7 @@ orc_sys* orc_create(const Structconst* c, const gc_ext_consts* ext,
         s->per[d] = ext->periodic[d];
     }
     s->force_mode = ext->force_mode; s->force_split = ext->force_split;
+    s->wall_split = ext->wall_split; s->wall_u_lo = ext->wall_u_lo; s->wall_u_hi = ext->wall_u_hi;
     s->noise_on = ext->noise_on;
     s->wlc_ratio = ext->wlc_ratio; s->pow_index = ext->pow_index;
     s->rep_scale = ext->rep_scale;
@@ -381,6 +383,14 @@ static void build_nlist(orc_sys* s, const double* pos)
 /* ------------------------------------------------------------------ */
 /* Integrator (Groot & Warren 1997, JCP 107:4423) mapped onto
  * UpdateFluidCoor / PredictFluidVel / CorrectFliudVel (ParticleGPU.cuh:11,14,21). */
+/* Moving Bound2 walls (plane Couette shear, an extension: the reference's
+ * walls are frozen) translate along x at their wall velocity. */
+static int walls_move(const orc_sys* s) { return s->wall_u_lo != 0.0 || s->wall_u_hi != 0.0; }
+static double wall_u(const orc_sys* s, double y)
+{
+    return (float)y >= (float)s->wall_split ? s->wall_u_hi : s->wall_u_lo;
+}
+
 void orc_update_coor(const orc_sys* s, double* pos, const double* vel, const double* force)
 {
     const double dt = s->dt, a = 0.5 * dt * dt / s->m;
@@ -388,13 +398,16 @@ void orc_update_coor(const orc_sys* s, double* pos, const double* vel, const dou
     for (int i = s->nf; i < s->N; ++i)
         for (int d = 0; d < 3; ++d)
             pos[3 * i + d] += dt * vel[3 * i + d] + a * force[3 * i + d];
+    if (walls_move(s))
+        for (int i = s->nb2; i < s->nf; ++i) pos[3 * i] += dt * wall_u(s, pos[3 * i + 1]);
 }
 
 /* AppFluidPeriodicBC (ParticleGPU.cuh:12): one wrap per step and axis. */
 void orc_periodic_bc(const orc_sys* s, double* pos, int* img)
 {
+    const int first = walls_move(s) ? s->nb2 : s->nf;
 #pragma omp parallel for
-    for (int i = s->nf; i < s->N; ++i)
+    for (int i = first; i < s->N; ++i)
         for (int d = 0; d < 3; ++d) {
             if (!s->per[d]) continue;
             double x = pos[3 * i + d];
@@ -450,8 +463,8 @@ void orc_density(orc_sys* s, const double* pos, double* rho)
 }
 
 /* Bound2 velocity (CalBound2Vel, ParticleGPU.cuh:15): no-slip image
- * v_w = -sum W v_j / sum W over moving neighbours (Adami, Hu & Adams 2012
- * style, wall at rest). */
+ * v_w = 2 u_w - sum W v_j / sum W over moving neighbours (Adami, Hu & Adams
+ * 2012 style), u_w the wall's prescribed x velocity (0: wall at rest). */
 static void bound2_vel_nl(orc_sys* s, const double* pos, double* vel)
 {
 #pragma omp parallel for schedule(dynamic, 256)
@@ -465,7 +478,11 @@ static void bound2_vel_nl(orc_sys* s, const double* pos, double* vel)
             sw += w;
             for (int d = 0; d < 3; ++d) sv[d] += w * vel[3 * j + d];
         }
+        /* no-slip image about the wall's own velocity u_w (x only):
+         * 2 u_w - sum(W v) / sum(W); u_w with no moving neighbour */
+        const double uw = wall_u(s, pos[3 * i + 1]);
         for (int d = 0; d < 3; ++d) vel[3 * i + d] = sw > 0 ? -sv[d] / sw : 0.0;
+        vel[3 * i] += sw > 0 ? 2.0 * uw : uw;
     }
 }
 void orc_bound2_vel(orc_sys* s, const double* pos, double* vel)

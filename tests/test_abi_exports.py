@@ -38,7 +38,8 @@ def test_struct_layouts_match_header():
     assert abi.Structconst.iSeed.offset == 24
     assert abi.Structconst.CutRadius.offset == 32
     assert abi.Structconst.Temp.offset == 88
-    assert ctypes.sizeof(abi.gc_ext_consts) == 48
+    assert ctypes.sizeof(abi.gc_ext_consts) == 56
+    assert abi.gc_ext_consts.noise_counter.offset == 48
     assert ctypes.sizeof(abi.int6) == 24 and ctypes.sizeof(abi.tfloat4) == 16
 
 

@@ -43,8 +43,15 @@ def big_cell_case():
     return casegen.periodic_box(n=20, rbc_diameter=11.0, subdiv=4, seed=12)
 
 
+def couette_case():
+    """Plane Couette shear between moving Bound2 walls, one cell."""
+    ph = util.replace_physics(kT=0.01)
+    return casegen.couette(n=16, n_rbc=1, rbc_diameter=7.0, shear_rate=0.01, seed=13, physics=ph)
+
+
 CASES = {"channel": lambda: channel_case(), "channel_morse": lambda: channel_case(De=0.02),
-         "box": box_case, "channel_bound1": bound1_case, "box_big_cell": big_cell_case}
+         "box": box_case, "channel_bound1": bound1_case, "box_big_cell": big_cell_case,
+         "couette": couette_case}
 
 
 @pytest.mark.parametrize("name", ["channel", "box"])
@@ -79,7 +86,7 @@ def _oracle_state(o, case, S, info):
     return st
 
 
-@pytest.mark.parametrize("name", ["channel", "channel_morse", "box", "channel_bound1", "box_big_cell"])
+@pytest.mark.parametrize("name", ["channel", "channel_morse", "box", "channel_bound1", "box_big_cell", "couette"])
 def test_engine_step_parity(gpu, name):
     case = CASES[name]()
     e = Engine(case)
@@ -211,3 +218,38 @@ def test_checkpoint_resume_is_bitwise(gpu, tmp_path):
     other = Engine(box_case())
     with pytest.raises(Exception, match="different case"):
         other.load(ck)
+
+
+def test_couette_walls_drive_shear_flow(gpu):
+    """Walls translating at -U and +U (no body force; thermal noise on, which
+    keeps the particle fluid from shearing like a lattice) drive plane
+    Couette flow.  Time-averaged over the last 2500 of 10000 steps: the
+    profile of the mean x velocity rises monotonically from -U to +U, the
+    fluid layers next to the walls move with them (no slip, within 15 %),
+    there is no net flow, and the profile's slope is 0.7-1.2 x 2U/H.  (The
+    first fluid layers by the particle walls shear faster than the bulk: the
+    bulk alone measures ~0.6 x 2U/H at this 12-layer height.)"""
+    rate, n = 0.02, 12
+    U = 0.5 * rate * n
+    case = casegen.couette(n=n, shear_rate=rate, noise=True, seed=3)
+    e = Engine(case)
+    nf = case.consts.nFluidStart
+    e.step(7500, graph=True)
+    acc = np.zeros(n)
+    mean_vx = 0.0
+    for _ in range(25):
+        e.step(100, graph=True)
+        S = e.state()
+        y = S["pos"][nf:, 1]
+        v = S["velrhop"][nf:, :3].astype(np.float64)
+        b = np.floor(y).astype(int)
+        acc += np.array([v[b == k, 0].mean() for k in range(n)]) / 25
+        mean_vx += v[:, 0].mean() / 25
+    yc = np.arange(n) + 0.5 - 0.5 * n
+    slope = np.polyfit(yc, acc, 1)[0]
+    print(f"Couette U={U}: profile {np.round(acc, 4).tolist()}, slope {slope:.4f} (2U/H {rate}), "
+          f"mean {mean_vx:.2e}")
+    assert np.all(np.diff(acc) > 0)
+    assert abs(acc[0] + U) < 0.15 * U and abs(acc[-1] - U) < 0.15 * U
+    assert abs(mean_vx) < 0.1 * U
+    assert 0.7 * rate < slope < 1.2 * rate

@@ -216,3 +216,31 @@ def test_constant_force_parabola_and_wrap():
     assert st["img"][0][0] >= 1
     steps = np.diff(np.array(unwrapped), axis=0)
     assert np.abs(steps).max() < 0.1
+
+
+def test_moving_wall_velocity_is_no_slip_image():
+    """Couette walls: a Bound2 particle's velocity is 2 u_w - sum(W v)/sum(W)
+    over its moving neighbours, u_w = -U below the mid-plane and +U above;
+    with the fluid in uniform motion v0 that is exactly 2 u_w - v0, and a
+    wall particle out of reach of the fluid (third layer) takes u_w."""
+    case = casegen.couette(n=8, shear_rate=0.02, noise=False)
+    o = orc.Oracle(case)
+    N, nb2, nf = case.N, case.consts.nBound2Start, case.consts.nFluidStart
+    pos = case.coor.astype(np.float64)
+    v0 = np.array([0.03, -0.01, 0.02])
+    vel = np.zeros((N, 3))
+    vel[nf:] = v0
+    o.bound2_vel(pos, vel)
+    U = 0.5 * 0.02 * 8.0
+    walls = np.arange(nb2, nf)
+    uw = np.where(pos[walls, 1] >= 4.0, U, -U)
+    # walls with a fluid particle within h (brute force, periodic in x and z)
+    d = pos[walls][:, None, :] - pos[nf:][None, :, :]
+    d[..., [0, 2]] -= 8.0 * np.round(d[..., [0, 2]] / 8.0)
+    reach = (np.linalg.norm(d, axis=2) < case.consts.CutRadius).any(axis=1)
+    exp = np.zeros((walls.size, 3))
+    exp[reach] = -v0
+    exp[reach, 0] += 2 * uw[reach]
+    exp[~reach, 0] = uw[~reach]
+    assert reach.any() and (~reach).any()
+    assert np.allclose(vel[walls], exp, atol=1e-12)
