@@ -67,9 +67,10 @@ def make_case(name):
     if name == "box32":
         return casegen.periodic_box(), "periodic 32^3 box, one RBC, reverse Poiseuille"
     if name == "ensemble64":
-        return casegen.ensemble64(), ("ensemble64: 64 independent periodic suspensions, "
-                                      "22k-390k particles (log-uniform), 1-40 RBCs each, per-replica "
-                                      "body force, Morse De and seed, one batched engine")
+        return casegen.ensemble64(), ("ensemble64: 64 independent sheared suspensions (plane Couette "
+                                      "cells, x and z periodic, walls moving at -U/+U), 27k-406k "
+                                      "particles (log-uniform), 1-40 RBCs each, per-replica shear rate "
+                                      "(0.002-0.01), Morse De and seed, one batched engine")
     raise SystemExit(f"unknown case {name}")
 
 

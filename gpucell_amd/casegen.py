@@ -570,16 +570,18 @@ def suspension_10m(seed: int = 20261203, physics: Physics | None = None, noise: 
 
 
 def ensemble64(seed: int = 20261203, nrep: int = 64, nmin: int = 20000, nmax: int = 400000,
-               rbc_max: int = 40, subdiv: int | None = None) -> "list[Case]":
-    """Independent periodic suspensions of uneven size for a batched run:
-    particle counts log-uniform in [nmin, nmax] (cubic boxes, unit lattice),
-    1..rbc_max RBCs each (log-uniform), about 10 % hematocrit, and per
-    replica a reverse-Poiseuille body force, Morse D_e (0 or 0.3 kT) and
-    noise seed drawn from `seed`.  The membrane mesh is refined with the
-    cell size so that vertices stay about one lattice spacing apart (a
-    denser membrane than fluid drives the SDPD pressure unstable): 642
-    vertices from D = 16, 162 from D = 8; D >= 8 caps the cell count of
-    small boxes at 20 % hematocrit."""
+               rbc_max: int = 40, subdiv: int | None = None, shear: bool = True) -> "list[Case]":
+    """Independent suspensions of uneven size for a batched run: particle
+    counts log-uniform in [nmin, nmax] (cubic cells, unit lattice), 1..rbc_max
+    RBCs each (log-uniform), about 10 % hematocrit, and per replica a shear
+    rate (log-uniform in [0.002, 0.01]: plane Couette cells between walls
+    moving at -U and +U, wall Mach number <= 0.1), Morse D_e (0 or 0.3 kT)
+    and noise seed drawn from `seed`; with shear=False fully periodic boxes
+    driven by a reverse-Poiseuille body force instead.  The membrane mesh is
+    refined with the cell size so that vertices stay about one lattice
+    spacing apart (a denser membrane than fluid drives the SDPD pressure
+    unstable): 642 vertices from D = 16, 162 from D = 8; D >= 8 caps the cell
+    count of small boxes at 20 % hematocrit."""
     rng = np.random.default_rng(seed)
     base = Physics()
     cases = []
@@ -592,11 +594,17 @@ def ensemble64(seed: int = 20261203, nrep: int = 64, nmin: int = 20000, nmax: in
         d = min(0.45 * n, max(8.0, (0.10 * n ** 3 / (n_rbc * vol)) ** (1.0 / 3.0)))
         sub = 3 if d >= 16.0 else 2
         De = float(rng.choice([0.0, 0.3])) * base.kT
-        ph = dataclasses.replace(base, fext=float(rng.uniform(0.01, 0.05)), De=De, r0=0.3,
-                                 beta=1.5, mcut=min(1.5, base.h),
+        fext = float(rng.uniform(0.01, 0.05))
+        rate = float(np.exp(rng.uniform(np.log(0.002), np.log(0.01))))
+        ph = dataclasses.replace(base, fext=fext, De=De, r0=0.3, beta=1.5, mcut=min(1.5, base.h),
                                  seed=int(rng.integers(1, 2 ** 31 - 1)))
-        c = periodic_box(n=n, n_rbc=n_rbc, rbc_diameter=d, seed=int(rng.integers(1, 2 ** 31 - 1)),
-                         physics=ph, subdiv=sub if subdiv is None else subdiv)
+        cseed = int(rng.integers(1, 2 ** 31 - 1))
+        if shear:
+            c = couette(n=n, n_rbc=n_rbc, rbc_diameter=d, shear_rate=rate, seed=cseed, physics=ph,
+                        subdiv=sub if subdiv is None else subdiv)
+        else:
+            c = periodic_box(n=n, n_rbc=n_rbc, rbc_diameter=d, seed=cseed, physics=ph,
+                             subdiv=sub if subdiv is None else subdiv)
         c.name = f"ensemble_replica_{r}"
         cases.append(c)
     return cases

@@ -24,6 +24,8 @@ def _cases():
         casegen.channel(30, 14, 3, rbc_diameter=5.0, subdiv=2, seed=7, physics=ph_b),
         casegen.periodic_box(n=14, rbc_diameter=6.0, subdiv=2, seed=11, physics=ph_c),
         casegen.periodic_box(n=10, n_rbc=0, seed=2),   # fluid only
+        casegen.couette(n=12, n_rbc=1, rbc_diameter=5.0, shear_rate=0.01, seed=4,
+                        physics=util.replace_physics(kT=0.01, De=0.02, r0=0.3, beta=1.5, mcut=1.5)),
     ]
 
 
