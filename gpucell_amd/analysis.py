@@ -1,6 +1,6 @@
 """Analysis of engine states: the quantities the reference writes per cell
 (Particle::OutPutCellGeo / OutPutCellForce, src/Particle.cu:1010-1246) and
-the flow diagnostics used to judge thermal runs.
+the flow diagnostics used to check thermal runs.
 
 * deformation_index: Taylor D = (L - B)/(L + B) of the ellipsoid with the
   same volume second moments as the closed membrane (exact for a closed

@@ -1,7 +1,7 @@
 """The learned membrane substepper end to end on the GPU: rows recorded
 from plain substepping, the network trained by the library's own
 BF16-MFMA kernels, and the surrogate run against plain substepping (the
-judge) on a held-out suspension.  What the trained network gains over the
+reference for its error) on a held-out suspension.  What the trained network gains over the
 method without it: its predictions pass the residual check, so steps need
 no plain substeps, while an untrained network is rejected on every step."""
 import numpy as np
