@@ -431,33 +431,43 @@ __device__ __forceinline__ TriPar tri_par(const DefArgs& a, int t)
 }
 
 // Forces of an edge on its vertices A, B, C, D from B, C, D relative to A.
+// Quotients and roots below use the hardware reciprocal, reciprocal square
+// root, base-2 logarithm and exponential (v_rcp/v_rsq/v_log/v_exp, ~1 ulp):
+// the terms stay well inside the 1e-5 force tolerance at a fraction of the
+// instructions of IEEE division and the library functions.
+__device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float rsq_hw(float x) { return __builtin_amdgcn_rsqf(x); }
+
 __device__ __forceinline__ void edge_terms(const GcParams& P, const EdgePar& q, float3 pB, float3 pC, float3 pD,
                                            float3 out[4])
 {
     // WLC-POW spring along d = A - B: s d on A, -s d on B
     const float3 d = make_float3(-pB.x, -pB.y, -pB.z);
-    const float ls = sqrtf(f3dot(d, d));
-    const float x = ls / q.lmax;
-    const float om = 1.0f - x;
-    // kp / ls^m with the hardware base-2 logarithm and exponential (~1 ulp each)
-    const float fm = -(P.kT / q.p) * (0.25f / (om * om) - 0.25f + x) +
+    const float dd = f3dot(d, d);
+    const float rls = rsq_hw(dd);
+    const float ls = dd * rls;
+    const float x = ls * rcp_hw(q.lmax);
+    const float iom = rcp_hw(1.0f - x);
+    // kp / ls^m = kp 2^(-m log2 ls)
+    const float fm = -(P.kT * rcp_hw(q.p)) * (0.25f * iom * iom - 0.25f + x) +
                      q.kp * __builtin_amdgcn_exp2f(-q.mexp * __builtin_amdgcn_logf(ls));
-    const float s = fm / ls;
+    const float s = fm * rls;
     // bending: dihedral of the wings C (n1 = e x c) and D (n2 = d x e)
     const float3 e3 = pB, ac = pC, ad = pD;
     const float3 n1 = f3cross(e3, ac), n2 = f3cross(ad, e3);
-    const float l2 = f3dot(e3, e3), l = sqrtf(l2);
+    const float l2 = f3dot(e3, e3), rl = rsq_hw(l2), l = l2 * rl;
     const float q1 = f3dot(n1, n1), q2 = f3dot(n2, n2);
-    const float im1 = rsqrtf(q1), im2 = rsqrtf(q2);
+    const float im1 = rsq_hw(q1), im2 = rsq_hw(q2);
     const float3 u1 = f3scale(n1, im1), u2 = f3scale(n2, im2);
     // dihedral angle th from (cos th, sin th) = (u1.u2, (u1 x u2).e3 / l) / r,
     // sin(th - th0) = sin th cos th0 - cos th sin th0
     const float3 xx = f3cross(u1, u2);
-    const float sy = f3dot(xx, e3) / l, cx = f3dot(u1, u2);
-    const float ir = rsqrtf(cx * cx + sy * sy);
+    const float sy = f3dot(xx, e3) * rl, cx = f3dot(u1, u2);
+    const float ir = rsq_hw(cx * cx + sy * sy);
     const float cb = -q.kb * ((sy * q.cth0 - cx * q.sth0) * ir);
-    const float3 gC = f3scale(n1, -l / q1), gD = f3scale(n2, -l / q2);
-    const float tC = f3dot(ac, e3) / l2, tD = f3dot(ad, e3) / l2;
+    const float3 gC = f3scale(n1, -l * (im1 * im1)), gD = f3scale(n2, -l * (im2 * im2));
+    const float rl2 = rl * rl;
+    const float tC = f3dot(ac, e3) * rl2, tD = f3dot(ad, e3) * rl2;
     const float wCA = -(1.0f - tC), wDA = -(1.0f - tD), wCB = -tC, wDB = -tD;
     out[0] = make_float3(s * d.x + cb * (wCA * gC.x + wDA * gD.x), s * d.y + cb * (wCA * gC.y + wDA * gD.y),
                          s * d.z + cb * (wCA * gC.z + wDA * gD.z));
@@ -473,11 +483,12 @@ __device__ __forceinline__ void tri_terms(const TriPar& q, float3 P1, float3 P2,
 {
     const float3 P0 = make_float3(0.f, 0.f, 0.f);
     const float3 n = f3cross(P1, P2);
-    const float nn = sqrtf(f3dot(n, n));
-    const float At = 0.5f * nn;
-    const float3 un = f3scale(n, 1.0f / nn);
-    const float cA = q.ka * (Ac - q.A0c) / q.A0c + q.kd * (At - q.ref) / q.ref;
-    const float cV = q.kv * (Vc - q.V0c) / q.V0c;
+    const float n2 = f3dot(n, n);
+    const float inn = rsq_hw(n2);
+    const float At = 0.5f * (n2 * inn);
+    const float3 un = f3scale(n, inn);
+    const float cA = q.ka * (Ac - q.A0c) * rcp_hw(q.A0c) + q.kd * (At - q.ref) * rcp_hw(q.ref);
+    const float cV = q.kv * (Vc - q.V0c) * rcp_hw(q.V0c);
     const float hv = cV * (1.0f / 6.0f);
     const float3 cn[3] = {P1, P2, P0};   // next corner of corner k
     const float3 cnn[3] = {P2, P0, P1};  // next-next corner
