@@ -183,9 +183,9 @@ __global__ __launch_bounds__(GC_GEO_BS) void k_geo_cell(GcParams P, long stepcou
         float3 p1 = urel(coor[s1], nthro[s1], xo, no, b);
         float3 p2 = urel(coor[s2], nthro[s2], xo, no, b);
         float3 n = f3cross(f3sub(p1, p0), f3sub(p2, p0));
-        float nn = sqrtf(f3dot(n, n));
-        float at = 0.5f * nn;
-        float in = 1.0f / nn;
+        const float n2 = f3dot(n, n);
+        float in = __builtin_amdgcn_rsqf(n2);
+        float at = n2 > 0.0f ? 0.5f * (n2 * in) : 0.0f;   // area 0 for a degenerate triangle
         trinor[t] = tfloat3{n.x * in, n.y * in, n.z * in};
         triarea[t] = at;
         if (stepcount == -1) reftriarea[t] = at;
@@ -193,9 +193,10 @@ __global__ __launch_bounds__(GC_GEO_BS) void k_geo_cell(GcParams P, long stepcou
         double ox = (double)xo.x + (b.per[0] ? (double)no.x * b.L[0] : 0.0);
         double oy = (double)xo.y + (b.per[1] ? (double)no.y * b.L[1] : 0.0);
         double oz = (double)xo.z + (b.per[2] ? (double)no.z * b.L[2] : 0.0);
-        tricen[t] = tfloat3{(float)(ox + ((double)p0.x + p1.x + p2.x) / 3.0),
-                            (float)(oy + ((double)p0.y + p1.y + p2.y) / 3.0),
-                            (float)(oz + ((double)p0.z + p1.z + p2.z) / 3.0)};
+        constexpr double third = 1.0 / 3.0;
+        tricen[t] = tfloat3{(float)(ox + ((double)p0.x + p1.x + p2.x) * third),
+                            (float)(oy + ((double)p0.y + p1.y + p2.y) * third),
+                            (float)(oz + ((double)p0.z + p1.z + p2.z) * third)};
         A += (double)at;
         // signed tetrahedron (O, p0, p1, p2) in FP64: six times its volume
         double ax = p0.x, ay = p0.y, az = p0.z, bx = p1.x, by = p1.y, bz = p1.z;
@@ -654,16 +655,17 @@ __device__ __forceinline__ void cell_force(const GcParams& P, const CellArgs& ca
             const float3 p0 = rel_geo(tr.x - 1 - v0, o), p1 = rel_geo(tr.y - 1 - v0, o),
                          p2 = rel_geo(tr.z - 1 - v0, o);
             const float3 n = f3cross(f3sub(p1, p0), f3sub(p2, p0));
-            const float nn = sqrtf(f3dot(n, n));
-            const float at = 0.5f * nn;
+            const float n2 = f3dot(n, n);
+            const float in = __builtin_amdgcn_rsqf(n2);
+            const float at = n2 > 0.0f ? 0.5f * (n2 * in) : 0.0f;
             if (geo) {
-                const float in = 1.0f / nn;
                 ca.trinor[t] = tfloat3{n.x * in, n.y * in, n.z * in};
                 ca.triarea[t] = at;
                 if (ca.step == -1) ca.reftriarea[t] = at;
-                ca.tricen[t] = tfloat3{(float)(ox + ((double)p0.x + p1.x + p2.x) / 3.0),
-                                       (float)(oy + ((double)p0.y + p1.y + p2.y) / 3.0),
-                                       (float)(oz + ((double)p0.z + p1.z + p2.z) / 3.0)};
+                constexpr double third = 1.0 / 3.0;
+                ca.tricen[t] = tfloat3{(float)(ox + ((double)p0.x + p1.x + p2.x) * third),
+                                       (float)(oy + ((double)p0.y + p1.y + p2.y) * third),
+                                       (float)(oz + ((double)p0.z + p1.z + p2.z) * third)};
             }
             A += (double)at;
             double ax = p0.x, ay = p0.y, az = p0.z, bx = p1.x, by = p1.y, bz = p1.z;
