@@ -12,7 +12,7 @@
 
 namespace gc {
 
-static std::string g_err = "ok";
+static thread_local std::string g_err = "ok";   // per host thread
 static hipStream_t g_stream = nullptr;
 static Structconst g_consts;
 static bool g_have_consts = false;

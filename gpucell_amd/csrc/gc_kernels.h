@@ -154,8 +154,11 @@ __device__ __forceinline__ double ms_drift(double x, float dts, float u) { retur
 // positions after substeps k-2 and k-1.
 struct SubArgs {
     int k; float dts, inv_m;
-    double* mx; tfloat3* mv; const tfloat3* fe; tfloat3* fm; tfloat3* mcoor;
-    double* mx6; double* mx7;
+    const double* mx_in; const tfloat3* mv_in; const tfloat3* fm_in;   // start state (may be mx, mv, fm)
+    double* mx; tfloat3* mv; const tfloat3* fe; tfloat3* fm; tfloat3* mcoor;   // end state
+    // non-null: positions after substeps k-2 and k-1 into mx6 / mx7 and
+    // d2x = (x_k - x_{k-1}) - (x_{k-1} - x_{k-2}) (FP64, then FP32)
+    double* mx6; double* mx7; tfloat3* d2x;
 };
 // all k substeps of each cell in one workgroup (cells as k_cell_membrane;
 // ca.step must not be -1)

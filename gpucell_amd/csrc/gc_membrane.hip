@@ -852,8 +852,8 @@ __global__ __launch_bounds__(GC_CELL_BS) void k_cell_substeps(GcParams P, CellAr
     const bool own = tid < nv;
     float3 u = make_float3(0.f, 0.f, 0.f), fe = u, fm = u;
     if (own) {
-        sx[tid] = sa_.mx[g]; sy[tid] = sa_.mx[nm + g]; sz[tid] = sa_.mx[2 * nm + g];
-        const tfloat3 u0 = sa_.mv[g], a0 = sa_.fe[g], b0 = sa_.fm[g];
+        sx[tid] = sa_.mx_in[g]; sy[tid] = sa_.mx_in[nm + g]; sz[tid] = sa_.mx_in[2 * nm + g];
+        const tfloat3 u0 = sa_.mv_in[g], a0 = sa_.fe[g], b0 = sa_.fm_in[g];
         u = make_float3(u0.x, u0.y, u0.z);
         fe = make_float3(a0.x, a0.y, a0.z);
         fm = make_float3(b0.x, b0.y, b0.z);
@@ -889,6 +889,18 @@ __global__ __launch_bounds__(GC_CELL_BS) void k_cell_substeps(GcParams P, CellAr
     sa_.mcoor[g] = tfloat3{(float)x, (float)y, (float)z};
     sa_.mv[g] = tfloat3{u.x, u.y, u.z};
     sa_.fm[g] = tfloat3{fm.x, fm.y, fm.z};
+    if (sa_.d2x) {   // this thread's own earlier stores
+        const double* x6 = sa_.mx6;
+        const double* x7 = sa_.mx7;
+        const double x8[3] = {x, y, z};
+        float c[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const size_t i = (size_t)d * nm + g;
+            c[d] = (float)((x8[d] - x7[i]) - (x7[i] - x6[i]));
+        }
+        sa_.d2x[g] = tfloat3{c[0], c[1], c[2]};
+    }
 }
 
 __global__ void k_sincos(int n, const float* __restrict__ th, float* __restrict__ c, float* __restrict__ s)

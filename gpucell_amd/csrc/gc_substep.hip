@@ -67,12 +67,20 @@ __global__ void k_ms_init(int nm, int nc, int N, const int* __restrict__ curpos,
     mv[v] = tfloat3{q.x, q.y, q.z};
 }
 
+// the fluid force of each membrane vertex (slot order -> id order); with
+// mx0 also a copy of the start state (positions, velocity, membrane force)
 __global__ void k_ms_fext(int nm, int nc, const int* __restrict__ curpos, const tfloat3* __restrict__ force,
-                          tfloat3* __restrict__ fe)
+                          tfloat3* __restrict__ fe, const double* __restrict__ mx, const tfloat3* __restrict__ mv,
+                          const tfloat3* __restrict__ fm, double* __restrict__ mx0, tfloat3* __restrict__ mv0,
+                          tfloat3* __restrict__ fm0)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nm) return;
     fe[v] = force[curpos[nc + v]];
+    if (!mx0) return;
+    mx0[v] = mx[v]; mx0[nm + v] = mx[nm + v]; mx0[2 * nm + v] = mx[2 * nm + v];
+    mv0[v] = mv[v];
+    fm0[v] = fm[v];
 }
 
 __global__ void k_ms_kick_drift(int nm, float dts, float inv_m, double* __restrict__ mx, tfloat3* __restrict__ mv,
@@ -353,14 +361,22 @@ __global__ void k_ms_d2x(int nm, const double* __restrict__ x8, const double* __
     d2x[v] = tfloat3{c[0], c[1], c[2]};
 }
 
-// residual partial sums (block tree, fixed order); with kick, first the
-// surrogate's end kick v8 = v7.5 + dts/2 a8
+// The residual R = sqrt(mean_v |d2x_v - dts^2 a_v|^2 / l_v^2); with kick,
+// first the surrogate's end kick v8 = v7.5 + dts/2 a8.  Each block writes the
+// tree sum of its vertices to part[block]; the block that arrives last (a
+// device-scope counter, cnt, which it resets to 0) sums part[] in block
+// order (strided, then a tree: the same order whichever block is last) and
+// writes out[0] = R; with flag, flag[0] = fallback needed (R > thr or not a
+// number) and flag[1] counts the fallbacks.
 __global__ __launch_bounds__(BS) void k_ms_finish(int nm, int kick, float dts, float inv_m, tfloat3* __restrict__ mv,
     const tfloat3* __restrict__ fe, const tfloat3* __restrict__ fm, const tfloat3* __restrict__ d2x,
-    const float* __restrict__ lref, float* __restrict__ part)
+    const float* __restrict__ lref, float* part, unsigned* cnt, float* __restrict__ out, float thr,
+    int* __restrict__ flag)
 {
     __shared__ float sh[BS];
-    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ bool last;
+    const int tid = threadIdx.x;
+    int v = blockIdx.x * blockDim.x + tid;
     float q = 0.f;
     if (v < nm) {
         tfloat3 a = fe[v], b = fm[v], d = d2x[v];
@@ -375,33 +391,32 @@ __global__ __launch_bounds__(BS) void k_ms_finish(int nm, int kick, float dts, f
         float r = norm3(sub3(dd, e)) / lref[v];
         q = r * r;
     }
-    sh[threadIdx.x] = q;
-    __syncthreads();
-    for (int o = BS / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
-}
-
-// out[0] = rms residual; with flag: flag[0] = fallback needed (R > thr or
-// not a number), flag[1] counts the fallbacks.  One block: strided sums,
-// then a tree (fixed order).
-constexpr int RES_BS = 1024;
-__global__ __launch_bounds__(RES_BS) void k_ms_res_final(int nb, int nm, const float* __restrict__ part,
-                                                         float* __restrict__ out, float thr, int* __restrict__ flag)
-{
-    __shared__ float sh[RES_BS];
-    const int tid = threadIdx.x;
-    float q = 0.f;
-    for (int i = tid; i < nb; i += RES_BS) q += part[i];
     sh[tid] = q;
     __syncthreads();
-    for (int o = RES_BS / 2; o > 0; o >>= 1) {
+    for (int o = BS / 2; o > 0; o >>= 1) {
+        if (tid < o) sh[tid] += sh[tid + o];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        part[blockIdx.x] = sh[0];
+        __threadfence();   // the partial is visible before the arrival is counted
+        last = atomicAdd(cnt, 1u) == gridDim.x - 1;
+        __threadfence();
+    }
+    __syncthreads();
+    if (!last) return;   // block-uniform
+    __threadfence();     // every other block's partial is visible from here
+    const int nb = (int)gridDim.x;
+    q = 0.f;
+    for (int i = tid; i < nb; i += BS) q += part[i];
+    sh[tid] = q;
+    __syncthreads();
+    for (int o = BS / 2; o > 0; o >>= 1) {
         if (tid < o) sh[tid] += sh[tid + o];
         __syncthreads();
     }
     if (tid != 0) return;
+    *cnt = 0u;   // for the next launch (kernel boundaries order it)
     const float r = sqrtf(sh[0] / (float)nm);
     out[0] = r;
     if (flag) {
@@ -448,6 +463,7 @@ struct MembraneSub {
     int* ring; float* lref;
     float *frame, *feat, *pred, *norm = nullptr, *part, *res;
     int* flag = nullptr;   // [0] fallback this step, [1] fallbacks, [2] steps (device counters)
+    unsigned* arrive = nullptr;   // blocks of the residual pass done (its last block resets it)
     bool have_norm = false;
     float* rec_x = nullptr; float* rec_y = nullptr;
     long rec_cap = 0, rec_count = 0;
@@ -520,14 +536,6 @@ FeatArgs feat_args(gc_engine* e, MembraneSub* m, bool normalise)
     return a;
 }
 
-int save_start(gc_engine* e, MembraneSub* m)
-{
-    k_ms_copy_state<<<blocks_for(m->nm, BS), BS, 0, e->s>>>(m->nm, m->mx, m->mv, m->fm, m->mx0, m->mv0, m->fm0,
-                                                             nullptr);
-    GC_LAUNCH_CHECK("ms save");
-    return GC_OK;
-}
-
 // back to the saved start state (only when *gate != 0 if gate is given)
 int restore_start(gc_engine* e, MembraneSub* m, const int* gate = nullptr)
 {
@@ -537,25 +545,29 @@ int restore_start(gc_engine* e, MembraneSub* m, const int* gate = nullptr)
     return GC_OK;
 }
 
-// k substeps; with a gate (the surrogate's device-decided fallback) every
-// kernel runs only when *gate != 0 and the exact trajectory's residual is
-// not formed
-int plain_substeps(gc_engine* e, MembraneSub* m, bool record, long row0, const int* gate = nullptr)
+// k substeps, from the current state or (from_saved) from the saved start
+// state; with a gate (the surrogate's device-decided fallback) every kernel
+// runs only when *gate != 0 and the exact trajectory's residual is not formed
+int plain_substeps(gc_engine* e, MembraneSub* m, bool record, long row0, const int* gate, bool from_saved)
 {
     hipStream_t s = e->s;
     const int nb = blocks_for(m->nm, BS);
     const float dts = e->hreps[0].P.dt / (float)m->k, inv_m = e->hreps[0].P.inv_m;
     const bool keep = !gate && m->k == KSUB;   // positions after substeps 6 and 7 for the residual
+    const bool cell = use_cell_kernel(e) && !record && e->step != -1;
     int st;
-    if (use_cell_kernel(e) && !record && e->step != -1) {
-        // all substeps of a cell in one workgroup
+    if (cell) {
+        // all substeps of a cell in one workgroup, which also forms d2x
         CellArgs ca = cell_args(e, e->step, 1);
         ca.gate = gate;
-        SubArgs sa{m->k, dts, inv_m, m->mx, m->mv, m->fe, m->fm, m->mcoor,
-                   keep ? m->mx6 : nullptr, keep ? m->mx7 : nullptr};
+        SubArgs sa{m->k, dts, inv_m,
+                   from_saved ? m->mx0 : m->mx, from_saved ? m->mv0 : m->mv, from_saved ? m->fm0 : m->fm,
+                   m->mx, m->mv, m->fe, m->fm, m->mcoor,
+                   keep ? m->mx6 : nullptr, keep ? m->mx7 : nullptr, keep ? m->d2x : nullptr};
         k_cell_substeps<<<e->ncell, GC_CELL_BS, 0, s>>>(m->Pm, ca, ms_defargs(e, m), sa);
         GC_LAUNCH_CHECK("membrane substeps (per cell)");
     } else {
+        if (from_saved && (st = restore_start(e, m, gate))) return st;
         for (int q = 1; q <= m->k; ++q) {
             k_ms_kick_drift<<<nb, BS, 0, s>>>(m->nm, dts, inv_m, m->mx, m->mv, m->fe, m->fm, m->mcoor, gate);
             if ((st = ms_force(e, m, gate))) return st;
@@ -568,11 +580,11 @@ int plain_substeps(gc_engine* e, MembraneSub* m, bool record, long row0, const i
                 k_ms_copy_d<<<blocks_for(3 * m->nm, BS), BS, 0, s>>>(3 * m->nm, m->mx,
                                                                      q == KSUB - 2 ? m->mx6 : m->mx7);
         }
+        if (keep) k_ms_d2x<<<nb, BS, 0, s>>>(m->nm, m->mx, m->mx7, m->mx6, m->d2x);
     }
     if (keep) {  // the residual of the exact trajectory (scale for the threshold)
-        k_ms_d2x<<<nb, BS, 0, s>>>(m->nm, m->mx, m->mx7, m->mx6, m->d2x);
-        k_ms_finish<<<nb, BS, 0, s>>>(m->nm, 0, dts, inv_m, m->mv, m->fe, m->fm, m->d2x, m->lref, m->part);
-        k_ms_res_final<<<1, RES_BS, 0, s>>>(nb, m->nm, m->part, m->res + 1, 0.0f, nullptr);
+        k_ms_finish<<<nb, BS, 0, s>>>(m->nm, 0, dts, inv_m, m->mv, m->fe, m->fm, m->d2x, m->lref, m->part,
+                                      m->arrive, m->res + 1, 0.0f, nullptr);
         GC_LAUNCH_CHECK("membrane plain residual");
     }
     return GC_OK;
@@ -605,11 +617,14 @@ int ms_step(gc_engine* e)
     hipStream_t s = e->s;
     const int nm = m->nm, nc = (int)e->consts.nCellStart;
     const int nb = blocks_for(nm, BS);
-    k_ms_fext<<<nb, BS, 0, s>>>(nm, nc, e->d_curpos, e->d_force, m->fe);
-    GC_LAUNCH_CHECK("membrane fluid force");
-    int st = save_start(e, m);
-    if (st) return st;
     const bool record = m->mode == 1 && m->rec_x && m->k == KSUB && m->rec_count + nm <= m->rec_cap;
+    // the start state is kept for the surrogate (its prediction and
+    // fallback) and for recorded targets
+    const bool save = m->mode == 2 || record;
+    k_ms_fext<<<nb, BS, 0, s>>>(nm, nc, e->d_curpos, e->d_force, m->fe, m->mx, m->mv, m->fm,
+                                save ? m->mx0 : nullptr, m->mv0, m->fm0);
+    GC_LAUNCH_CHECK("membrane fluid force");
+    int st;
     if (m->mode == 1) {
         if (record) {
             k_ms_features<<<nb, BS, 0, s>>>(feat_args(e, m, false));
@@ -619,7 +634,7 @@ int ms_step(gc_engine* e)
                                 "record features")))
                 return st;
         }
-        if ((st = plain_substeps(e, m, record, m->rec_count))) return st;
+        if ((st = plain_substeps(e, m, record, m->rec_count, nullptr, false))) return st;
         if (record) m->rec_count += nm;
     } else {
         const float dts = e->hreps[0].P.dt / (float)KSUB, inv_m = e->hreps[0].P.inv_m;
@@ -630,24 +645,23 @@ int ms_step(gc_engine* e)
                                      m->mx0, m->mv0, m->mx, m->mv, m->mcoor, m->d2x);
         GC_LAUNCH_CHECK("membrane surrogate apply");
         if ((st = ms_force(e, m))) return st;
-        k_ms_finish<<<nb, BS, 0, s>>>(nm, 1, dts, inv_m, m->mv, m->fe, m->fm, m->d2x, m->lref, m->part);
-        if (use_cell_kernel(e)) {
-            // the fallback is decided on the device: the residual kernel sets
-            // the flag that gates the restore and the plain substeps (no host
-            // read; the step can be replayed from a graph)
-            k_ms_res_final<<<1, RES_BS, 0, s>>>(nb, nm, m->part, m->res, m->thr, m->flag);
-            GC_LAUNCH_CHECK("membrane residual");
-            if ((st = restore_start(e, m, m->flag)) || (st = plain_substeps(e, m, false, 0, m->flag))) return st;
+        const bool dev = use_cell_kernel(e);
+        // with the per-cell kernel the fallback is decided on the device: the
+        // residual kernel sets the flag that gates the plain substeps from the
+        // saved start (no host read; the step can be replayed from a graph)
+        k_ms_finish<<<nb, BS, 0, s>>>(nm, 1, dts, inv_m, m->mv, m->fe, m->fm, m->d2x, m->lref, m->part, m->arrive,
+                                      m->res, dev ? m->thr : 0.0f, dev ? m->flag : nullptr);
+        GC_LAUNCH_CHECK("membrane residual");
+        if (dev) {
+            if ((st = plain_substeps(e, m, false, 0, m->flag, true))) return st;
         } else {
-            k_ms_res_final<<<1, RES_BS, 0, s>>>(nb, nm, m->part, m->res, 0.0f, nullptr);
-            GC_LAUNCH_CHECK("membrane residual");
             float r = 0.f;
             if ((st = hip_check(hipMemcpyAsync(&r, m->res, sizeof(float), hipMemcpyDeviceToHost, s), "residual D2H")) ||
                 (st = hip_check(hipStreamSynchronize(s), "residual sync")))
                 return st;
             if (!(r <= m->thr)) {   // also catches NaN
                 m->fallbacks += 1;
-                if ((st = restore_start(e, m)) || (st = plain_substeps(e, m, false, 0))) return st;
+                if ((st = plain_substeps(e, m, false, 0, nullptr, true))) return st;
             }
         }
     }
@@ -732,7 +746,7 @@ extern "C" int gc_EngineSetMembraneMode(gc_engine* e, int mode, int k, gc_mlp* m
             (st = m->alloc(&m->lref, nm)) || (st = m->alloc(&m->frame, 9 * nm)) ||
             (st = m->alloc(&m->feat, (size_t)NF * nm)) || (st = m->alloc(&m->pred, (size_t)NO * nm)) ||
             (st = m->alloc(&m->norm, 2 * (NF + NO))) || (st = m->alloc(&m->part, (size_t)blocks_for(m->nm, BS))) ||
-            (st = m->alloc(&m->res, 2)) || (st = m->alloc(&m->flag, 4))) {
+            (st = m->alloc(&m->res, 2)) || (st = m->alloc(&m->flag, 4)) || (st = m->alloc(&m->arrive, 1))) {
             ms_destroy(m);
             return st;
         }
@@ -765,7 +779,8 @@ extern "C" int gc_EngineSetMembraneMode(gc_engine* e, int mode, int k, gc_mlp* m
         hipStream_t s = e->s;
         if ((st = upload(m->ring, ring.data(), ring.size(), s)) || (st = upload(m->lref, lref.data(), nm, s)) ||
             (st = upload(m->mid, mid.data(), nm, s)) || (st = upload(m->mnt, zero.data(), nm, s)) ||
-            (st = hip_check(hipMemsetAsync(m->flag, 0, 4 * sizeof(int), s), "ms flags"))) {
+            (st = hip_check(hipMemsetAsync(m->flag, 0, 4 * sizeof(int), s), "ms flags")) ||
+            (st = hip_check(hipMemsetAsync(m->arrive, 0, sizeof(unsigned), s), "ms counter"))) {
             ms_destroy(m);
             return st;
         }

@@ -39,7 +39,8 @@ enum {
     GC_ERR_NONFINITE = -7    /* engine detected a non-finite state */
 };
 
-/* Last error message of this library (never NULL). */
+/* Last error message of this library on the calling host thread (never
+ * NULL; valid until that thread's next call into the library). */
 const char* gc_GetLastError(void);
 /* Library version string. */
 const char* gc_Version(void);

@@ -76,8 +76,8 @@ int gc_MlpTrain(gc_mlp* m, const float* x, const float* y, int nsamples, int bat
  * is in defforce).  norm: 120 floats (feature mean[36], feature std[36],
  * target mean[24], target std[24]) or NULL for none.  With the per-cell
  * membrane kernel (tuning variant 3) the fallback is decided on the device:
- * the residual kernel sets a flag that gates the restore and the plain
- * substeps, so no value is read back per step and gc_EngineStepGraph
+ * the residual kernel sets a flag that gates the plain substeps from the
+ * saved start state, so no value is read back per step and gc_EngineStepGraph
  * replays modes 1 and 2 from its graph.  It steps eagerly (as
  * gc_EngineStep) while samples are being recorded, and in mode 2 with the
  * three-pass membrane variants, whose fallback is decided on the host. */
