@@ -361,22 +361,14 @@ __global__ void k_ms_d2x(int nm, const double* __restrict__ x8, const double* __
     d2x[v] = tfloat3{c[0], c[1], c[2]};
 }
 
-// The residual R = sqrt(mean_v |d2x_v - dts^2 a_v|^2 / l_v^2); with kick,
-// first the surrogate's end kick v8 = v7.5 + dts/2 a8.  Each block writes the
-// tree sum of its vertices to part[block]; the block that arrives last (a
-// device-scope counter, cnt, which it resets to 0) sums part[] in block
-// order (strided, then a tree: the same order whichever block is last) and
-// writes out[0] = R; with flag, flag[0] = fallback needed (R > thr or not a
-// number) and flag[1] counts the fallbacks.
+// residual partial sums (block tree, fixed order); with kick, first the
+// surrogate's end kick v8 = v7.5 + dts/2 a8
 __global__ __launch_bounds__(BS) void k_ms_finish(int nm, int kick, float dts, float inv_m, tfloat3* __restrict__ mv,
     const tfloat3* __restrict__ fe, const tfloat3* __restrict__ fm, const tfloat3* __restrict__ d2x,
-    const float* __restrict__ lref, float* part, unsigned* cnt, float* __restrict__ out, float thr,
-    int* __restrict__ flag)
+    const float* __restrict__ lref, float* __restrict__ part)
 {
     __shared__ float sh[BS];
-    __shared__ bool last;
-    const int tid = threadIdx.x;
-    int v = blockIdx.x * blockDim.x + tid;
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
     float q = 0.f;
     if (v < nm) {
         tfloat3 a = fe[v], b = fm[v], d = d2x[v];
@@ -391,32 +383,35 @@ __global__ __launch_bounds__(BS) void k_ms_finish(int nm, int kick, float dts, f
         float r = norm3(sub3(dd, e)) / lref[v];
         q = r * r;
     }
-    sh[tid] = q;
+    sh[threadIdx.x] = q;
     __syncthreads();
     for (int o = BS / 2; o > 0; o >>= 1) {
-        if (tid < o) sh[tid] += sh[tid + o];
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
         __syncthreads();
     }
-    if (tid == 0) {
-        part[blockIdx.x] = sh[0];
-        __threadfence();   // the partial is visible before the arrival is counted
-        last = atomicAdd(cnt, 1u) == gridDim.x - 1;
-        __threadfence();
-    }
-    __syncthreads();
-    if (!last) return;   // block-uniform
-    __threadfence();     // every other block's partial is visible from here
-    const int nb = (int)gridDim.x;
-    q = 0.f;
-    for (int i = tid; i < nb; i += BS) q += part[i];
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// out[0] = rms residual; with flag: flag[0] = fallback needed (R > thr or
+// not a number), flag[1] counts the fallbacks.  One block: strided sums,
+// then a tree (fixed order).  (A last-block reduction inside k_ms_finish
+// needs a device-scope release per block, which writes back the L2: 10M
+// 17 -> 250 us.)
+constexpr int RES_BS = 1024;
+__global__ __launch_bounds__(RES_BS) void k_ms_res_final(int nb, int nm, const float* __restrict__ part,
+                                                         float* __restrict__ out, float thr, int* __restrict__ flag)
+{
+    __shared__ float sh[RES_BS];
+    const int tid = threadIdx.x;
+    float q = 0.f;
+    for (int i = tid; i < nb; i += RES_BS) q += part[i];
     sh[tid] = q;
     __syncthreads();
-    for (int o = BS / 2; o > 0; o >>= 1) {
+    for (int o = RES_BS / 2; o > 0; o >>= 1) {
         if (tid < o) sh[tid] += sh[tid + o];
         __syncthreads();
     }
     if (tid != 0) return;
-    *cnt = 0u;   // for the next launch (kernel boundaries order it)
     const float r = sqrtf(sh[0] / (float)nm);
     out[0] = r;
     if (flag) {
@@ -463,7 +458,6 @@ struct MembraneSub {
     int* ring; float* lref;
     float *frame, *feat, *pred, *norm = nullptr, *part, *res;
     int* flag = nullptr;   // [0] fallback this step, [1] fallbacks, [2] steps (device counters)
-    unsigned* arrive = nullptr;   // blocks of the residual pass done (its last block resets it)
     bool have_norm = false;
     float* rec_x = nullptr; float* rec_y = nullptr;
     long rec_cap = 0, rec_count = 0;
@@ -583,8 +577,8 @@ int plain_substeps(gc_engine* e, MembraneSub* m, bool record, long row0, const i
         if (keep) k_ms_d2x<<<nb, BS, 0, s>>>(m->nm, m->mx, m->mx7, m->mx6, m->d2x);
     }
     if (keep) {  // the residual of the exact trajectory (scale for the threshold)
-        k_ms_finish<<<nb, BS, 0, s>>>(m->nm, 0, dts, inv_m, m->mv, m->fe, m->fm, m->d2x, m->lref, m->part,
-                                      m->arrive, m->res + 1, 0.0f, nullptr);
+        k_ms_finish<<<nb, BS, 0, s>>>(m->nm, 0, dts, inv_m, m->mv, m->fe, m->fm, m->d2x, m->lref, m->part);
+        k_ms_res_final<<<1, RES_BS, 0, s>>>(nb, m->nm, m->part, m->res + 1, 0.0f, nullptr);
         GC_LAUNCH_CHECK("membrane plain residual");
     }
     return GC_OK;
@@ -649,8 +643,8 @@ int ms_step(gc_engine* e)
         // with the per-cell kernel the fallback is decided on the device: the
         // residual kernel sets the flag that gates the plain substeps from the
         // saved start (no host read; the step can be replayed from a graph)
-        k_ms_finish<<<nb, BS, 0, s>>>(nm, 1, dts, inv_m, m->mv, m->fe, m->fm, m->d2x, m->lref, m->part, m->arrive,
-                                      m->res, dev ? m->thr : 0.0f, dev ? m->flag : nullptr);
+        k_ms_finish<<<nb, BS, 0, s>>>(nm, 1, dts, inv_m, m->mv, m->fe, m->fm, m->d2x, m->lref, m->part);
+        k_ms_res_final<<<1, RES_BS, 0, s>>>(nb, nm, m->part, m->res, dev ? m->thr : 0.0f, dev ? m->flag : nullptr);
         GC_LAUNCH_CHECK("membrane residual");
         if (dev) {
             if ((st = plain_substeps(e, m, false, 0, m->flag, true))) return st;
@@ -746,7 +740,7 @@ extern "C" int gc_EngineSetMembraneMode(gc_engine* e, int mode, int k, gc_mlp* m
             (st = m->alloc(&m->lref, nm)) || (st = m->alloc(&m->frame, 9 * nm)) ||
             (st = m->alloc(&m->feat, (size_t)NF * nm)) || (st = m->alloc(&m->pred, (size_t)NO * nm)) ||
             (st = m->alloc(&m->norm, 2 * (NF + NO))) || (st = m->alloc(&m->part, (size_t)blocks_for(m->nm, BS))) ||
-            (st = m->alloc(&m->res, 2)) || (st = m->alloc(&m->flag, 4)) || (st = m->alloc(&m->arrive, 1))) {
+            (st = m->alloc(&m->res, 2)) || (st = m->alloc(&m->flag, 4))) {
             ms_destroy(m);
             return st;
         }
@@ -779,8 +773,7 @@ extern "C" int gc_EngineSetMembraneMode(gc_engine* e, int mode, int k, gc_mlp* m
         hipStream_t s = e->s;
         if ((st = upload(m->ring, ring.data(), ring.size(), s)) || (st = upload(m->lref, lref.data(), nm, s)) ||
             (st = upload(m->mid, mid.data(), nm, s)) || (st = upload(m->mnt, zero.data(), nm, s)) ||
-            (st = hip_check(hipMemsetAsync(m->flag, 0, 4 * sizeof(int), s), "ms flags")) ||
-            (st = hip_check(hipMemsetAsync(m->arrive, 0, sizeof(unsigned), s), "ms counter"))) {
+            (st = hip_check(hipMemsetAsync(m->flag, 0, 4 * sizeof(int), s), "ms flags"))) {
             ms_destroy(m);
             return st;
         }
