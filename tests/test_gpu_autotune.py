@@ -72,9 +72,10 @@ def test_autotune_keeps_trajectory_and_reports_each_variant(gpu):
 def test_membrane_substeps_per_cell_kernel_bit_identical(gpu):
     """Plain membrane substepping evaluates the membrane force 8 times per
     step: the per-cell kernel (variant 3) and the three-pass form (variant
-    0) give the same bits."""
+    0) give the same bits, and the same residual of the exact trajectory
+    (the per-cell kernel forms x8 - 2 x7 + x6 itself)."""
     c = case()
-    outs = []
+    outs, res = [], []
     for v in (0, 3):
         e = Engine(c)
         t = e.tuning()
@@ -83,6 +84,8 @@ def test_membrane_substeps_per_cell_kernel_bit_identical(gpu):
         e.set_membrane_mode(1, k=8)
         e.step(3)
         outs.append(e.state())
+        res.append(e.membrane_stats()["plain_residual"])
         e.close()
     for k in KEYS:
         assert np.array_equal(outs[0][k], outs[1][k]), k
+    assert res[0] > 0 and res[0] == res[1]
