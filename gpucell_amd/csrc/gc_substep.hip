@@ -421,8 +421,8 @@ __global__ __launch_bounds__(RES_BS) void k_ms_res_final(int nb, int nm, const f
     }
 }
 
-// saves or restores the substep state (positions, velocity, membrane
-// force); with a gate only when *gate != 0
+// copies the substep state (positions, velocity, membrane force): the
+// restore of the saved start state; with a gate only when *gate != 0
 __global__ void k_ms_copy_state(int nm, const double* __restrict__ mx, const tfloat3* __restrict__ mv,
                                 const tfloat3* __restrict__ fm, double* __restrict__ mx_to, tfloat3* __restrict__ mv_to,
                                 tfloat3* __restrict__ fm_to, const int* __restrict__ gate)
