@@ -736,12 +736,38 @@ __global__ __launch_bounds__(BS) void k2_agg(int nm, GcRep R0, const GcRep* __re
     };
     const int cnt = nlc[p];
     if (cnt >= 0) {
-        // the lanes' slots are scattered: each lane's own list block
+        // the lanes' slots are scattered: each lane's own list block.  Eight
+        // entries at a time: their codes, then their positions, then their
+        // cells are loaded from clamped indices before any is used (one entry
+        // after another, each pair would wait out three dependent loads);
+        // the pairs are then summed in list order
         const int* wl = nl + (size_t)(p >> 6) * (NL_CAP * 64) + (p & 63);
-        for (int k = 0; k < cnt; ++k) {
-            const int code = wl[64 * k];
-            morse(code >> 6, code_shift(code, R.L[0]), code_shift(code >> 2, R.L[1]),
-                  code_shift(code >> 4, R.L[2]));
+        constexpr int G = 8;
+        for (int k0 = 0; k0 < cnt; k0 += G) {
+            int code[G], cj[G];
+            float4 pj[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) code[u] = wl[64 * min(k0 + u, cnt - 1)];
+#pragma unroll
+            for (int u = 0; u < G; ++u) pj[u] = posid[code[u] >> 6];
+#pragma unroll
+            for (int u = 0; u < G; ++u) cj[u] = celltab[max(__float_as_int(pj[u].w) - P.nc, 0)];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                if (k0 + u >= cnt) break;
+                const int idj = __float_as_int(pj[u].w);
+                if (idj < P.nc) continue;
+                const float sx = code_shift(code[u], R.L[0]), sy = code_shift(code[u] >> 2, R.L[1]),
+                            sz = code_shift(code[u] >> 4, R.L[2]);
+                const float rx = pi.x - (pj[u].x + sx), ry = pi.y - (pj[u].y + sy), rz = pi.z - (pj[u].z + sz);
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                if (r2 >= P.mcut2 || r2 == 0.f || cj[u] == cell) continue;
+                const float ir = __builtin_amdgcn_rsqf(r2);
+                const float r = r2 * ir;
+                const float ex = expf(P.beta * (P.r0 - r));
+                const float fm = tb * (ex * ex - ex) * ir;
+                fx += fm * rx; fy += fm * ry; fz += fm * rz;
+            }
         }
     } else {
         visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
