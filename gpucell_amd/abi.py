@@ -74,6 +74,7 @@ class gc_engine_info(C.Structure):
         ("step", C.c_long), ("noise_counter", C.c_uint64), ("nTotalPar", C.c_int),
         ("nCellPar", C.c_int), ("nCell", C.c_int), ("sizecube", C.c_int),
         ("ncube", C.c_int * 3), ("outerregion", C.c_float * 3), ("invwid", C.c_float * 3),
+        ("fault", C.c_int), ("nl_overflow", C.c_int),
     ]
 
 

@@ -283,7 +283,8 @@ __global__ __launch_bounds__(CUBE_SCAN_BS) void k2_tile_sums(int K, const int* _
 // step it also advances the noise counter (the force pass of this step
 // reads the value before the advance).
 __global__ __launch_bounds__(CUBE_SCAN_BS) void k2_cube_offsets(int K, const int* __restrict__ cnt,
-    const int* __restrict__ tsum, int* __restrict__ offs, unsigned long long* __restrict__ ctr, int bump)
+    const int* __restrict__ tsum, int* __restrict__ offs, unsigned long long* __restrict__ ctr, int bump,
+    int* __restrict__ nlover)
 {
     __shared__ int red[CUBE_SCAN_BS];
     const int b = blockIdx.x, tid = threadIdx.x;
@@ -320,7 +321,10 @@ __global__ __launch_bounds__(CUBE_SCAN_BS) void k2_cube_offsets(int K, const int
         run += v[k];
     }
     if (b == (int)gridDim.x - 1 && tid == CUBE_SCAN_BS - 1) offs[K] = before + red[CUBE_SCAN_BS - 1];
-    if (bump && b == 0 && tid == 0) *ctr += 1ull;
+    if (b == 0 && tid == 0) {
+        if (bump) *ctr += 1ull;
+        *nlover = 0;   // list overflows of this step's density pass
+    }
 }
 
 // [4] (cube, id) into cube segments; clears the histogram for the next step
@@ -355,7 +359,7 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     const tint3* __restrict__ nt_old, tint3* __restrict__ nt_new, int* __restrict__ sp_new,
     int* __restrict__ key_new, float4* __restrict__ posid, tfloat3* __restrict__ coor,
     float4* __restrict__ velrho, tfloat3* __restrict__ velhalf, const int* __restrict__ wincl,
-    int* __restrict__ act)
+    int* __restrict__ act, int* __restrict__ fault)
 {
     int q = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (q >= n) return;
@@ -392,6 +396,9 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     }
     });
     pos_new[s] = x; pos_new[n + s] = y; pos_new[2 * n + s] = z;
+    // a non-finite position has no neighbours (every distance test fails)
+    // and so a finite force: it is caught here
+    if (!isfinite(x + y + z)) atomicOr(fault, GC_FAULT_NONFINITE);
     float fx = (float)x, fy = (float)y, fz = (float)z;
     posid[s] = make_float4(fx, fy, fz, __int_as_float(id));
     coor[s] = tfloat3{fx, fy, fz};
@@ -481,7 +488,7 @@ template <bool ENS, int TBS, int TSTAGE>
 __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, const float4* __restrict__ posid, float4* __restrict__ velrho,
     const int* __restrict__ offs, const int* __restrict__ key, int* __restrict__ nl,
-    int* __restrict__ nlc)
+    int* __restrict__ nlc, int* __restrict__ nlover)
 {
     __shared__ float4 stage[TBS / 64][TSTAGE > 0 ? TSTAGE : 1];
     const int lane = threadIdx.x & 63;
@@ -568,6 +575,7 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
         const int cnt = (int)((woff - (unsigned)lane) >> 6);
         velrho[p].w = P.m * P.wnorm * (d + 1.0f);
         nlc[p] = cnt <= NL_CAP ? cnt : -1;
+        if (cnt > NL_CAP) atomicAdd(nlover, 1);
     } else if (act && id < P.nf) {
         const float3 u = wall_velocity(P, pi.y, sw, svx, svy, svz);
         float4 v = velrho[p];
@@ -586,7 +594,7 @@ __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep*
     const float4* __restrict__ velrho, const int* __restrict__ offs, const int* __restrict__ key,
     const int* __restrict__ nl, const int* __restrict__ nlc,
     const tfloat3* __restrict__ velhalf, tfloat3* __restrict__ force, float4* __restrict__ vout,
-    const unsigned long long* __restrict__ ctr)
+    const unsigned long long* __restrict__ ctr, int* __restrict__ fault)
 {
     // one lane per moving particle (walls get no force: no lane idles on a
     // wall slot); act lists their slots in slot order
@@ -687,6 +695,7 @@ __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep*
         });
     }
     if (driveon) fx += ext_force_x(P, pi.y);
+    if (!isfinite(fx + fy + fz)) atomicOr(fault, GC_FAULT_NONFINITE);
     force[p] = tfloat3{fx, fy, fz};
     if (id < P.nc) {
         float h = 0.5f * P.dt * P.inv_m;
@@ -883,7 +892,8 @@ int launch_build_t(gc_engine* e, int move, Prof* pr = nullptr)
     GC_LAUNCH_CHECK("engine integrate");
     mark(pr, s, "integrate+key");
     k2_tile_sums<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum);
-    k2_cube_offsets<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, e->d_offs, e->d_ctr, move);
+    k2_cube_offsets<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, e->d_offs, e->d_ctr, move,
+                                                      e->d_nlover);
     GC_LAUNCH_CHECK("engine cube offsets");
     mark(pr, s, "cube offsets");
     k2_scatter<<<gb, BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[c], e->d_offs, e->d_tk, e->d_tv,
@@ -893,7 +903,7 @@ int launch_build_t(gc_engine* e, int move, Prof* pr = nullptr)
         e->d_tk, e->d_tv,
         e->d_to, e->d_offs, e->d_curpos, e->d_pos64[c], e->d_pos64[nx], e->d_vout, e->d_force, e->d_nt[c],
         e->d_nt[nx], e->d_sp[nx], e->d_key, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf,
-        e->moving_walls ? nullptr : e->d_wincl, e->d_act);
+        e->moving_walls ? nullptr : e->d_wincl, e->d_act, e->d_fault);
     GC_LAUNCH_CHECK("engine sort/gather");
     e->cur = nx;
     return GC_OK;
@@ -919,7 +929,7 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
     {
         auto L = [&](auto kern, int tbs) {
             kern<<<xcd_grid(blocks_for(N, tbs)), tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid,
-                e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc);
+                e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc, e->d_nlover);
         };
         switch (e->tune.v[TUNE_DENSITY]) {   // block size x LDS row stage
             case 0: L(k2_density<ENS, 64, 0>, 64); break;
@@ -943,7 +953,7 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
             if (e->nmov > 0)
                 kern<<<xcd_grid(blocks_for(e->nmov, tbs)), tbs, 0, s>>>(e->nmov, R0, e->d_reps, e->d_rep_of,
                     driveon, e->d_act, e->d_posid, e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc,
-                    e->d_velhalf, e->d_force, e->d_vout, e->d_ctr);
+                    e->d_velhalf, e->d_force, e->d_vout, e->d_ctr, e->d_fault);
         };
         switch (e->tune.v[TUNE_FORCE]) {
             case 0: L(k2_force<ENS, 64>, 64); break;
@@ -985,6 +995,7 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
         a.defforce = e->d_defforce; a.force = e->d_force;
         a.velhalf = e->d_velhalf; a.velrho = e->d_velrho; a.vout = e->d_vout;
         a.nm = e->nm;
+        a.fault = e->d_fault;
         if (ENS) { a.reps = e->d_reps; a.cell_rep = e->d_cell_rep; a.celltab = e->d_celltab; }
         if (use_cell_kernel(e)) {
             // geometry, membrane force and corrector in one workgroup per cell
@@ -1249,7 +1260,9 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
         (st = e->alloc(&e->d_offs, K + 1)) || (st = e->alloc(&e->d_tk, N)) ||
         (st = e->alloc(&e->d_tv, N)) || (st = e->alloc(&e->d_to, N)) || (st = e->alloc(&e->d_curpos, N)) ||
         (st = e->alloc(&e->d_nl, (size_t)NL_CAP * (((size_t)N + 63) & ~(size_t)63))) || (st = e->alloc(&e->d_nlc, N)) ||
-        (st = e->alloc(&e->d_ctr, 1)) ||
+        (st = e->alloc(&e->d_ctr, 1)) || (st = e->alloc(&e->d_fault, 1)) || (st = e->alloc(&e->d_nlover, 1)) ||
+        (st = hip_check(hipMemset(e->d_fault, 0, sizeof(int)), "engine fault word")) ||
+        (st = hip_check(hipMemset(e->d_nlover, 0, sizeof(int)), "engine overflow counter")) ||
         (st = e->alloc(&e->d_tsum, (size_t)e->ntile)) || (st = e->alloc(&e->d_act, N)) ||
         (st = e->alloc(&e->d_wincl, K + 1)) ||
         (st = hip_check(hipMemset(e->d_wincl, 0, sizeof(int) * (K + 1)), "engine wall table")) ||
@@ -1467,6 +1480,10 @@ extern "C" int gc_EngineInfo(const gc_engine* e, gc_engine_info* info)
     if ((st = hip_check(hipMemcpy(&c, e->d_ctr, sizeof(c), hipMemcpyDeviceToHost), "EngineInfo D2H")))
         return st;
     info->noise_counter = c;
+    if ((st = hip_check(hipMemcpy(&info->fault, e->d_fault, sizeof(int), hipMemcpyDeviceToHost), "EngineInfo D2H")) ||
+        (st = hip_check(hipMemcpy(&info->nl_overflow, e->d_nlover, sizeof(int), hipMemcpyDeviceToHost),
+                        "EngineInfo D2H")))
+        return st;
     info->nTotalPar = e->N; info->nCellPar = e->nm; info->nCell = e->ncell; info->sizecube = e->K;
     for (int d = 0; d < 3; ++d) {
         info->ncube[d] = e->hreps[0].nc[d]; info->outerregion[d] = e->hreps[0].L[d];
@@ -1476,6 +1493,21 @@ extern "C" int gc_EngineInfo(const gc_engine* e, gc_engine_info* info)
 }
 
 namespace {
+
+// GC_ERR_NONFINITE while a kernel has raised a fault bit (after a sync).
+int fault_status(gc_engine* e)
+{
+    int f = 0;
+    int st = hip_check(hipStreamSynchronize(e->s), "fault sync");
+    if (!st) st = hip_check(hipMemcpy(&f, e->d_fault, sizeof(int), hipMemcpyDeviceToHost), "fault D2H");
+    if (st) return st;
+    if (f & GC_FAULT_WLC)
+        return set_error(GC_ERR_NONFINITE, "a membrane edge reached its WLC maximum length l_max "
+                                           "(singular spring force); the state is not physical");
+    if (f & GC_FAULT_NONFINITE)
+        return set_error(GC_ERR_NONFINITE, "non-finite position or force in the engine state");
+    return GC_OK;
+}
 
 // The whole state in global id order.
 struct Snapshot {
@@ -1559,7 +1591,7 @@ int export_state(gc_engine* e, int r, gc_host_state* o)
         if (o->defforce && (st = dl(o->defforce, e->d_defforce + m.vtx_off, sizeof(tfloat3) * m.nm))) return st;
         if (o->aggforce && (st = dl(o->aggforce, e->d_aggforce + m.vtx_off, sizeof(tfloat3) * m.nm))) return st;
     }
-    return GC_OK;
+    return fault_status(e);
 }
 
 }  // namespace
@@ -1594,10 +1626,11 @@ int restore_state(gc_engine* e, const std::vector<double>& pos, const std::vecto
     e->cur = 0;
     hipStream_t s = e->s;
     int st;
+    const int zero = 0;
     if ((st = upload(e->d_pos64[0], pos_soa.data(), 3 * (size_t)N, s)) || (st = upload(e->d_vout, vo.data(), N, s)) ||
         (st = upload(e->d_force, f.data(), N, s)) || (st = upload(e->d_nt[0], nt.data(), N, s)) ||
         (st = upload(e->d_sp[0], idn.data(), N, s)) || (st = upload(e->d_curpos, idn.data(), N, s)) ||
-        (st = upload(e->d_ctr, &ctr, 1, s)))
+        (st = upload(e->d_ctr, &ctr, 1, s)) || (st = upload(e->d_fault, &zero, 1, s)))
         return st;
     if ((st = launch_build(e, 0))) return st;
     if ((st = reorder_carried(e, vo, f, vh))) return st;

@@ -77,6 +77,8 @@ struct gc_engine {
     int ntile = 0;
     int *d_nl, *d_nlc;   // neighbour lists of moving particles (density -> force)
     unsigned long long* d_ctr;
+    int* d_fault = nullptr;    // GC_FAULT_* bits raised by the kernels (sticky)
+    int* d_nlover = nullptr;   // density-pass list overflows of the last step
     // membrane
     tint3* d_tri; int6* d_edge; int* d_celltab; tint2* d_betri;
     float *d_mu0, *d_kc, *d_kd, *d_ka, *d_kv;

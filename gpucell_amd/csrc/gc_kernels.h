@@ -89,6 +89,9 @@ struct DefArgs {
     const double* mpos = nullptr; int nm = 0;
     // batched runs: per-vertex replica (physics of the vertex's suspension)
     const GcRep* reps = nullptr; const int* cell_rep = nullptr; const int* celltab = nullptr;
+    // engine: fault word (GC_FAULT_* bits OR-ed in on a non-finite vertex
+    // force or an edge at or past l_max); NULL: not watched
+    int* fault = nullptr;
 };
 
 // reps/cell_rep: batched runs (NULL for one suspension) -- the cell's

@@ -439,8 +439,10 @@ __device__ __forceinline__ TriPar tri_par(const DefArgs& a, int t)
 __device__ __forceinline__ float rcp_hw(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float rsq_hw(float x) { return __builtin_amdgcn_rsqf(x); }
 
+// bad: GC_FAULT_WLC is OR-ed in when the edge is at or past l_max (the
+// WLC force is singular at x = 1 and has the wrong sign beyond it).
 __device__ __forceinline__ void edge_terms(const GcParams& P, const EdgePar& q, float3 pB, float3 pC, float3 pD,
-                                           float3 out[4])
+                                           float3 out[4], int& bad)
 {
     // WLC-POW spring along d = A - B: s d on A, -s d on B
     const float3 d = make_float3(-pB.x, -pB.y, -pB.z);
@@ -448,6 +450,7 @@ __device__ __forceinline__ void edge_terms(const GcParams& P, const EdgePar& q, 
     const float rls = rsq_hw(dd);
     const float ls = dd * rls;
     const float x = ls * rcp_hw(q.lmax);
+    bad |= x >= 1.0f ? GC_FAULT_WLC : 0;
     const float iom = rcp_hw(1.0f - x);
     // kp / ls^m = kp 2^(-m log2 ls)
     const float fm = -(P.kT * rcp_hw(q.p)) * (0.25f * iom * iom - 0.25f + x) +
@@ -516,7 +519,9 @@ __global__ __launch_bounds__(BS) void k_edge_force(GcParams P, int nedge, const 
     const float3 pC = vpos(a, P, b, E.z, xo, no, ox, oy, oz);
     const float3 pD = vpos(a, P, b, E.k, xo, no, ox, oy, oz);
     float3 f[4];
-    edge_terms(P, edge_par(a, e), pB, pC, pD, f);
+    int bad = 0;
+    edge_terms(P, edge_par(a, e), pB, pC, pD, f, bad);
+    if (bad && a.fault) atomicOr(a.fault, bad);
     tfloat3* o = ef + 4 * (size_t)e;
 #pragma unroll
     for (int k = 0; k < 4; ++k) o[k] = tfloat3{f[k].x, f[k].y, f[k].z};
@@ -559,6 +564,7 @@ __global__ __launch_bounds__(BS) void k_vert_force(GcParams P, int nm, DefArgs a
         const tfloat3 g = tf[3 * (code >> 2) + (code & 3)];
         fx += g.x; fy += g.y; fz += g.z;
     }
+    if (a.fault && !isfinite(fx + fy + fz)) atomicOr(a.fault, GC_FAULT_NONFINITE);
     a.defforce[v] = tfloat3{fx, fy, fz};
     if (!a.force) return;   // membrane-only evaluation (substepping)
     const int sv = a.curpos[P.nc + v];
@@ -689,6 +695,7 @@ __device__ __forceinline__ void cell_force(const GcParams& P, const CellArgs& ca
         }
     }
     // edges -> buf[12 (e - e0) + 3 role]
+    int bad = 0;
     for (int i0 = tid; i0 < ne; i0 += 2 * BS) {
         int6 Es[2];
         EdgePar qs[2];
@@ -705,7 +712,7 @@ __device__ __forceinline__ void cell_force(const GcParams& P, const CellArgs& ca
             const int6 E = Es[u];
             const int ua = E.x - 1 - v0;
             float3 fe[4];
-            edge_terms(P, qs[u], rel(E.y - 1 - v0, ua), rel(E.z - 1 - v0, ua), rel(E.k - 1 - v0, ua), fe);
+            edge_terms(P, qs[u], rel(E.y - 1 - v0, ua), rel(E.z - 1 - v0, ua), rel(E.k - 1 - v0, ua), fe, bad);
             float* q = buf + 12 * i;
 #pragma unroll
             for (int k = 0; k < 4; ++k) { q[3 * k] = fe[k].x; q[3 * k + 1] = fe[k].y; q[3 * k + 2] = fe[k].z; }
@@ -758,6 +765,10 @@ __device__ __forceinline__ void cell_force(const GcParams& P, const CellArgs& ca
         }
     }
     f = make_float3(fx, fy, fz);
+    if (a.fault) {
+        if (tid < nv && !isfinite(fx + fy + fz)) bad |= GC_FAULT_NONFINITE;
+        if (bad) atomicOr(a.fault, bad);
+    }
 }
 
 __global__ __launch_bounds__(GC_CELL_BS) void k_cell_membrane(GcParams P, const float* __restrict__ bbox,

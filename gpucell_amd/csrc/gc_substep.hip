@@ -497,14 +497,18 @@ DefArgs ms_defargs(gc_engine* e, MembraneSub* m)
     a.V0c = e->d_refvol_para; a.es = e->d_es; a.ee = e->d_ee; a.ts = e->d_ts; a.te = e->d_te;
     a.defforce = m->fm; a.force = nullptr;
     a.mpos = m->mx; a.nm = m->nm;
+    a.fault = e->d_fault;
     return a;
 }
 
-// membrane force (and geometry) at the membrane system's positions
-int ms_force(gc_engine* e, MembraneSub* m, const int* gate = nullptr)
+// membrane force (and geometry) at the membrane system's positions; watch =
+// false for the surrogate's predicted state (a bad prediction is rejected by
+// its residual and replaced by plain substeps, not reported as a fault)
+int ms_force(gc_engine* e, MembraneSub* m, const int* gate = nullptr, bool watch = true)
 {
     hipStream_t s = e->s;
-    const DefArgs a = ms_defargs(e, m);
+    DefArgs a = ms_defargs(e, m);
+    if (!watch) a.fault = nullptr;
     if (use_cell_kernel(e)) {   // geometry from the FP32 positions, as k_geo_cell below
         CellArgs ca = cell_args(e, e->step, 1);
         ca.gate = gate;
@@ -638,7 +642,7 @@ int ms_step(gc_engine* e)
         k_ms_apply<<<nb, BS, 0, s>>>(nm, dts, m->pred, m->have_norm ? m->norm : nullptr, m->frame, m->lref,
                                      m->mx0, m->mv0, m->mx, m->mv, m->mcoor, m->d2x);
         GC_LAUNCH_CHECK("membrane surrogate apply");
-        if ((st = ms_force(e, m))) return st;
+        if ((st = ms_force(e, m, nullptr, false))) return st;
         const bool dev = use_cell_kernel(e);
         // with the per-cell kernel the fallback is decided on the device: the
         // residual kernel sets the flag that gates the plain substeps from the
@@ -724,12 +728,22 @@ extern "C" int gc_EngineSetMembraneMode(gc_engine* e, int mode, int k, gc_mlp* m
         return set_error(GC_ERR_INVALID, "SetMembraneMode: k must be >= 1 (8 for the surrogate)");
     if (mode == 2 && (!mlp || !(threshold > 0)))
         return set_error(GC_ERR_INVALID, "SetMembraneMode: the surrogate needs a network and a threshold > 0");
+    // every argument is checked before the engine is touched: a rejected call
+    // leaves the engine as it was
+    if (norm) {
+        for (int c = 0; c < NF; ++c)
+            if (!(norm[NF + c] > 0)) return set_error(GC_ERR_INVALID, "SetMembraneMode: feature std must be > 0");
+        for (int c = 0; c < NO; ++c)
+            if (!(norm[2 * NF + NO + c] > 0)) return set_error(GC_ERR_INVALID, "SetMembraneMode: target std must be > 0");
+    }
     int st = hip_check(hipStreamSynchronize(e->s), "SetMembraneMode sync");
     if (st) return st;
     if (e->gexec) { (void)hipGraphExecDestroy(e->gexec); e->gexec = nullptr; }
-    bool fresh = e->ms == nullptr;
+    MembraneSub* m = e->ms;
+    const bool fresh = m == nullptr;
     if (fresh) {
-        MembraneSub* m = new MembraneSub();
+        // built aside and installed only once it is complete
+        m = new MembraneSub();
         const size_t nm = (size_t)e->nm;
         m->nm = e->nm;
         if ((st = m->alloc(&m->mx, 3 * nm)) || (st = m->alloc(&m->mx0, 3 * nm)) ||
@@ -782,22 +796,25 @@ extern "C" int gc_EngineSetMembraneMode(gc_engine* e, int mode, int k, gc_mlp* m
         m->Pm.nb2 = m->Pm.nf = m->Pm.nc = 0;
         m->Pm.N = m->nm;
         m->Pm.per[0] = m->Pm.per[1] = m->Pm.per[2] = 0;
-        e->ms = m;
     }
-    MembraneSub* m = e->ms;
+    if (norm && (st = upload(m->norm, norm, 2 * (NF + NO), e->s))) {
+        if (fresh) ms_destroy(m);
+        return st;
+    }
     m->mode = mode;
     m->k = k;
     m->mlp = mlp;
     m->thr = threshold;
     m->have_norm = norm != nullptr;
-    if (norm) {
-        for (int c = 0; c < NF; ++c)
-            if (!(norm[NF + c] > 0)) return set_error(GC_ERR_INVALID, "SetMembraneMode: feature std must be > 0");
-        for (int c = 0; c < NO; ++c)
-            if (!(norm[2 * NF + NO + c] > 0)) return set_error(GC_ERR_INVALID, "SetMembraneMode: target std must be > 0");
-        if ((st = upload(m->norm, norm, 2 * (NF + NO), e->s))) return st;
+    if (fresh) {
+        e->ms = m;
+        if ((st = ms_reset(e))) {
+            (void)hipStreamSynchronize(e->s);
+            ms_destroy(m);
+            e->ms = nullptr;
+            return st;
+        }
     }
-    if (fresh && (st = ms_reset(e))) return st;
     return hip_check(hipStreamSynchronize(e->s), "SetMembraneMode sync");
 }
 

@@ -258,13 +258,25 @@ int gc_EngineStep(gc_engine* e, int nsteps, int driveon);
 /* Same, captured once into a HIP graph and replayed (no host work per step). */
 int gc_EngineStepGraph(gc_engine* e, int nsteps, int driveon);
 
+/* Fault bits an engine's kernels raise (sticky until gc_EngineSetState or
+ * gc_EngineLoad): a non-finite force, or a membrane edge stretched to or
+ * past its WLC maximum length l_max (x = l / l_max >= 1, where the spring
+ * force is singular).  Export and checkpoint calls return GC_ERR_NONFINITE
+ * while any bit is set (the data are still copied out for inspection). */
+enum { GC_FAULT_NONFINITE = 1, GC_FAULT_WLC = 2 };
+
 typedef struct {
     long step;                    /* steps taken */
     uint64_t noise_counter;
     int nTotalPar, nCellPar, nCell, sizecube;
     int ncube[3];
     float outerregion[3], invwid[3];
+    int fault;                    /* GC_FAULT_* bits raised so far */
+    int nl_overflow;              /* particles of the last step with more
+                                     neighbours than the list holds (their
+                                     force and Morse passes scan the cubes) */
 } gc_engine_info;
+/* Synchronises; returns GC_OK whatever the fault bits (they are in info). */
 int gc_EngineInfo(const gc_engine* e, gc_engine_info* info);
 
 /* State copied to host in id order; any pointer may be NULL. */

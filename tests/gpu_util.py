@@ -108,3 +108,62 @@ def oracle_forces(o, case, pos, vel, rho_in, img, counter, driveon, want_rho=Tru
         extras["def"] = o.def_force(pos, img, F, ab)
         extras["agg"] = o.agg_force(pos, F, ab)
     return F, ab, rho, extras
+
+
+def oracle_state(case, S, info):
+    """The oracle's step input from an engine state S (id order) and the
+    engine's step / noise counter."""
+    st = {
+        "pos": S["pos"].copy(), "img": S["nthrobound"].copy(),
+        "vel": S["velrhop"][:, :3].astype(np.float64), "rho": S["velrhop"][:, 3].astype(np.float64),
+        "velhalf": S["velhalf"].astype(np.float64), "force": S["force"].astype(np.float64),
+        "press": S["press"].astype(np.float64), "abs_sum": np.zeros(case.N),
+        "step": int(info.step), "counter": int(info.noise_counter)}
+    if case.nCell:
+        st.update(defforce=np.zeros((case.nCellPar, 3)), aggforce=np.zeros((case.nCellPar, 3)),
+                  cell_area=np.zeros(case.nCell), cell_vol=np.zeros(case.nCell))
+    return st
+
+
+def step_parity(o, case, S, S1, info, driveon=1, label=""):
+    """One engine step (state S before, S1 after; info = the engine's
+    step and noise counter before the step) against the FP64 oracle `o`
+    (reference geometry and mechanics already set by o.initial_state()):
+
+    * FP64 positions after the step to 1e-9 and the image counters exactly
+      (the oracle advances S with S's own velocities and forces);
+    * densities to 2e-6 relative and every particle's total force to the
+      per-particle bound |F - F*| / (sum_j |f_ij*| + |F_ext|) <= 1e-5, both
+      evaluated by the oracle at the FP32 positions the kernels read;
+    * cell areas and volumes to 2e-6 relative; the corrector
+      v = velhalf + dt/2 F/m.
+    Returns (max error, relative L2, oracle extras)."""
+    st = oracle_state(case, S, info)
+    o.step(st, driveon=driveon)
+    dpos = np.abs(S1["pos"] - st["pos"]).max()
+    assert dpos < 1e-9, (label, dpos)
+    assert np.array_equal(S1["nthrobound"], st["img"]), label
+    mov = np.arange(case.N) >= case.consts.nFluidStart
+    dt, lam, m = float(case.consts.TimeStep), float(case.consts.VVsigma), float(case.consts.Mass)
+    vt = S["velrhop"][:, :3].astype(np.float64).copy()
+    vt[mov] += lam * dt * S["force"][mov].astype(np.float64) / m
+    pos32 = S1["coor"].astype(np.float64)
+    o.bound2_vel(pos32, vt)
+    rho = np.full(case.N, float(case.consts.DenInit))
+    F_o, ab, rho, extra = oracle_forces(o, case, pos32, vt, rho, S1["nthrobound"], info.noise_counter,
+                                        driveon)
+    assert np.allclose(S1["velrhop"][:, 3], rho, rtol=2e-6), label
+    den = ab + abs(float(case.consts.ConstExtForce))
+    den = np.where(den > 0, den, 1.0)
+    err = np.linalg.norm(S1["force"].astype(np.float64) - F_o, axis=1) / den
+    emax = float(err.max())
+    l2 = float(np.linalg.norm(S1["force"] - F_o) / max(np.linalg.norm(F_o), 1e-300))
+    assert emax <= 1e-5, (label, emax, int(err.argmax()))
+    if case.nCell:
+        assert np.allclose(S1["area"], extra["geo"]["cell_area"], rtol=2e-6), label
+        assert np.allclose(S1["vol"], extra["geo"]["cell_vol"], rtol=2e-6), label
+    vh = S1["velhalf"].astype(np.float64)
+    v_exp = vh + 0.5 * dt * S1["force"].astype(np.float64) / m
+    assert np.allclose(S1["velrhop"][mov, :3], v_exp[mov], rtol=1e-6, atol=1e-7), label
+    extra["pos32"] = pos32
+    return emax, l2, extra

@@ -29,12 +29,28 @@ def box_case():
     return casegen.periodic_box(n=16, rbc_diameter=7.0, subdiv=2, seed=11)
 
 
+def morse_case(De=0.02, seed=5):
+    """Two RBCs face to face, rims 0.6 apart: Morse (MaxCut 1.5) acts on
+    many vertex pairs from the first step."""
+    ph = util.replace_physics(kT=0.01, De=De, r0=0.3, beta=1.5, mcut=1.5, fext=0.05)
+    return util.stacked_pair_channel(24, 12, diameter=5.0, subdiv=2, gap=0.6, physics=ph, seed=seed)
+
+
+def dense_morse_case():
+    """Two finely meshed RBCs (642 vertices on a 5-wide disc, ~0.33 apart)
+    face to face: membrane vertices have more than 64 neighbours within h,
+    so the force and Morse passes take their cube-scan fallback."""
+    ph = util.replace_physics(kT=0.01, De=0.05, r0=0.3, beta=1.5, mcut=1.5, fext=0.05)
+    return util.stacked_pair_channel(24, 12, diameter=5.0, subdiv=3, gap=0.5, physics=ph, seed=6)
+
+
 def bound1_case():
-    """Channel whose outermost wall layer is Bound1 (repulsion only) and
-    inner layers Bound2 (no-slip velocity)."""
+    """Channel whose innermost wall layer is half Bound1 (repulsion only,
+    within h of the first fluid layer) and otherwise Bound2 (no-slip
+    velocity)."""
     ph = util.replace_physics(kT=0.01, fext=0.05)
     c = casegen.channel(24, 12, 2, rbc_diameter=5.0, subdiv=2, seed=8, physics=ph)
-    return util.outer_walls_as_bound1(c, ph, 6.0)
+    return util.inner_walls_as_bound1(c, ph, 6.0)
 
 
 def big_cell_case():
@@ -49,9 +65,12 @@ def couette_case():
     return casegen.couette(n=16, n_rbc=1, rbc_diameter=7.0, shear_rate=0.01, seed=13, physics=ph)
 
 
-CASES = {"channel": lambda: channel_case(), "channel_morse": lambda: channel_case(De=0.02),
+CASES = {"channel": lambda: channel_case(), "channel_morse": morse_case, "dense_morse": dense_morse_case,
          "box": box_case, "channel_bound1": bound1_case, "box_big_cell": big_cell_case,
          "couette": couette_case}
+# cases whose particles exceed the 64-entry neighbour list (the force and
+# Morse passes then scan the cubes)
+OVERFLOW = ("dense_morse", "box_big_cell")
 
 
 @pytest.mark.parametrize("name", ["channel", "box"])
@@ -86,7 +105,8 @@ def _oracle_state(o, case, S, info):
     return st
 
 
-@pytest.mark.parametrize("name", ["channel", "channel_morse", "box", "channel_bound1", "box_big_cell", "couette"])
+@pytest.mark.parametrize("name", ["channel", "channel_morse", "dense_morse", "box", "channel_bound1",
+                                  "box_big_cell", "couette"])
 def test_engine_step_parity(gpu, name):
     case = CASES[name]()
     e = Engine(case)
@@ -113,8 +133,26 @@ def test_engine_step_parity(gpu, name):
                                            info.noise_counter, 1)
     assert np.allclose(S1["velrhop"][:, 3], rho, rtol=2e-6)
     emax, l2 = util.b11_error(S1["force"].astype(np.float64), F_o, ab, case.consts.ConstExtForce)
-    print(f"{name}: step force max per-particle error {emax:.3e}, relative L2 {l2:.3e}")
+    info1 = e.info()
+    print(f"{name}: step force max per-particle error {emax:.3e}, relative L2 {l2:.3e}, "
+          f"list overflows {info1.nl_overflow}")
     assert emax <= 1e-5
+    assert info1.fault == 0
+    if name in OVERFLOW:
+        assert info1.nl_overflow > 100
+    else:
+        assert info1.nl_overflow == 0
+    if case.consts.surEnergy > 0:
+        # Morse acts (non-zero on many vertices, on both sides) and the
+        # engine's Morse force matches the oracle's
+        agg = extra["agg"]
+        assert np.count_nonzero(np.linalg.norm(agg, axis=1) > 0) > 60
+        scale = np.abs(agg).max()
+        assert np.allclose(S1["aggforce"], agg, rtol=1e-4, atol=2e-5 * scale)
+    if case.nBound1:
+        F_rep = np.zeros((case.N, 3))
+        o.rep_force(pos32, F_rep)
+        assert np.count_nonzero(np.linalg.norm(F_rep, axis=1) > 0) > 200
     if case.nCell:
         assert np.allclose(S1["area"], extra["geo"]["cell_area"], rtol=2e-6)
         assert np.allclose(S1["vol"], extra["geo"]["cell_vol"], rtol=2e-6)
@@ -200,7 +238,7 @@ def test_engine_tecplot_frames(gpu, tmp_path):
 
 
 def test_checkpoint_resume_is_bitwise(gpu, tmp_path):
-    case = channel_case(De=0.02)
+    case = morse_case()
     a = Engine(case)
     a.step(6, graph=True)
     ck = tmp_path / "state.ck"

@@ -19,9 +19,9 @@ def _cases():
     ph_b = util.replace_physics(kT=0.02, De=0.02, r0=0.3, beta=1.5, mcut=1.5, fext=0.03, seed=9)
     ph_c = util.replace_physics(kT=0.005, eta=2.0, seed=3)
     return [
-        util.outer_walls_as_bound1(casegen.channel(24, 12, 2, rbc_diameter=5.0, subdiv=2,
+        util.inner_walls_as_bound1(casegen.channel(24, 12, 2, rbc_diameter=5.0, subdiv=2,
                                                    seed=5, physics=ph_a), ph_a, 6.0),
-        casegen.channel(30, 14, 3, rbc_diameter=5.0, subdiv=2, seed=7, physics=ph_b),
+        util.stacked_pair_channel(30, 14, diameter=5.0, subdiv=2, gap=0.6, physics=ph_b, seed=7),
         casegen.periodic_box(n=14, rbc_diameter=6.0, subdiv=2, seed=11, physics=ph_c),
         casegen.periodic_box(n=10, n_rbc=0, seed=2),   # fluid only
         casegen.couette(n=12, n_rbc=1, rbc_diameter=5.0, shear_rate=0.01, seed=4,

@@ -21,23 +21,15 @@ pytestmark = pytest.mark.gpu
 
 
 def small_channel(noise=True, De=0.0, with_bound1=True, seed=3):
+    """24 x 12 x 12 duct with two RBCs face to face, their rims 0.6 apart
+    (Morse acts between many vertex pairs when De > 0), and half of the
+    innermost wall layer as Bound1 (within h of the first fluid layer)."""
     ph = util.replace_physics(kT=0.01, De=De, r0=0.3, beta=1.5, mcut=1.5, mu0=1.0, kc=0.1,
                               ka=20.0, kd=5.0, kv=50.0, fext=0.02)
-    c = casegen.channel(24, 12, 2, rbc_diameter=5.0, subdiv=2, seed=seed, physics=ph,
-                        noise=noise)
+    c = util.stacked_pair_channel(24, 12, diameter=5.0, subdiv=2, gap=0.6, physics=ph, noise=noise,
+                                  seed=seed)
     if with_bound1:
-        # the outermost wall layer becomes Bound1 (repulsion only)
-        walls = c.coor[:c.nBound2]
-        outer = np.maximum(np.abs(walls[:, 1]), np.abs(walls[:, 2])) > 6.0 + 2.0
-        b1, b2 = walls[outer], walls[~outer]
-        rest = c.coor[c.nBound2:]
-        vel = np.concatenate([np.zeros((b1.shape[0] + b2.shape[0], 3), np.float32),
-                              c.vel[c.nBound2:]])
-        c.coor = np.concatenate([b1, b2, rest]).astype(np.float32)
-        c.vel = vel
-        c.nBound1, c.nBound2 = int(b1.shape[0]), int(b2.shape[0])
-        c.consts = casegen.make_consts(ph, c.nBound1, c.nBound2, c.nFluid, c.nCellPar)
-    # membranes start wrapped into the box for the per-entry tests
+        c = util.inner_walls_as_bound1(c, ph, 6.0)
     return c
 
 
@@ -92,6 +84,38 @@ def test_con_cube_list_bit_exact(gpu):
         assert np.array_equal(cur[sp_s], np.arange(case.N))
         # move particles a little and rebuild from a non-identity order
         rng = np.random.default_rng(rnd)
+        moved = gu.host(g.coor) + rng.uniform(-0.4, 0.4, size=(case.N, 3)).astype(np.float32)
+        moved[:, 0] = np.mod(moved[:, 0], np.float32(case.bbox[1]))
+        g.coor.copy_(gu.dev(moved, np.float32))
+
+
+def test_con_cube_list_fluid_only_bit_exact(gpu):
+    """Without membranes Particle::ConCubeList gathers with SortBasicArrays
+    (Particle.cu:676-677): no nThroBound, no CurPos."""
+    ph = util.replace_physics(kT=0.01, fext=0.02)
+    case = casegen.channel(24, 12, 0, seed=4, physics=ph)
+    assert case.nCell == 0
+    g, (nc, outer, inv, K) = setup(case)
+    cut = np.float32(case.consts.CutRadius)
+    per = list(case.ext.periodic)
+    rng = np.random.default_rng(5)
+    g.velhalf.copy_(gu.dev(rng.normal(size=(case.N, 3)), np.float32))
+    g.force.copy_(gu.dev(rng.normal(size=(case.N, 3)), np.float32))
+    g.press.copy_(gu.dev(rng.normal(size=case.N), np.float32))
+    for rnd in range(2):
+        coor_slot, sp_before = gu.host(g.coor), gu.host(g.sortpart)
+        arrs = {k: gu.host(getattr(g, k)) for k in ("velrhop", "velhalf", "force", "press")}
+        cc_o, key_o, old_o = orc.preconlist(coor_slot, cut, per, case.bbox, inv, nc, sp_before)
+        g.con_cube_list()
+        key_s, sp_s = orc.sort(key_o, sp_before)
+        assert np.array_equal(gu.host(g.cubepart), key_s)
+        assert np.array_equal(gu.host(g.sortpart), sp_s)
+        assert np.array_equal(gu.host(g.be), orc.begin_end(key_s, K))
+        src = old_o[sp_s]
+        assert np.array_equal(gu.host(g.coor), coor_slot[src])
+        assert np.array_equal(gu.host(g.cc), cc_o[src])
+        for k, a in arrs.items():
+            assert np.array_equal(gu.host(getattr(g, k)), a[src]), k
         moved = gu.host(g.coor) + rng.uniform(-0.4, 0.4, size=(case.N, 3)).astype(np.float32)
         moved[:, 0] = np.mod(moved[:, 0], np.float32(case.bbox[1]))
         g.coor.copy_(gu.dev(moved, np.float32))
@@ -169,6 +193,10 @@ def test_fluid_entry_points(gpu, noise):
     F_o = np.zeros((case.N, 3))
     ab = np.zeros(case.N)
     o.sdpd_force(pos, vr3[:, :3].copy(), vr3[:, 3].copy(), press_o, counter, F_o, ab)
+    F_rep = np.zeros((case.N, 3))
+    o.rep_force(pos, F_rep)
+    # the Bound1 layer lies within h of the fluid: repulsion on many particles
+    assert np.count_nonzero(np.linalg.norm(F_rep, axis=1) > 0) > 200
     o.rep_force(pos, F_o, ab)
     o.ext_force(pos, 1, F_o, ab)
     F = gu.id_order(gu.host(g.force), sp).astype(np.float64)
@@ -297,13 +325,14 @@ def test_membrane_entry_points(gpu):
     print(f"membrane force max per-vertex error {e_def.max():.3e}")
     assert e_def.max() <= 1e-5
     assert np.array_equal(hdef, gu.host(defF))
-    # Morse between the two cells
+    # Morse between the two cells (rims 0.6 apart, MaxCut 1.5)
     g.force.zero_()
     aggF = z(M, 3)
     gu.call("gc_CalCellAggForce", Cn, case.N, M, g.coor, aggF, g.cc, g.ncube, g.sortpart, celltab,
             g.outer, g.be, g.force, d["pararea"])
     ab2 = np.zeros(case.N)
     af_o = o.agg_force(pos, None, ab2)
+    assert np.count_nonzero(np.linalg.norm(af_o, axis=1) > 0) > 60
     assert np.allclose(gu.host(aggF), af_o, rtol=1e-5, atol=1e-6 * max(1e-30, np.abs(af_o).max()))
     F = gu.id_order(gu.host(g.force), sp)
     assert np.allclose(F[case.consts.nCellStart:], af_o, rtol=1e-5,

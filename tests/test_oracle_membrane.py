@@ -5,6 +5,11 @@
   energy term (WLC-POW, bending, local/global area, volume);
 * translation and rotation invariance: sum F = 0, sum r x F = 0;
 * the reference state is force-free (k_p balances the WLC spring at l0);
+* SetMechPara's moduli mapping by lattice mathematics: the WLC-POW springs
+  it parametrises give a triangular network the linear shear modulus mu0
+  (energy density mu0 gamma^2 / 2 under simple shear), and its bending
+  constant k_b gives a gently bent triangulated sheet the Helfrich energy
+  density k_c (c - c0)^2 / 2;
 * unwrapping across a periodic face changes nothing;
 * Morse: -dU/dr by central differences, zero force at r0, only between
   different cells, Newton's third law."""
@@ -72,11 +77,13 @@ def _deformed_rbc(seed=0, scale=0.08, subdiv=2, D0=6.0, **phys):
 @pytest.mark.parametrize("term", ["all", "spring", "bending", "local", "global", "volume"])
 def test_membrane_force_is_energy_gradient(term):
     k = dict(mu0=1.0, kc=0.5, kd=3.0, ka=20.0, kv=40.0, kT=0.02)
+    # mu0 = 0 switches the springs off (p = inf, k_p = 0: no force, no
+    # energy); a large mu0 would make them stiff instead
     off = {"spring": dict(kc=0.0, kd=0.0, ka=0.0, kv=0.0),
-           "bending": dict(mu0=1e30, kd=0.0, ka=0.0, kv=0.0),
-           "local": dict(mu0=1e30, kc=0.0, ka=0.0, kv=0.0),
-           "global": dict(mu0=1e30, kc=0.0, kd=0.0, kv=0.0),
-           "volume": dict(mu0=1e30, kc=0.0, kd=0.0, ka=0.0)}.get(term, {})
+           "bending": dict(mu0=0.0, kd=0.0, ka=0.0, kv=0.0),
+           "local": dict(mu0=0.0, kc=0.0, ka=0.0, kv=0.0),
+           "global": dict(mu0=0.0, kc=0.0, kd=0.0, kv=0.0),
+           "volume": dict(mu0=0.0, kc=0.0, kd=0.0, ka=0.0)}.get(term, {})
     k.update(off)
     o, case, pos0, pos, img = _deformed_rbc(**k)
     o.geo(0, pos, img)
@@ -109,6 +116,120 @@ def test_membrane_invariances_and_reference_state():
     # a force-free state is the energy minimum (WLC and POW cancel in slope
     # at l0, not in value)
     assert o.membrane_energy(pos0, img) < o.membrane_energy(pos, img)
+
+
+def test_mech_para_gives_the_shear_modulus():
+    """A stress-free triangular network of central springs with pair energy
+    U(l) has, under simple shear gamma (displacement gradient [[0, g], [0,
+    0]]), the energy density W = sum_k [U(l0 |F a_k|) - U(l0)] / (sqrt(3)/2
+    l0^2) over its three bond directions a_k, and W = mu gamma^2 / 2 defines
+    its linear shear modulus.  U is read from the oracle: every edge of a
+    regular icosahedron has the same rest length, so scaling the
+    icosahedron by s gives E(s) = 30 U(s l0) (springs only).  The moduli
+    mapping passes if mu = mu0 for several rest lengths and moduli (a
+    missing sqrt(3), a (m+1) h for (m+1) g, or a wrong power of l0 in p or
+    k_p fails it by far more than the tolerance)."""
+    V, F = casegen.icosphere(0)
+    g = 1e-3
+    dirs = np.array([[1.0, 0.0], [0.5, math.sqrt(3) / 2], [-0.5, math.sqrt(3) / 2]])
+    Fg = np.array([[1.0, g], [0.0, 1.0]])
+    stretch = np.linalg.norm(dirs @ Fg.T, axis=1)
+    for mu0, R in ((1.0, 1.0), (3.0, 0.6), (0.2, 2.5)):
+        ph = util.replace_physics(mu0=mu0, kc=0.0, kd=0.0, ka=0.0, kv=0.0, kT=0.02)
+        case = _cell_case(R * V, F, ph, center=0.0)
+        o = orc.Oracle(case)
+        pos0 = case.coor.astype(np.float64)
+        img = np.zeros((12, 3), np.int32)
+        o.geo(-1, pos0, img)
+        mp = o.mech_para(pos0, img)
+        l0 = float(mp["l0"].mean())
+        assert np.allclose(mp["l0"], l0, rtol=1e-6)
+        c = pos0.mean(0)
+
+        def U(s):   # energy of one edge at length s l0
+            return o.membrane_energy(c + s * (pos0 - c), img) / 30.0
+
+        W = (sum(U(k) for k in stretch) - 3.0 * U(1.0)) / (math.sqrt(3) / 2 * l0 * l0)
+        mu = 2.0 * W / (g * g)
+        assert mu == pytest.approx(mu0, rel=2e-4), (mu0, R, mu)
+        # stress-free at rest: the slope of U vanishes at l0
+        dU = (U(1 + 1e-5) - U(1 - 1e-5)) / (2e-5 * l0)
+        assert abs(dU) < 1e-6 * mu0 * l0
+
+
+def _hex_patch(n, a=1.0):
+    """Vertices of a regular triangular lattice within n rings of the origin
+    (a hexagon: its three bond directions are equally represented), its
+    consistently oriented triangles and the edge records (a, b, c, d, t1,
+    t2) of its interior edges (0-based)."""
+    idx, P = {}, []
+    for j in range(-n, n + 1):
+        for i in range(-n, n + 1):
+            if abs(i) <= n and abs(j) <= n and abs(i + j) <= n:
+                idx[(i, j)] = len(P)
+                P.append((a * (i + 0.5 * j), a * math.sqrt(3) / 2 * j))
+    T = []
+    for (i, j), v in idx.items():
+        if (i + 1, j) in idx and (i, j + 1) in idx:
+            T.append((v, idx[(i + 1, j)], idx[(i, j + 1)]))
+        if (i + 1, j) in idx and (i + 1, j - 1) in idx:
+            T.append((v, idx[(i + 1, j - 1)], idx[(i + 1, j)]))
+    owner = {}
+    for t, (p, q, r) in enumerate(T):
+        for (u, v, w) in ((p, q, r), (q, r, p), (r, p, q)):
+            owner.setdefault((min(u, v), max(u, v)), []).append((t, w))
+    E = [(u, v, lst[0][1], lst[1][1], lst[0][0], lst[1][0])
+         for (u, v), lst in sorted(owner.items()) if len(lst) == 2]
+    return np.array(P), np.array(T), np.array(E)
+
+
+def test_mech_para_gives_the_bending_rigidity():
+    """Helfrich energy density of a sheet whose curvature changes from c0
+    to c along one direction (a cylinder): k_c (c - c0)^2 / 2.  A regular
+    triangulated hexagonal patch is wrapped isometrically onto cylinders of
+    curvature c0 (the reference state of SetMechPara, whose theta0 it
+    records) and c; per interior edge (area a^2 / (2 sqrt(3)) each, the
+    three lattice directions equally represented) the oracle's bending
+    energy must match the Helfrich density.  This is the continuum limit of
+    sum k_b [1 - cos(theta - theta0)] for small a c, in which k_b = 2 k_c /
+    sqrt(3) (the sheet's energy per area is sqrt(3)/4 k_b (c - c0)^2 for
+    any orientation of the lattice on the cylinder)."""
+    P2, T, E = _hex_patch(12)
+    kc = 0.7
+
+    def wrap(c, axis_angle):
+        ca, sa = math.cos(axis_angle), math.sin(axis_angle)
+        u = ca * P2[:, 0] + sa * P2[:, 1]      # along the cylinder axis
+        w = -sa * P2[:, 0] + ca * P2[:, 1]     # around it
+        X = np.stack([u, np.sin(c * w) / c, (1.0 - np.cos(c * w)) / c], 1)
+        return X + 20.0
+
+    for c0, c1, ang in ((0.002, 0.022, 0.0), (0.004, -0.016, 0.4), (0.003, 0.018, math.pi / 2)):
+        ph = util.replace_physics(mu0=0.0, kc=kc, kd=0.0, ka=0.0, kv=0.0, kT=0.02)
+        pos0 = wrap(c0, ang)
+        case = util.mini_case(pos0, nm=pos0.shape[0], L=(40.0, 40.0, 40.0), physics=ph)
+        nt, ne = T.shape[0], E.shape[0]
+        rec = E.copy()
+        rec[:, :4] += 1
+        rec[:, 4:] += 1
+        case.tri = (T + 1).astype(np.int32)
+        case.edge = rec.astype(np.int32)
+        case.celltab = np.zeros(pos0.shape[0], np.int32)
+        case.beginendtri = np.array([[0, nt - 1]], np.int32)
+        case.shearmod = np.zeros(ne, np.float32)
+        case.bendmod = np.full(ne, kc, np.float32)
+        for k in ("hlarea", "hgarea", "hvol"):
+            setattr(case, k, np.zeros(nt, np.float32))
+        o = orc.Oracle(case)
+        img = np.zeros((pos0.shape[0], 3), np.int32)
+        p0 = case.coor.astype(np.float64)
+        o.geo(-1, p0, img)
+        mp = o.mech_para(p0, img)
+        assert o.membrane_energy(p0, img) == pytest.approx(0.0, abs=1e-12)
+        # the reference positions passed through float32: re-wrap exactly
+        E1 = o.membrane_energy(wrap(c1, ang), img) - o.membrane_energy(p0, img)
+        expect = ne * (1.0 / (2 * math.sqrt(3))) * 0.5 * kc * (c1 - c0) ** 2
+        assert E1 == pytest.approx(expect, rel=1e-2), (c0, c1, ang, E1 / expect, mp["kb"][0])
 
 
 def test_unwrapping_across_periodic_face():

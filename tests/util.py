@@ -79,17 +79,69 @@ def replace_physics(**kw):
     return dataclasses.replace(casegen.Physics(), **kw)
 
 
-def outer_walls_as_bound1(case, physics, half_width: float):
-    """Channel case whose outermost wall layer (beyond half_width + 2 from
-    the axis) becomes Bound1 (repulsion only); the other wall layers stay
-    Bound2.  Rebuilds the load order and the parameter block's id ranges."""
+def inner_walls_as_bound1(case, physics, half_width: float):
+    """Channel case whose innermost wall layer (the one next to the fluid,
+    max(|y|, |z|) < half_width + 1) is split in a checkerboard: half of it
+    becomes Bound1 (repulsion only), within h of the first fluid layer, and
+    the rest of the walls stay Bound2 (some of them with fluid neighbours,
+    so the no-slip wall velocity acts as well).  Rebuilds the load order
+    and the parameter block's id ranges."""
     c = dataclasses.replace(case)
     walls = c.coor[:c.nBound2]
-    outer = np.maximum(np.abs(walls[:, 1]), np.abs(walls[:, 2])) > half_width + 2.0
-    b1, b2 = walls[outer], walls[~outer]
+    inner = np.maximum(np.abs(walls[:, 1]), np.abs(walls[:, 2])) < half_width + 1.0
+    checker = (np.floor(walls).astype(np.int64).sum(axis=1) % 2) == 0
+    pick = inner & checker
+    b1, b2 = walls[pick], walls[~pick]
     rest = c.coor[c.nBound2:]
     c.vel = np.concatenate([np.zeros((walls.shape[0], 3), np.float32), c.vel[c.nBound2:]])
     c.coor = np.concatenate([b1, b2, rest]).astype(np.float32)
     c.nBound1, c.nBound2 = int(b1.shape[0]), int(b2.shape[0])
     c.consts = casegen.make_consts(physics, c.nBound1, c.nBound2, c.nFluid, c.nCellPar)
     return c
+
+
+def stacked_pair_channel(length: int = 24, width: int = 12, diameter: float = 5.0, subdiv: int = 2,
+                         gap: float = 0.6, physics=None, noise: bool = True, seed: int = 3,
+                         name: str = "stacked_pair"):
+    """Square duct (periodic along x, three Bound2 wall layers) holding two
+    biconcave RBCs face to face along the flow axis, their rims `gap` apart
+    (the closest pair of vertices of different cells), so that a Morse
+    potential with MaxCut > gap acts between many vertex pairs from the
+    first step (a rouleau of two)."""
+    from scipy.spatial import cKDTree
+    ph = physics or casegen.Physics()
+    rng = np.random.default_rng(seed)
+    W, Lx = float(width), float(length)
+    half = 0.5 * W
+    bbox = np.array([0, Lx, -half, half, -half, half], np.float32)
+    ext = abi.default_ext()
+    ext.noise_on = int(noise)
+    pts = casegen._lattice([0, -half - 3, -half - 3], [Lx, half + 3, half + 3], 1.0)
+    sd = np.maximum(np.abs(pts[:, 1]), np.abs(pts[:, 2])) - half
+    fluid, walls = pts[sd < 0], pts[(sd >= 0) & (sd < 3)]
+    V0, F0 = casegen.icosphere(subdiv)
+    disc = casegen.biconcave(V0, diameter)
+    # axis of the disc along x
+    disc = disc @ np.array([[0.0, 0.0, 1.0], [0.0, 1.0, 0.0], [-1.0, 0.0, 0.0]]).T
+    tree = cKDTree(disc)
+    lo, hi = 0.0, diameter
+    for _ in range(60):   # centre separation s with min vertex distance = gap
+        s = 0.5 * (lo + hi)
+        d = tree.query(disc + np.array([s, 0.0, 0.0]), k=1)[0].min()
+        lo, hi = (s, hi) if d < gap else (lo, s)
+    s = 0.5 * (lo + hi)
+    c0 = np.array([0.5 * Lx - 0.5 * s, 0.0, 0.0])
+    cells = [disc + c0, disc + c0 + np.array([s, 0.0, 0.0])]
+    E0 = casegen.edge_records(F0)
+    return casegen._assemble(name, ph, ext, bbox, np.zeros((0, 3)), walls, fluid, cells, F0, E0, rng,
+                             0.1, 1.0)
+
+
+def morse_pairs(case, pos=None):
+    """Number of membrane-vertex pairs of different cells closer than MaxCut."""
+    from scipy.spatial import cKDTree
+    nc = int(case.consts.nCellStart)
+    X = (case.coor if pos is None else pos)[nc:].astype(np.float64)
+    pairs = cKDTree(X).query_pairs(float(case.consts.MaxCut), output_type="ndarray")
+    ct = case.celltab
+    return int(np.count_nonzero(ct[pairs[:, 0]] != ct[pairs[:, 1]]))
