@@ -26,9 +26,13 @@ SOURCES = ["gc_abi.hip", "gc_sdpd.hip", "gc_membrane.hip", "gc_engine.hip", "gc_
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-# Packed-FP32 instructions are disabled for all device code.
+# Packed-FP32 instructions are disabled for all device code.  Multiply-adds
+# are fused only within one source expression (-ffp-contract=on), never
+# across statements: the rounding of a shared device function is then the
+# same in every kernel it is inlined into, which the bit-identical launch
+# variants (e.g. the per-cell and three-pass membrane forms) rely on.
 HIP_FLAGS = [
-    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=on",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]

@@ -520,7 +520,7 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
     auto consider = [&](auto SH, int j, float4 pj, float sx, float sy, float sz, int code) {
         float rx, ry, rz;
         if constexpr (decltype(SH)::value) {
-            rx = pi.x - (pj.x + sx); ry = pi.y - (pj.y + sy); rz = pi.z - (pj.z + sz);
+            rx = pdiff(pi.x, pj.x, sx); ry = pdiff(pi.y, pj.y, sy); rz = pdiff(pi.z, pj.z, sz);
         } else {
             rx = pi.x - pj.x; ry = pi.y - pj.y; rz = pi.z - pj.z;
         }
@@ -631,7 +631,7 @@ __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep*
         float rx, ry, rz;
         if (__ballot(code & 63)) {   // an image neighbour in the wave (x + 0 is x: same bits)
             float sx = code_shift(code, L0), sy = code_shift(code >> 2, L1), sz = code_shift(code >> 4, L2);
-            rx = pi.x - (pj.x + sx); ry = pi.y - (pj.y + sy); rz = pi.z - (pj.z + sz);
+            rx = pdiff(pi.x, pj.x, sx); ry = pdiff(pi.y, pj.y, sy); rz = pdiff(pi.z, pj.z, sz);
         } else {
             rx = pi.x - pj.x; ry = pi.y - pj.y; rz = pi.z - pj.z;
         }
@@ -688,7 +688,7 @@ __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep*
             const int code = shift_code(sx, sy, sz);
             for (int j = jb; j < je; ++j) {
                 float4 pj = posid[j];
-                float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
+                float rx = pdiff(pi.x, pj.x, sx), ry = pdiff(pi.y, pj.y, sy), rz = pdiff(pi.z, pj.z, sz);
                 float r2 = rx * rx + ry * ry + rz * rz;
                 if (r2 < P.h2 && j != p) pair((j << 6) | code);
             }
@@ -734,7 +734,7 @@ __global__ __launch_bounds__(BS) void k2_agg(int nm, GcRep R0, const GcRep* __re
         float4 pj = posid[j];
         int idj = __float_as_int(pj.w);
         if (idj < P.nc) return;
-        float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
+        float rx = pdiff(pi.x, pj.x, sx), ry = pdiff(pi.y, pj.y, sy), rz = pdiff(pi.z, pj.z, sz);
         float r2 = rx * rx + ry * ry + rz * rz;
         if (r2 >= P.mcut2 || r2 == 0.f || celltab[idj - P.nc] == cell) return;
         float ir = __builtin_amdgcn_rsqf(r2);
@@ -768,7 +768,8 @@ __global__ __launch_bounds__(BS) void k2_agg(int nm, GcRep R0, const GcRep* __re
                 if (idj < P.nc) continue;
                 const float sx = code_shift(code[u], R.L[0]), sy = code_shift(code[u] >> 2, R.L[1]),
                             sz = code_shift(code[u] >> 4, R.L[2]);
-                const float rx = pi.x - (pj[u].x + sx), ry = pi.y - (pj[u].y + sy), rz = pi.z - (pj[u].z + sz);
+                const float rx = pdiff(pi.x, pj[u].x, sx), ry = pdiff(pi.y, pj[u].y, sy),
+                            rz = pdiff(pi.z, pj[u].z, sz);
                 const float r2 = rx * rx + ry * ry + rz * rz;
                 if (r2 >= P.mcut2 || r2 == 0.f || cj[u] == cell) continue;
                 const float ir = __builtin_amdgcn_rsqf(r2);
@@ -811,7 +812,7 @@ __global__ __launch_bounds__(BS) void k2_wallvel(int n, GcRep R0, const GcRep* _
     visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
         for (int j = jb; j < je; ++j) {
             float4 pj = posid[j];
-            float rx = pi.x - (pj.x + sx), ry = pi.y - (pj.y + sy), rz = pi.z - (pj.z + sz);
+            float rx = pdiff(pi.x, pj.x, sx), ry = pdiff(pi.y, pj.y, sy), rz = pdiff(pi.z, pj.z, sz);
             float r2 = rx * rx + ry * ry + rz * rz;
             if (r2 < P.h2 && __float_as_int(pj.w) >= P.nf) {
                 float q = __builtin_amdgcn_sqrtf(r2) * ih, om = 1.0f - q;

@@ -141,6 +141,19 @@ __device__ __forceinline__ void pair_normals(const GcParams& P, uint32_t ida, ui
 
 // Cube coordinate of one axis, FP32 with no contraction so the host oracle
 // reproduces it bit for bit: floor(((x - lo) + off) * invw), clamped.
+// x_i - (x_j + s) for a periodic image shift s in {-L, 0, +L} of a
+// neighbour j, with a single rounding at the end.  The shift is applied to
+// the coordinate next to the far face of the box (with lo = 0 it lies in
+// [L/2, L], so subtracting L is exact by Sterbenz's lemma): s = -L means x_j
+// is near L, s = +L means x_i is.  Forming x_j + L instead would round to
+// the spacing of floats near L (7.6e-6 at L = 230), an error in the pair
+// distance of the same size for every pair across a periodic face.
+// s = 0 gives x_i - x_j.
+__device__ __forceinline__ float pdiff(float xi, float xj, float s)
+{
+    return s > 0.0f ? (xi - s) - xj : xi - (xj + s);
+}
+
 __device__ __forceinline__ int cube_coord(float x, float lo, float off, float invw, int nc)
 {
 #pragma clang fp contract(off)

@@ -991,7 +991,7 @@ __global__ __launch_bounds__(BS) void k_agg_force(int n, GcParams P, const tfloa
             int idj = sortpart[j];
             if (idj < P.nc || celltab[idj - P.nc] == cell) continue;
             tfloat3 xj = coor[j];
-            float rx = xi.x - (xj.x + sx), ry = xi.y - (xj.y + sy), rz = xi.z - (xj.z + sz);
+            float rx = pdiff(xi.x, xj.x, sx), ry = pdiff(xi.y, xj.y, sy), rz = pdiff(xi.z, xj.z, sz);
             float r2 = rx * rx + ry * ry + rz * rz;
             if (r2 >= P.mcut2 || r2 == 0.f) continue;
             float r = sqrtf(r2);

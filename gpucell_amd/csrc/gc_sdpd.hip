@@ -88,7 +88,7 @@ __global__ __launch_bounds__(BS) void k_density(int n, GcParams P, const tfloat3
         for (int j = jb; j < je; ++j) {
             if (j == p || sortpart[j] < P.nb2) continue;
             tfloat3 xj = coor[j];
-            float rx = xi.x - (xj.x + sx), ry = xi.y - (xj.y + sy), rz = xi.z - (xj.z + sz);
+            float rx = pdiff(xi.x, xj.x, sx), ry = pdiff(xi.y, xj.y, sy), rz = pdiff(xi.z, xj.z, sz);
             float r2 = rx * rx + ry * ry + rz * rz;
             if (r2 < P.h2) d += lucy_w(sqrtf(r2), P);
         }
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(BS) void k_bound2_vel(int n, GcParams P, const tflo
         for (int j = jb; j < je; ++j) {
             if (sortpart[j] < P.nf) continue;
             tfloat3 xj = coor[j];
-            float rx = xi.x - (xj.x + sx), ry = xi.y - (xj.y + sy), rz = xi.z - (xj.z + sz);
+            float rx = pdiff(xi.x, xj.x, sx), ry = pdiff(xi.y, xj.y, sy), rz = pdiff(xi.z, xj.z, sz);
             float r2 = rx * rx + ry * ry + rz * rz;
             if (r2 < P.h2) {
                 float w = lucy_w(sqrtf(r2), P);
@@ -174,7 +174,7 @@ __global__ __launch_bounds__(BS) void k_sdpd_force(int n, GcParams P, const tflo
             int idj = sortpart[j];
             if (idj < P.nb2) continue;
             tfloat3 xj = coor[j];
-            float rx = xi.x - (xj.x + sx), ry = xi.y - (xj.y + sy), rz = xi.z - (xj.z + sz);
+            float rx = pdiff(xi.x, xj.x, sx), ry = pdiff(xi.y, xj.y, sy), rz = pdiff(xi.z, xj.z, sz);
             float r2 = rx * rx + ry * ry + rz * rz;
             if (r2 >= P.h2 || r2 == 0.f) continue;
             float r = sqrtf(r2);
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(BS) void k_rep_force(int n, GcParams P, const tfloa
         for (int j = jb; j < je; ++j) {
             if (sortpart[j] >= P.nb2) continue;
             tfloat3 xj = coor[j];
-            float rx = xi.x - (xj.x + sx), ry = xi.y - (xj.y + sy), rz = xi.z - (xj.z + sz);
+            float rx = pdiff(xi.x, xj.x, sx), ry = pdiff(xi.y, xj.y, sy), rz = pdiff(xi.z, xj.z, sz);
             float r2 = rx * rx + ry * ry + rz * rz;
             if (r2 >= P.h2 || r2 == 0.f) continue;
             float k = P.krep * lucy_f(sqrtf(r2), P);
