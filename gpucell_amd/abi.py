@@ -123,6 +123,7 @@ PROTOTYPES = {
     "gc_ConfigDomain": (I, [P, C.c_float, P, P, P, P, P]),
     "gc_PreConList": (I, [I, I, P, P, P, P, P, P, P, P]),
     "gc_Sort": (I, [I, P, P]),
+    "gc_SortNoAlloc": (I, [I, P, P]),
     "gc_CalBeginEndCube": (I, [I, I, P, P]),
     "gc_SortBasicArrays": (I, [I, P, P, P, P, P, P, P, P]),
     "gc_SortArraysWithCells": (I, [I, P, P, P, P, P, P, P, P, P, P]),

@@ -425,18 +425,19 @@ __global__ void k_min_int(int n, const int* __restrict__ a, int* __restrict__ ou
 // square of the population).
 static const int GC_MAX_CUBE_POP = 8192;
 
-extern "C" int gc_Sort(int n, int* keys, int* values)
+// Buffers of a sort of n elements with keys in [0, K): red holds 4 ints,
+// cnt and offs K + 1, rank, tk and tv n each, ws scan_ws_ints(K) (NULL:
+// library scratch for the scan).
+struct SortBufs {
+    int *red = nullptr, *cnt = nullptr, *offs = nullptr, *rank = nullptr, *tk = nullptr, *tv = nullptr;
+    int* ws = nullptr;
+};
+
+// Smallest and largest key (one sync); K = largest + 1.
+static int sort_key_range(int n, const int* keys, int* red, hipStream_t s, int* K)
 {
-    if (n < 0) return set_error(GC_ERR_INVALID, "Sort: n < 0");
-    if (n == 0) return GC_OK;
-    if (!keys || !values) return set_error(GC_ERR_INVALID, "Sort: null pointer");
-    hipStream_t s = stream();
-    void* p;
-    int st = scratch(0, sizeof(int) * 4, &p);
-    if (st) return st;
-    int* red = (int*)p;
     int init[2] = {-1, 0x7fffffff};
-    st = hip_check(hipMemcpyAsync(red, init, sizeof(init), hipMemcpyHostToDevice, s), "Sort init");
+    int st = hip_check(hipMemcpyAsync(red, init, sizeof(init), hipMemcpyHostToDevice, s), "Sort init");
     if (st) return st;
     k_max_int<<<blocks_for(n, 256), 256, 0, s>>>(n, keys, red);
     k_min_int<<<blocks_for(n, 256), 256, 0, s>>>(n, keys, red + 1);
@@ -447,31 +448,120 @@ extern "C" int gc_Sort(int n, int* keys, int* values)
     st = hip_check(hipStreamSynchronize(s), "Sort sync");
     if (st) return st;
     if (mm[1] < 0) return set_error(GC_ERR_INVALID, "Sort: negative key");
-    int K = mm[0] + 1;
+    *K = mm[0] + 1;
+    return GC_OK;
+}
+
+// Population check (largest key count, one sync), then the counting sort
+// by (key, value) in place.
+static int sort_run(int n, int K, int* keys, int* values, const SortBufs& b, hipStream_t s)
+{
+    k_fill_int<<<blocks_for(K + 1, 256), 256, 0, s>>>(b.cnt, 0, K + 1);
+    k_hist<<<blocks_for(n, 256), 256, 0, s>>>(n, keys, b.cnt, b.rank);
+    int neg1 = -1;
+    int st = hip_check(hipMemcpyAsync(b.red, &neg1, sizeof(int), hipMemcpyHostToDevice, s), "Sort init2");
+    if (st) return st;
+    k_max_int<<<blocks_for(K, 256), 256, 0, s>>>(K, b.cnt, b.red);
+    GC_LAUNCH_CHECK("Sort population");
+    int maxpop = 0;
+    st = hip_check(hipMemcpyAsync(&maxpop, b.red, sizeof(int), hipMemcpyDeviceToHost, s), "Sort D2H2");
+    if (st) return st;
+    st = hip_check(hipStreamSynchronize(s), "Sort sync2");
+    if (st) return st;
+    if (maxpop > GC_MAX_CUBE_POP)
+        return set_error(GC_ERR_GRID, "Sort: a key holds more than 8192 elements (over-full cube)");
+    return counting_sort(n, K, keys, values, keys, values, b.cnt, b.offs, b.rank, b.tk, b.tv, s, b.ws);
+}
+
+extern "C" int gc_Sort(int n, int* keys, int* values)
+{
+    if (n < 0) return set_error(GC_ERR_INVALID, "Sort: n < 0");
+    if (n == 0) return GC_OK;
+    if (!keys || !values) return set_error(GC_ERR_INVALID, "Sort: null pointer");
+    hipStream_t s = stream();
+    SortBufs b;
+    void* p;
+    int st = scratch(0, sizeof(int) * 4, &p);
+    if (st) return st;
+    b.red = (int*)p;
+    int K = 0;
+    if ((st = sort_key_range(n, keys, b.red, s, &K))) return st;
     void *pc, *po, *pr, *ptk, *ptv;
     if ((st = scratch(1, sizeof(int) * (size_t)(K + 1), &pc))) return st;
     if ((st = scratch(2, sizeof(int) * (size_t)(K + 1), &po))) return st;
     if ((st = scratch(3, sizeof(int) * (size_t)n, &pr))) return st;
     if ((st = scratch(4, sizeof(int) * (size_t)n, &ptk))) return st;
     if ((st = scratch(5, sizeof(int) * (size_t)n, &ptv))) return st;
-    int* cnt = (int*)pc;
-    // population check: largest histogram bin
-    k_fill_int<<<blocks_for(K + 1, 256), 256, 0, s>>>(cnt, 0, K + 1);
-    k_hist<<<blocks_for(n, 256), 256, 0, s>>>(n, keys, cnt, (int*)pr);
-    int neg1 = -1;
-    st = hip_check(hipMemcpyAsync(red, &neg1, sizeof(int), hipMemcpyHostToDevice, s), "Sort init2");
+    b.cnt = (int*)pc; b.offs = (int*)po; b.rank = (int*)pr; b.tk = (int*)ptk; b.tv = (int*)ptv;
+    return sort_run(n, K, keys, values, b, s);
+}
+
+// ---------------------------------------------------------------------------
+// gc_SortNoAlloc: gc_Sort with every buffer in one fixed device array.
+// Sizes: n <= GC_NOALLOC_MAX_N elements, keys < GC_NOALLOC_MAX_K.  The array
+// holds the 4 range/population words, the key counts and offsets (K + 1
+// each), the ranks and the two scatter arrays (n each) and the scan
+// workspace, laid out from its start for each call.
+namespace noalloc {
+constexpr int SCAN_WS = 16384;   // >= scan_ws_ints(GC_NOALLOC_MAX_K) = 8205 (checked per call)
+constexpr size_t INTS = 4 + 2 * ((size_t)GC_NOALLOC_MAX_K + 1) + 3 * (size_t)GC_NOALLOC_MAX_N + SCAN_WS;
+// One copy per GPU, for the life of the process (84 MB).  In host code this
+// name is a stand-in, not a device address: the address is taken once per
+// device with hipGetSymbolAddress (noalloc_base) and only that is passed on.
+static __device__ __align__(256) int scratch[INTS];
+}  // namespace noalloc
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static void* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// The current device's address of `scratch`, looked up on the first call
+// on that device.
+int noalloc_base(int** base)
+{
+    int dev = 0;
+    int st = hip_check(hipGetDevice(&dev), "SortNoAlloc: hipGetDevice");
     if (st) return st;
-    k_max_int<<<blocks_for(K, 256), 256, 0, s>>>(K, cnt, red);
-    GC_LAUNCH_CHECK("Sort population");
-    int maxpop = 0;
-    st = hip_check(hipMemcpyAsync(&maxpop, red, sizeof(int), hipMemcpyDeviceToHost, s), "Sort D2H2");
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return set_error(GC_ERR_INVALID, "SortNoAlloc: device index");
+    if (!g_noalloc_ptr[dev]) {
+        void* ptr = nullptr;
+        st = hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(noalloc::scratch)), "SortNoAlloc: hipGetSymbolAddress");
+        if (st) return st;
+        if (!ptr) return set_error(GC_ERR_HIP, "SortNoAlloc: no device address for the scratch array");
+        g_noalloc_ptr[dev] = ptr;
+    }
+    *base = (int*)g_noalloc_ptr[dev];
+    return GC_OK;
+}
+}  // namespace
+
+extern "C" int gc_SortNoAlloc(int n, int* keys, int* values)
+{
+    if (n < 0) return set_error(GC_ERR_INVALID, "SortNoAlloc: n < 0");
+    if (n == 0) return GC_OK;
+    if (!keys || !values) return set_error(GC_ERR_INVALID, "SortNoAlloc: null pointer");
+    if (n > GC_NOALLOC_MAX_N)
+        return set_error(GC_ERR_INVALID, "SortNoAlloc: more than GC_NOALLOC_MAX_N elements");
+    hipStream_t s = stream();
+    int* base = nullptr;
+    int st = noalloc_base(&base);
     if (st) return st;
-    st = hip_check(hipStreamSynchronize(s), "Sort sync2");
-    if (st) return st;
-    if (maxpop > GC_MAX_CUBE_POP)
-        return set_error(GC_ERR_GRID, "Sort: a key holds more than 8192 elements (over-full cube)");
-    return counting_sort(n, K, keys, values, keys, values, cnt, (int*)po, (int*)pr,
-                         (int*)ptk, (int*)ptv, s);
+    SortBufs b;
+    b.red = base;
+    int K = 0;
+    if ((st = sort_key_range(n, keys, b.red, s, &K))) return st;
+    if (K > GC_NOALLOC_MAX_K)
+        return set_error(GC_ERR_INVALID, "SortNoAlloc: a key >= GC_NOALLOC_MAX_K");
+    if (scan_ws_ints(K) > (size_t)noalloc::SCAN_WS)
+        return set_error(GC_ERR_INVALID, "SortNoAlloc: scan workspace too small");
+    int* q = base + 4;
+    b.cnt = q; q += K + 1;
+    b.offs = q; q += K + 1;
+    b.rank = q; q += n;
+    b.tk = q; q += n;
+    b.tv = q; q += n;
+    b.ws = q;
+    return sort_run(n, K, keys, values, b, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -83,6 +83,20 @@ int gc_PreConList(int n, int cutradius, tfloat3* d_totalcoor, float* d_bboxsph,
  * Deterministic (counting sort); synchronises once to size its histogram. */
 int gc_Sort(int n, int* keys, int* values);
 
+/* gc_Sort without any device allocation.  No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same results as gc_Sort.  Its
+ * scratch (key range words, key counts and offsets, ranks, scatter arrays,
+ * scan workspace) is one fixed device array per GPU of about 84 MB, held
+ * until the process ends; sizes are limited to n <= GC_NOALLOC_MAX_N
+ * elements and keys < GC_NOALLOC_MAX_K, larger ones return GC_ERR_INVALID.
+ * Every call on a GPU shares that array: calls that overlap on one GPU
+ * (two host threads, or two streams) race.  Like gc_Sort it synchronises
+ * the library stream twice (key range, population check). */
+#define GC_NOALLOC_MAX_N (1 << 22)
+#define GC_NOALLOC_MAX_K (1 << 22)
+int gc_SortNoAlloc(int n, int* keys, int* values);
+
 /* cudiv::CalBeginEndCube, src/ConfigDomain.cuh:13; call Particle.cu:672.
  * beginendcube[k] = [first slot, last slot + 1) of cube k in the sorted
  * key array; an empty cube has x == y (== the slot where it would start). */
