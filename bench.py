@@ -1,6 +1,6 @@
 """SDPD step benchmark on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--case suspension_1m]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--case suspension_10m]
 
 A step is one Particle::UpdatePar (neighbour build, forces, integration) of
 the whole suspension, replayed from a HIP graph.  Inputs are generated on
@@ -38,6 +38,8 @@ KERNEL_BYTES_PER_PARTICLE = {"integrate+key": 56.0, "sort+gather+predict": 142.3
                              "density+wall velocity": 16.0, "sdpd force": 72.0}
 KERNEL_NAMES = {"integrate+key": "k2_integrate", "sort+gather+predict": "k2_sortgather",
                 "density+wall velocity": "k2_density", "sdpd force": "k2_force"}
+# autotuner family of each pass (its chosen shape names the launch variant)
+KERNEL_FAMILY = {"integrate+key": "integrate", "density+wall velocity": "density", "sdpd force": "sdpd force"}
 # the bound of the neighbour-loop passes (instruction issue, not HBM)
 KERNEL_BOUND_NOTE = {
     "density+wall velocity": "VALU issue in the 27-cube scan (~95 candidates per particle, lane imbalance)",
@@ -74,6 +76,17 @@ def make_case(name):
     raise SystemExit(f"unknown case {name}")
 
 
+def cpu_model():
+    """The host CPU's model string (/proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(case, seconds_target=15.0):
     """FP64 OpenMP oracle on the same workload: as many whole steps as fit
     in ~seconds_target (at least one), from the same initial state."""
@@ -94,7 +107,7 @@ def cpu_baseline(case, seconds_target=15.0):
     steps = n if n > 0 else 1
     dt = (t3 - t2) if n > 0 else per
     return {"value": case.N * steps / dt, "unit": "particle-steps/s", "cores": threads,
-            "kind": "port",
+            "cpu_model": cpu_model(), "kind": "port",
             "sample": f"{steps} full steps of the same {case.N}-particle case (FP64 oracle, "
                       f"{threads} OpenMP threads, after 1 untimed step)"}
 
@@ -139,7 +152,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=400)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--case", default="suspension_1m")
+    ap.add_argument("--case", default="suspension_10m",
+                    help="suspension_10m (default: 10M particles + 2000 RBCs, the largest single-GPU "
+                         "workload), suspension_1m, stenosis_2m, ensemble64 or box32")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--eager", action="store_true", help="launch kernels without a graph")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -211,10 +226,17 @@ def main():
     dom = max(("integrate+key", "density+wall velocity", "sdpd force"), key=lambda k: prof.get(k, 0.0))
     dom_us = prof[dom]
     achieved = KERNEL_BYTES_PER_PARTICLE[dom] * N / (dom_us * 1e-6) / 1e9
-    traffic = None
+    # measured HBM bytes per launch from the committed counter profile of this
+    # case, used only when it was taken on the launch variant this run uses
+    traffic, traffic_variant = None, None
     pmc = os.path.join(ROOT, "profiles", f"traffic_{KERNEL_NAMES[dom]}_{a.case}.json")
+    used = None if tune_rows is None else {r["family"]: r["shape"] for r in tune_rows if r["chosen"]}
     if os.path.exists(pmc):
-        traffic = json.load(open(pmc)).get("bytes_per_launch")
+        tj = json.load(open(pmc))
+        traffic_variant = tj.get("variant")
+        fam = KERNEL_FAMILY.get(dom)
+        if used is None or fam is None or used.get(fam) == tj.get("shape"):
+            traffic = tj.get("bytes_per_launch")
     # instruction-issue roofline of the dominant kernel: VALU instructions per
     # wave from the committed counter profile of this case (rocprofv3 SQ
     # counters), 4 cycles per wave64 instruction on each of 1024 SIMDs
@@ -251,7 +273,7 @@ def main():
                    "graph": graph, "finite": finite},
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "avg_us": dom_us,
+                     "traffic": traffic, "traffic_variant": traffic_variant, "avg_us": dom_us,
                      "algorithmic_bytes_per_particle": KERNEL_BYTES_PER_PARTICLE[dom],
                      "note": KERNEL_BOUND_NOTE.get(dom)},
         "valu_roofline": valu,
