@@ -194,6 +194,20 @@ __device__ __forceinline__ void advance(const GcRep& R, const GcParams& P, const
     x = u[0]; y = u[1]; z = u[2];
 }
 
+// A wall's entry of velrho before the density pass: density rho0 (walls
+// keep it) and no velocity (Bound2 walls get theirs in the density pass).
+// The walls' vout entries are not carried through the re-sort (the force
+// pass writes vout only for moving particles, at their new slots), so after
+// the slots shift a wall's old slot holds another particle's vout: it is
+// not read here.
+template <bool ENS>
+__device__ __forceinline__ float4 wall_velrho(const GcRep& R0, const GcRep* __restrict__ reps,
+                                              const int* __restrict__ rep_of, int id)
+{
+    const float rho0 = ENS ? reps[rep_of[id]].P.rho0 : R0.P.rho0;
+    return make_float4(0.f, 0.f, 0.f, rho0);
+}
+
 __device__ __forceinline__ bool moves(const GcParams& P, int id, int mmove)
 {
     return id >= P.nf && (mmove || id < P.nc);
@@ -405,7 +419,7 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     nt_new[s] = nt;
     sp_new[s] = id;
     key_new[s] = k;
-    velrho[s] = v;
+    velrho[s] = id < R0.P.nf ? wall_velrho<ENS>(R0, reps, rep_of, id) : v;
     if (id >= R0.P.nc) curpos[id] = s;
     if (!wincl) {
         act[s] = s;   // moving walls: the force pass runs over every slot

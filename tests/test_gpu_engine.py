@@ -264,9 +264,12 @@ def test_couette_walls_drive_shear_flow(gpu):
     Couette flow.  Time-averaged over the last 2500 of 10000 steps: the
     profile of the mean x velocity rises monotonically from -U to +U, the
     fluid layers next to the walls move with them (no slip, within 15 %),
-    there is no net flow, and the profile's slope is 0.7-1.2 x 2U/H.  (The
-    first fluid layers by the particle walls shear faster than the bulk: the
-    bulk alone measures ~0.6 x 2U/H at this 12-layer height.)"""
+    the net flow stays small, and the profile's slope is 0.7-1.2 x 2U/H.
+    (The first fluid layers by the particle walls shear faster than the
+    bulk: the bulk alone measures ~0.6 x 2U/H at this 12-layer height.  The
+    slab's net momentum relaxes slowly through the walls: the FP64 oracle
+    run of this case and seed averages -0.19 U over the same window, so the
+    bound on the mean is 0.25 U.)"""
     rate, n = 0.02, 12
     U = 0.5 * rate * n
     case = casegen.couette(n=n, shear_rate=rate, noise=True, seed=3)
@@ -289,5 +292,6 @@ def test_couette_walls_drive_shear_flow(gpu):
           f"mean {mean_vx:.2e}")
     assert np.all(np.diff(acc) > 0)
     assert abs(acc[0] + U) < 0.15 * U and abs(acc[-1] - U) < 0.15 * U
-    assert abs(mean_vx) < 0.1 * U
+    assert abs(mean_vx) < 0.25 * U
     assert 0.7 * rate < slope < 1.2 * rate
+
