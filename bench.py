@@ -127,8 +127,10 @@ def surrogate_leg(case, case_name):
     t0 = time.perf_counter()
     box = casegen.periodic_box()
     src = case if case_name == "suspension_1m" else casegen.suspension_1m()
-    X1, Y1 = substepper.collect([box], steps=150, every=1, warmup=5)
-    X2, Y2 = substepper.collect([src], steps=56, every=4, warmup=5,
+    # rows from relaxed suspensions, as the comparison below starts from
+    # one (substepper.compare, relax)
+    X1, Y1 = substepper.collect([box], steps=150, every=1, warmup=150)
+    X2, Y2 = substepper.collect([src], steps=56, every=4, warmup=150,
                                 max_rows=14 * src.nCellPar)
     del src
     X, Y = torch.cat([X1, X2]), torch.cat([Y1, Y2])

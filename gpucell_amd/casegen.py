@@ -348,13 +348,24 @@ def _remove_near(points: np.ndarray, verts: np.ndarray, dist: float) -> np.ndarr
     return ~np.isfinite(d)
 
 
+# Fluid sites closer than this (in lattice spacings) to a membrane vertex are
+# removed.  The membrane vertices count in every neighbour's number density,
+# so the fluid next to a membrane must thin out by about as many particles
+# as the vertices add: with 0.5 the fluid inside a 642-vertex cell of
+# diameter 16-20 starts ~10 % denser than outside and the cell inflates by
+# several per cent over the first ~100 steps (FP64 oracle runs of the 32^3
+# box); with 0.7 the inside and outside densities agree to ~1 % and the
+# volume stays within ~1 % of its initial value.
+MEMBRANE_CLEARANCE = 0.7
+
+
 def _assemble(name, ph, ext, bbox, bound1, bound2, fluid, cells, tri0, edge0, rng, jitter, dx):
     nb1, nb2 = bound1.shape[0], bound2.shape[0]
     if jitter > 0:
         fluid = fluid + rng.uniform(-jitter * dx, jitter * dx, size=fluid.shape)
     mverts = np.concatenate(cells, axis=0) if cells else np.zeros((0, 3))
     if cells:
-        keep = _remove_near(fluid, mverts, 0.5 * dx)
+        keep = _remove_near(fluid, mverts, MEMBRANE_CLEARANCE * dx)
         fluid = fluid[keep]
     nfl = fluid.shape[0]
     nv = cells[0].shape[0] if cells else 0
@@ -390,11 +401,15 @@ def _assemble(name, ph, ext, bbox, bound1, bound2, fluid, cells, tri0, edge0, rn
                 hvol=np.full(ncell * T, ph.kv, np.float32))
 
 
-def periodic_box(n: int = 32, n_rbc: int = 1, rbc_diameter: float = 12.0, seed: int = 20261203,
+def periodic_box(n: int = 32, n_rbc: int = 1, rbc_diameter: float = 16.0, seed: int = 20261203,
                  physics: Physics | None = None, jitter: float = 0.1, subdiv: int = 3,
                  reverse_poiseuille: bool = True, noise: bool = True) -> Case:
     """Fully periodic n^3 box of unit-spaced fluid with `n_rbc` RBCs
-    (n=32, one RBC: the 32x32x32 / one-RBC reverse-Poiseuille case)."""
+    (n=32, one RBC: the 32x32x32 / one-RBC reverse-Poiseuille case).  The
+    default diameter 16 puts the 642 vertices of a 3x subdivided mesh about
+    one lattice spacing apart; a finer membrane counts for more than the
+    fluid it displaces in its neighbours' densities, and a 12-wide cell
+    (spacing ~0.7) swells its area by ~30 % in the first 150 steps."""
     ph = physics or Physics()
     rng = np.random.default_rng(seed)
     L = float(n)
@@ -529,17 +544,19 @@ def mesh_volume(V, F):
 
 
 def suspension_1m(seed: int = 20261203, physics: Physics | None = None, noise: bool = True) -> Case:
-    """1M fluid+wall particles (1,001,880 lattice sites), 100 RBCs at 20 %
-    hematocrit, 3 wall layers: a 230 x 60 x 60 duct (66 x 66 with walls)."""
-    return channel(230, 60, 100, hematocrit=0.20, seed=seed, physics=physics, noise=noise,
+    """1M fluid+wall particles (1,015,188 of 1,097,712 lattice sites remain
+    around the cells), 100 RBCs at 20 % hematocrit (diameter 21.0), 3 wall
+    layers: a 252 x 60 x 60 duct (66 x 66 with walls)."""
+    return channel(252, 60, 100, hematocrit=0.20, seed=seed, physics=physics, noise=noise,
                    name="suspension_1m_100rbc")
 
 
 def stenosis_2m(seed: int = 20261203, physics: Physics | None = None, noise: bool = True) -> Case:
-    """~1.9M particles, 300 RBCs, 50 % cosine stenosis, Morse aggregation
-    De = 0.3 kT, r0 = 0.3, beta = 1.5: a 300 x 80 x 80 duct (86 x 86 with
-    walls).  The hematocrit is the highest of 30 %, 27 %, ... at which the
-    discs fit on aligned lattice slots (18 % with the default seed)."""
+    """2M fluid+wall particles (2,062,828) and 300 RBCs, 50 % cosine
+    stenosis, Morse aggregation De = 0.3 kT, r0 = 0.3, beta = 1.5: a
+    380 x 80 x 80 duct (86 x 86 with walls).  The hematocrit is the highest
+    of 30 %, 27 %, ... at which the discs fit on aligned lattice slots
+    (15 % with the default seed: the throat limits the slots)."""
     ph = physics or Physics()
     ph = dataclasses.replace(ph, De=0.3 * ph.kT, r0=0.3, beta=1.5, mcut=min(1.5, ph.h))
     # 300 rigid biconcave discs pack (with a 0.8 gap, throat included) only
@@ -547,7 +564,7 @@ def stenosis_2m(seed: int = 20261203, physics: Physics | None = None, noise: boo
     last = None
     for hct in (0.30, 0.27, 0.24, 0.21, 0.18, 0.15, 0.12):
         try:
-            c = channel(300, 80, 300, hematocrit=hct, seed=seed, physics=ph, noise=noise,
+            c = channel(380, 80, 300, hematocrit=hct, seed=seed, physics=ph, noise=noise,
                         stenosis=0.5, name="stenosis_2m_300rbc", dense=True)
             c.hematocrit = hct
             return c
@@ -558,12 +575,12 @@ def stenosis_2m(seed: int = 20261203, physics: Physics | None = None, noise: boo
 
 def suspension_10m(seed: int = 20261203, physics: Physics | None = None, noise: bool = True,
                    n_rbc: int = 2000, hematocrit: float = 0.18) -> Case:
-    """10M fluid+wall lattice sites (10,002,096; 9.37M remain around the
-    cells) and 2000 RBCs x 642 vertices at 18 % hematocrit (random
-    placement of 2000 discs jams near 19 % in this duct), 3 wall layers: a
-    411 x 150 x 150 duct (156 x 156 with walls), Poiseuille body force,
-    aggregation off."""
-    c = channel(411, 150, n_rbc, hematocrit=hematocrit, seed=seed, physics=physics,
+    """10M fluid+wall particles (10,109,183 of 11,486,592 lattice sites
+    remain around the cells) and 2000 RBCs x 642 vertices at 18 %
+    hematocrit (diameter 17.0; random placement of 2000 discs jams near
+    19 % in this duct), 3 wall layers: a 472 x 150 x 150 duct (156 x 156
+    with walls), Poiseuille body force, aggregation off."""
+    c = channel(472, 150, n_rbc, hematocrit=hematocrit, seed=seed, physics=physics,
                 noise=noise, name=f"suspension_10m_{n_rbc}rbc")
     c.hematocrit = hematocrit
     return c

@@ -28,7 +28,7 @@ def test_trained_surrogate_tracks_plain_substepping(gpu):
     print(f"rows {X.shape[0]}, loss {loss[0]:.3g} -> {loss[-1]:.3g} in {secs:.2f} s")
     assert loss[-1] < 0.2 * loss[0]
     held = _box(9)
-    r = substepper.compare(held, net, norm, THRESHOLD, steps=30)
+    r = substepper.compare(held, net, norm, THRESHOLD, steps=30, relax=0)
     print("trained:", r)
     assert r["fallback_rate"] < 0.2
     assert r["di_rel_err_max"] < 1e-2          # deformation index vs plain
@@ -46,7 +46,7 @@ def test_trained_surrogate_tracks_plain_substepping(gpu):
         assert np.array_equal(runs[0][0][key], runs[1][0][key]), key
     assert runs[0][1]["fallbacks"] == runs[1][1]["fallbacks"]
     assert runs[0][1]["steps"] == runs[1][1]["steps"] == 9
-    u = substepper.compare(held, Mlp(seed=5), norm, THRESHOLD, steps=5)
+    u = substepper.compare(held, Mlp(seed=5), norm, THRESHOLD, steps=5, relax=0)
     print("untrained:", u)
     assert u["fallback_rate"] == 1.0
     # with every step falling back, the untrained network reproduces plain
