@@ -148,6 +148,8 @@ PROTOTYPES = {
     "gc_EngineStep": (I, [P, I, I]),
     "gc_EngineStepGraph": (I, [P, I, I]),
     "gc_EngineInfo": (I, [P, C.POINTER(gc_engine_info)]),
+    "gc_CheckSlotMaps": (I, [I, I, I, I, P, P, P]),
+    "gc_EngineCheckSlotMaps": (I, [P]),
     "gc_EngineGetState": (I, [P, C.POINTER(gc_host_state)]),
     "gc_EngineSetState": (I, [P, C.POINTER(gc_host_state), L, C.c_uint64]),
     "gc_EngineGetCells": (I, [P, C.POINTER(gc_host_cells)]),

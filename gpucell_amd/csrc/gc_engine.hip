@@ -924,6 +924,8 @@ int launch_build_t(gc_engine* e, int move, Prof* pr = nullptr)
     return GC_OK;
 }
 
+int debug_check_slot_maps(gc_engine* e);
+
 int launch_build(gc_engine* e, int move)
 {
     return e->ens() ? launch_build_t<true>(e, move) : launch_build_t<false>(e, move);
@@ -939,6 +941,7 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
     mark(pr, s, "start");
     int st = launch_build_t<ENS>(e, 1, pr);
     if (st) return st;
+    if ((st = debug_check_slot_maps(e))) return st;
     mark(pr, s, "sort+gather+predict");
     const int c = e->cur;
     {
@@ -1236,6 +1239,10 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
         cg.nTotalPar = e->consts.nTotalPar;
         make_params(cg, d.ext, &R.P);
         if (d.ext.wall_u_lo != 0.0f || d.ext.wall_u_hi != 0.0f) e->moving_walls = 1;
+        if (r == 0) {
+            const char* cs = getenv("GPUCELL_CHECK_SLOTS");
+            e->check_slots = cs && cs[0] == '1';
+        }
         int Kr = 0;
         float outer[3];
         if ((st = gc_ConfigDomain(d.bbox, d.consts.CutRadius, d.ext.periodic, R.nc, outer, R.invw, &Kr)))
@@ -1482,6 +1489,84 @@ extern "C" int gc_EngineStepGraph(gc_engine* e, int nsteps, int driveon)
     e->walls_dirty = e->any_walls;
     if (nsteps & 1) return launch_step(e, driveon, nullptr);
     return GC_OK;
+}
+
+// Host-side check of the slot maps the re-sort produces (sp: slot -> id,
+// CurPos: membrane vertex id -> slot, act: the moving particles' slots).
+extern "C" int gc_CheckSlotMaps(int n, int nf, int nc, int nmov, const int* sp, const int* curpos,
+                                const int* act)
+{
+    if (n < 0 || !sp || nf < 0 || nc < nf || nc > n || nmov < 0 || nmov > n || (nmov > 0 && !act) ||
+        (nc < n && !curpos))
+        return set_error(GC_ERR_INVALID, "CheckSlotMaps: bad arguments");
+    char msg[160];
+    auto bad = [&](const char* fmt, int a, int b, int c) {
+        snprintf(msg, sizeof(msg), fmt, a, b, c);
+        return set_error(GC_ERR_INVALID, std::string("CheckSlotMaps: ") + msg);
+    };
+    std::vector<unsigned char> seen((size_t)n, 0);
+    int movers = 0;
+    for (int q = 0; q < n; ++q) {
+        const int id = sp[q];
+        if (id < 0 || id >= n) return bad("slot %d holds id %d, outside [0, %d)", q, id, n);
+        if (seen[id]++) return bad("id %d is in two slots (the second is %d)%.0d", id, q, 0);
+        movers += id >= nf;
+    }
+    for (int id = nc; id < n; ++id) {
+        const int q = curpos[id];
+        if (q < 0 || q >= n || sp[q] != id)
+            return bad("CurPos of membrane vertex %d is %d, which holds id %d", id, q,
+                       q >= 0 && q < n ? sp[q] : -1);
+    }
+    if (nmov == n && movers < n) {   // moving walls: the force pass visits every slot
+        for (int t = 0; t < n; ++t)
+            if (act[t] != t) return bad("act[%d] = %d (every slot expected in order)%.0d", t, act[t], 0);
+        return GC_OK;
+    }
+    if (nmov != movers) return bad("act holds %d slots for %d moving particles%.0d", nmov, movers, 0);
+    int prev = -1;
+    for (int t = 0; t < nmov; ++t) {
+        const int q = act[t];
+        if (q <= prev || q >= n) return bad("act[%d] = %d after %d (slots in order expected)", t, q, prev);
+        if (sp[q] < nf) return bad("act[%d] = slot %d, which holds wall %d", t, q, sp[q]);
+        prev = q;
+    }
+    return GC_OK;
+}
+
+namespace {
+
+int check_slot_maps(gc_engine* e)
+{
+    const int N = e->N;
+    std::vector<int> sp(N), cp(N), act((size_t)e->nmov);
+    int st = hip_check(hipStreamSynchronize(e->s), "slot maps sync");
+    if (!st) st = hip_check(hipMemcpy(sp.data(), e->d_sp[e->cur], sizeof(int) * N, hipMemcpyDeviceToHost), "slot maps D2H");
+    if (!st) st = hip_check(hipMemcpy(cp.data(), e->d_curpos, sizeof(int) * N, hipMemcpyDeviceToHost), "slot maps D2H");
+    if (!st && e->nmov > 0)
+        st = hip_check(hipMemcpy(act.data(), e->d_act, sizeof(int) * e->nmov, hipMemcpyDeviceToHost), "slot maps D2H");
+    if (st) return st;
+    const GcParams& P = e->hreps[0].P;
+    return gc_CheckSlotMaps(N, P.nf, P.nc, e->nmov, sp.data(), cp.data(), act.data());
+}
+
+// with GPUCELL_CHECK_SLOTS=1 in the environment at creation: after every
+// re-sort of an eager step (not while a graph is being captured)
+int debug_check_slot_maps(gc_engine* e)
+{
+    if (!e->check_slots) return GC_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    int st = hip_check(hipStreamIsCapturing(e->s, &cs), "capture status");
+    if (st || cs != hipStreamCaptureStatusNone) return st;
+    return check_slot_maps(e);
+}
+
+}  // namespace
+
+extern "C" int gc_EngineCheckSlotMaps(gc_engine* e)
+{
+    if (!e) return set_error(GC_ERR_INVALID, "EngineCheckSlotMaps: null");
+    return check_slot_maps(e);
 }
 
 extern "C" int gc_EngineInfo(const gc_engine* e, gc_engine_info* info)

@@ -96,6 +96,7 @@ struct gc_engine {
     hipGraphExec_t gexec = nullptr;
     int g_driveon = -1;
     bool walls_dirty = false;
+    bool check_slots = false;   // GPUCELL_CHECK_SLOTS=1: slot maps checked on the host after each eager re-sort
     bool any_morse = false, any_walls = false, any_bound1 = false;
     gc::MembraneSub* ms = nullptr;   // membrane substepping (mode 1/2)
     gc::TuneCfg tune;

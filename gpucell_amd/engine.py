@@ -68,6 +68,10 @@ class Engine:
         abi.check(self.lib.gc_EngineInfo(self.h, C.byref(i)), "EngineInfo")
         return i
 
+    def check_slot_maps(self):
+        """Raises GcError if the slot maps (sp, CurPos, act) are inconsistent."""
+        abi.check(self.lib.gc_EngineCheckSlotMaps(self.h), "EngineCheckSlotMaps")
+
     def state(self) -> dict:
         out = _state_arrays(self.case)
         hs = _host_state(out)
@@ -208,6 +212,10 @@ class Ensemble:
         i = abi.gc_engine_info()
         abi.check(self.lib.gc_EngineInfo(self.h, C.byref(i)), "EngineInfo")
         return i
+
+    def check_slot_maps(self):
+        """Raises GcError if the slot maps (sp, CurPos, act) are inconsistent."""
+        abi.check(self.lib.gc_EngineCheckSlotMaps(self.h), "EngineCheckSlotMaps")
 
     def state(self, rep: int) -> dict:
         out = _state_arrays(self.cases[rep])

@@ -293,6 +293,20 @@ typedef struct {
 /* Synchronises; returns GC_OK whatever the fault bits (they are in info). */
 int gc_EngineInfo(const gc_engine* e, gc_engine_info* info);
 
+/* Consistency of the slot maps of an engine's particle order (a debugging
+ * aid; no counterpart in the reference, which keeps CurPos for every
+ * particle): sp[s] = id at slot s must be a permutation of [0, n);
+ * curpos[id] = slot of membrane vertex id (ids in [nc, n)) with
+ * sp[curpos[id]] == id; act = the nmov slots the force pass visits: the
+ * slots of the moving particles (ids >= nf) in increasing order, or, with
+ * moving walls (nmov == n), every slot in order.  Host code only (no GPU).
+ * GC_OK, or GC_ERR_INVALID with a message naming the first violation. */
+int gc_CheckSlotMaps(int n, int nf, int nc, int nmov, const int* sp, const int* curpos, const int* act);
+/* The engine's current maps (valid after a step) copied to the host and
+ * checked as above.  Setting GPUCELL_CHECK_SLOTS=1 before gc_EngineCreate
+ * runs the same check after the re-sort of every eager step. */
+int gc_EngineCheckSlotMaps(gc_engine* e);
+
 /* State copied to host in id order; any pointer may be NULL. */
 typedef struct {
     double* pos;        /* [N*3] FP64 position accumulator (wrapped) */
