@@ -1,9 +1,12 @@
 """SDPD step benchmark on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--case suspension_10m]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--case suspension_10m] [--skin 0.2]
 
 A step is one Particle::UpdatePar (neighbour build, forces, integration) of
-the whole suspension, replayed from a HIP graph.  Inputs are generated on
+the whole suspension, replayed from a HIP graph; by default the engine keeps
+Verlet lists with a skin of 0.2 (re-sorted and rebuilt on the device when
+the particles have moved far enough; --skin 0 re-sorts every step as the
+reference does).  Inputs are generated on
 the host from a seed and resident in HBM before the timed region.  Prints
 one JSON line: whole-job particle-steps/s, the dominant kernel's HBM
 roofline (HIP events on the engine stream), the step-level roofline, and
@@ -38,12 +41,28 @@ KERNEL_BYTES_PER_PARTICLE = {"integrate+key": 56.0, "sort+gather+predict": 142.3
                              "density+wall velocity": 16.0, "sdpd force": 72.0}
 KERNEL_NAMES = {"integrate+key": "k2_integrate", "sort+gather+predict": "k2_sortgather",
                 "density+wall velocity": "k2_density", "sdpd force": "k2_force"}
+# Verlet skin mode: no per-step re-sort.  The advance pass reads the FP64
+# position 24 B, velocity 16 B and force 12 B and writes the position 24 B,
+# its FP32 copy 16 B, the predicted velocity 16 B and the half-step velocity
+# 12 B (120 B); density 16 B and force 72 B as above.  The list rebuild
+# (re-sort and list build, every few tens of steps) is its own phase.
+SKIN_STEP_BYTES_PER_PARTICLE = 208.0
+SKIN_KERNEL_BYTES_PER_PARTICLE = {"advance+predict": 120.0, "density+wall velocity": 16.0, "sdpd force": 72.0}
+SKIN_KERNEL_NAMES = {"advance+predict": "k3_advance", "density+wall velocity": "k3_density",
+                     "sdpd force": "k2_force"}
+DEFAULT_SKIN = 0.2
 # autotuner family of each pass (its chosen shape names the launch variant)
-KERNEL_FAMILY = {"integrate+key": "integrate", "density+wall velocity": "density", "sdpd force": "sdpd force"}
+KERNEL_FAMILY = {"integrate+key": "integrate", "advance+predict": "integrate",
+                 "density+wall velocity": "density", "sdpd force": "sdpd force"}
 # the bound of the neighbour-loop passes (instruction issue, not HBM)
 KERNEL_BOUND_NOTE = {
     "density+wall velocity": "VALU issue in the 27-cube scan (~95 candidates per particle, lane imbalance)",
     "sdpd force": "VALU issue: Philox-4x32-10 (20 quarter-rate 64-bit products per pair) + pair math"}
+SKIN_BOUND_NOTE = {
+    "density+wall velocity": "scattered 16 B position gathers over the Verlet lists (~20 entries per particle)",
+    "sdpd force": "VALU issue: Philox-4x32-10 (20 quarter-rate 64-bit products per pair) + pair math; "
+                  "scattered list-entry loads through the neighbour bits",
+    "advance+predict": "hbm"}
 
 
 def log(msg):
@@ -115,7 +134,7 @@ def cpu_baseline(case, seconds_target=15.0):
 SURROGATE_THRESHOLD = 2e-5   # residual of the Verlet relation, in rest edge lengths
 
 
-def surrogate_leg(case, case_name):
+def surrogate_leg(case, case_name, skin=0.0):
     """Learned membrane substepper applied to this suspension: ~1e6
     training rows from plain 8-substep trajectories of the 32^3 one-RBC box
     and of the 1M-particle 100-RBC channel, training with the library's
@@ -138,7 +157,7 @@ def surrogate_leg(case, case_name):
     log(f"[bench] surrogate: {X.shape[0]} training rows in {t_data:.1f}s")
     net, norm, loss, t_train = substepper.train(X, Y, epochs=8, batch=4096, lr=2e-3)
     log(f"[bench] surrogate: trained in {t_train:.1f}s, loss {loss[0]:.4g} -> {loss[-1]:.4g}")
-    r = substepper.compare(case, net, norm, SURROGATE_THRESHOLD, steps=40)
+    r = substepper.compare(case, net, norm, SURROGATE_THRESHOLD, steps=40, skin=skin)
     log("[bench] surrogate vs plain: " + json.dumps(r))
     r.update({"metric": "simulated time per wall-second, surrogate / plain substepping "
                         "(k=8, equal dt)",
@@ -166,6 +185,9 @@ def main():
                     help="skip the learned-substepper leg (training + surrogate vs plain)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (no autotuning before the timed run)")
+    ap.add_argument("--skin", type=float, default=DEFAULT_SKIN,
+                    help="Verlet skin of the engine's neighbour lists (default %(default)s; 0 = the "
+                         "reference's per-step re-sort and cube scan)")
     ap.add_argument("--surrogate-only", action="store_true",
                     help="only the learned-substepper leg on --case (prints its JSON and exits; "
                          "for kernel traces of the two substep modes)")
@@ -192,14 +214,15 @@ def main():
     t = time.perf_counter()
     case, workload = make_case(a.case)
     if a.surrogate_only:
-        print(json.dumps(surrogate_leg(case, a.case)), flush=True)
+        print(json.dumps(surrogate_leg(case, a.case, a.skin)), flush=True)
         return
     batch = isinstance(case, list)
     cases = case if batch else [case]
     N, Nm = sum(c.N for c in cases), sum(c.nCellPar for c in cases)
     log(f"[bench] case ready: {len(cases)} suspension(s), N={N} Nm={Nm} "
         f"({time.perf_counter() - t:.1f}s)")
-    eng = Ensemble(cases) if batch else Engine(case)
+    skin = a.skin
+    eng = Ensemble(cases, skin=skin) if batch else Engine(case, skin=skin)
     tune_rows = None
     if not a.no_autotune:
         log("[bench] autotuning launch shapes")
@@ -222,16 +245,20 @@ def main():
         S = eng.state(r) if batch else eng.state()
         finite = finite and bool(np.isfinite(S["pos"]).all() and np.isfinite(S["force"]).all())
     log(f"[bench] {a.steps} steps: {ms:.4f} ms/step, finite={finite}")
+    nrb = eng.info().nrebuild
     prof = eng.profile(20)
     log("[bench] per-phase us: " + json.dumps(prof))
+    kbytes = SKIN_KERNEL_BYTES_PER_PARTICLE if skin > 0 else KERNEL_BYTES_PER_PARTICLE
+    knames = SKIN_KERNEL_NAMES if skin > 0 else KERNEL_NAMES
+    knotes = SKIN_BOUND_NOTE if skin > 0 else KERNEL_BOUND_NOTE
     # dominant single kernel of the step (HIP events on the engine stream)
-    dom = max(("integrate+key", "density+wall velocity", "sdpd force"), key=lambda k: prof.get(k, 0.0))
+    dom = max((k for k in kbytes if k in prof), key=lambda k: prof.get(k, 0.0))
     dom_us = prof[dom]
-    achieved = KERNEL_BYTES_PER_PARTICLE[dom] * N / (dom_us * 1e-6) / 1e9
+    achieved = kbytes[dom] * N / (dom_us * 1e-6) / 1e9
     # measured HBM bytes per launch from the committed counter profile of this
     # case, used only when it was taken on the launch variant this run uses
     traffic, traffic_variant = None, None
-    pmc = os.path.join(ROOT, "profiles", f"traffic_{KERNEL_NAMES[dom]}_{a.case}.json")
+    pmc = os.path.join(ROOT, "profiles", f"traffic_{knames[dom]}_{a.case}.json")
     used = None if tune_rows is None else {r["family"]: r["shape"] for r in tune_rows if r["chosen"]}
     if os.path.exists(pmc):
         tj = json.load(open(pmc))
@@ -245,18 +272,21 @@ def main():
     valu = None
     vp = os.path.join(ROOT, "profiles", f"valu_{a.case}.json")
     if os.path.exists(vp):
-        vj = json.load(open(vp)).get("kernels", {}).get(KERNEL_NAMES[dom])
+        vj = json.load(open(vp)).get("kernels", {}).get(knames[dom])
         if vj:
             # waves the kernel launches on this case (the force pass runs one
             # lane per moving particle), as recorded with the counters
             waves = vj.get("waves", (N + 63) // 64)
             issue_us = vj["valu_per_wave"] * waves * 4 / (1024 * 2.4e9) * 1e6
-            valu = {"kernel": KERNEL_NAMES[dom], "valu_per_wave": vj["valu_per_wave"],
+            valu = {"kernel": knames[dom], "valu_per_wave": vj["valu_per_wave"],
                     "issue_bound_us": issue_us, "avg_us": dom_us, "frac": issue_us / dom_us,
                     "peak": "1024 SIMDs x 16 lanes x 2.4 GHz (wave64 VALU instruction = 4 cycles)",
                     "note": "quarter-rate instructions counted once: a lower bound"}
-    step_bytes = STEP_BYTES_PER_PARTICLE * N + STEP_BYTES_PER_VERTEX * Nm
+    step_bytes = (SKIN_STEP_BYTES_PER_PARTICLE if skin > 0 else STEP_BYTES_PER_PARTICLE) * N \
+        + STEP_BYTES_PER_VERTEX * Nm
     step_gbs = step_bytes / (ms * 1e-3) / 1e9
+    # the same time against the per-step scheme's dataflow (re-sort every step)
+    ref_bytes = STEP_BYTES_PER_PARTICLE * N + STEP_BYTES_PER_VERTEX * Nm
     out = {
         "metric": "SDPD particle-steps/s on 1 GPU (% HBM roofline); surrogate vs plain speedup",
         "value": N * a.steps / (t1 - t0),
@@ -272,28 +302,33 @@ def main():
         "data": "synthetic (seeded lattice fluid, walls and biconcave RBCs)",
         "config": {"workload": workload, "N": N, "Nm": Nm, "suspensions": len(cases),
                    "h": float(cases[0].consts.CutRadius), "dt": float(cases[0].consts.TimeStep),
-                   "graph": graph, "finite": finite},
-        "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": achieved,
+                   "graph": graph, "finite": finite, "skin": skin,
+                   "list_rebuilds_per_step": (nrb / (a.warmup + a.steps)) if skin > 0 else 1.0},
+        "roofline": {"bound": "hbm", "kernel": knames[dom], "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "traffic_variant": traffic_variant, "avg_us": dom_us,
-                     "algorithmic_bytes_per_particle": KERNEL_BYTES_PER_PARTICLE[dom],
-                     "note": KERNEL_BOUND_NOTE.get(dom)},
+                     "algorithmic_bytes_per_particle": kbytes[dom],
+                     "note": knotes.get(dom)},
         "valu_roofline": valu,
         "step_roofline": {"bytes_per_step": step_bytes, "achieved": step_gbs,
-                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS},
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": step_gbs / HBM_PEAK_GBS,
+                          "dataflow": "skin mode: advance 120 + density 16 + force 72 B per particle, "
+                                      "440 B per membrane vertex" if skin > 0 else
+                                      "per-step re-sort: 286 B per particle, 440 B per membrane vertex",
+                          "per_step_dataflow_frac": ref_bytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "phases_us": prof,
-        "passes": {k: {"kernel": KERNEL_NAMES[k], "avg_us": prof.get(k),
+        "passes": {k: {"kernel": knames[k], "avg_us": prof.get(k),
                        "algorithmic_bytes_per_particle": b,
                        "achieved_gbs": (b * N / (prof[k] * 1e-6) / 1e9) if prof.get(k) else None,
                        "frac_hbm": (b * N / (prof[k] * 1e-6) / 1e9 / HBM_PEAK_GBS) if prof.get(k) else None,
-                       "bound": KERNEL_BOUND_NOTE.get(k, "hbm")}
-                   for k, b in KERNEL_BYTES_PER_PARTICLE.items()},
+                       "bound": knotes.get(k, "hbm")}
+                   for k, b in kbytes.items()},
         "tuning": None if tune_rows is None else {
             r["family"]: r["shape"] for r in tune_rows if r["chosen"]},
     }
     eng.close()
     if not a.no_surrogate and not batch and case.nCell > 0:
-        out["surrogate"] = surrogate_leg(case, a.case)
+        out["surrogate"] = surrogate_leg(case, a.case, skin)
     if not a.no_cpu:
         log("[bench] CPU baseline (FP64 oracle)")
         out["cpu_baseline"] = cpu_baseline(max(cases, key=lambda c: c.N))

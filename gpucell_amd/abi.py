@@ -65,7 +65,7 @@ class gc_case_desc(C.Structure):
         ("celltab", C.c_void_p), ("beginendtri", C.c_void_p), ("shearmod_para", C.c_void_p),
         ("bendingmod_para", C.c_void_p), ("hlarea_para", C.c_void_p),
         ("hgarea_para", C.c_void_p), ("hvol_para", C.c_void_p), ("bbox", C.c_float * 6),
-        ("consts", Structconst), ("ext", gc_ext_consts),
+        ("consts", Structconst), ("ext", gc_ext_consts), ("skin", C.c_float),
     ]
 
 
@@ -74,7 +74,7 @@ class gc_engine_info(C.Structure):
         ("step", C.c_long), ("noise_counter", C.c_uint64), ("nTotalPar", C.c_int),
         ("nCellPar", C.c_int), ("nCell", C.c_int), ("sizecube", C.c_int),
         ("ncube", C.c_int * 3), ("outerregion", C.c_float * 3), ("invwid", C.c_float * 3),
-        ("fault", C.c_int), ("nl_overflow", C.c_int),
+        ("fault", C.c_int), ("nl_overflow", C.c_int), ("nrebuild", C.c_int), ("skin", C.c_float),
     ]
 
 

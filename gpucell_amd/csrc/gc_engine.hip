@@ -61,6 +61,15 @@ constexpr int BS = GC_BS;
 // more than NL_CAP neighbours gets count -1 and the force pass scans its
 // cubes instead.
 constexpr int NL_CAP = 64;
+// Skin mode: Verlet lists of up to VL_CAP entries, and after them one row of
+// neighbour bits per 32 entries (density pass -> force pass): VL_ROWS rows.
+constexpr int VL_CAP = 128, VL_ROWS = VL_CAP + VL_CAP / 32;
+// ints per wave block of the neighbour lists
+inline int nl_stride(const gc_engine* e) { return (e->skin_mode() ? VL_ROWS : NL_CAP) * 64; }
+// Skin mode: workgroup displacement maxima, the advance's groups (at most
+// one per 64 particles) first, the membrane substeps' groups after them.
+inline int skin_ms_off(const gc_engine* e) { return (e->N + 63) / 64; }
+inline size_t skin_bmax_size(const gc_engine* e) { return (size_t)skin_ms_off(e) + (e->nm + GC_BS - 1) / GC_BS + 1; }
 constexpr int CUBE_SCAN_BS = 256, CUBE_IPT = 4, CUBE_TILE = CUBE_SCAN_BS * CUBE_IPT;
 
 // Image code of a range: 2 bits per axis, 1 = shift -L, 2 = shift +L.
@@ -73,6 +82,39 @@ __device__ __forceinline__ int shift_code(float sx, float sy, float sz)
 __device__ __forceinline__ float code_shift(int c, float L)
 {
     return (c & 3) == 1 ? -L : ((c & 3) == 2 ? L : 0.0f);
+}
+
+// posid.w = id | (wrap code << ID_BITS).  In skin mode the wrap code holds a
+// particle's periodic wraps since the last list rebuild (an image code: per
+// axis 1 = -1, 2 = +1); it is 0 in per-step mode and right after a rebuild.
+// Ids are below 2^25.
+constexpr int ID_BITS = 25;
+constexpr int ID_MASK = (1 << ID_BITS) - 1;
+__device__ __forceinline__ int pid_of(float w) { return __float_as_int(w) & ID_MASK; }
+__device__ __forceinline__ int woff_of(float w) { return (int)((unsigned)__float_as_int(w) >> ID_BITS); }
+__device__ __forceinline__ float id_word(int id, int woff) { return __int_as_float(id | (woff << ID_BITS)); }
+
+// L coefficient (-1, 0, +1) of axis a in an image code
+__device__ __forceinline__ int code_coef(int c, int a)
+{
+    const int b = (c >> (2 * a)) & 3;
+    return (b & 2) ? 1 : -(b & 1);
+}
+__device__ __forceinline__ int coef_code(int k0, int k1, int k2)
+{
+    auto e = [](int k) { return k < 0 ? 1 : (k > 0 ? 2 : 0); };
+    return e(k0) | (e(k1) << 2) | (e(k2) << 4);
+}
+
+// Skin mode: the image shift of a list entry along axis a.  The entry's code
+// was fixed at the rebuild; positions are wrapped every step, so a particle
+// that wrapped since (wrap code o) sits o L away from where the code
+// assumed: S + (o_j - o_i) L, an exact multiple of L (the physical image of
+// the pair, i.e. the shift the per-step cube scan would use).
+__device__ __forceinline__ float eff_shift(int code, int oi, int oj, int a, float L)
+{
+    const int k = code_coef(code, a) + code_coef(oj, a) - code_coef(oi, a);
+    return (float)k * L;
 }
 
 // Slots [jb, je) in order, four loads issued before their use.
@@ -178,11 +220,9 @@ __device__ __forceinline__ float ext_force_x(const GcParams& P, float y)
 // counters in img).  The key pass and the gather both run it on the same
 // inputs, so the gather reproduces the key pass's positions bit for bit
 // and the advanced positions are not stored in between.
-__device__ __forceinline__ void advance(const GcRep& R, const GcParams& P, const float4& v, const tfloat3& f,
-                                        double& x, double& y, double& z, int img[3])
+// One periodic wrap of u (a position at most one period outside the box).
+__device__ __forceinline__ void wrap_once(const GcRep& R, const GcParams& P, double u[3], int img[3])
 {
-    double dt = P.dt, a = 0.5 * (double)P.dt * (double)P.dt * (double)P.inv_m;
-    double u[3] = {x + (dt * v.x + a * f.x), y + (dt * v.y + a * f.y), z + (dt * v.z + a * f.z)};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         if (!P.per[d]) continue;
@@ -191,6 +231,23 @@ __device__ __forceinline__ void advance(const GcRep& R, const GcParams& P, const
         if (u[d] < lo) { u[d] += L; img[d] -= 1; }
         else if (u[d] >= hi) { u[d] -= L; img[d] += 1; }
     }
+}
+
+// r + dt v + dt^2/(2m) F in FP64, not wrapped
+__device__ __forceinline__ void advance_step(const GcParams& P, const float4& v, const tfloat3& f, double u[3])
+{
+    double dt = P.dt, a = 0.5 * (double)P.dt * (double)P.dt * (double)P.inv_m;
+    u[0] = u[0] + (dt * v.x + a * f.x);
+    u[1] = u[1] + (dt * v.y + a * f.y);
+    u[2] = u[2] + (dt * v.z + a * f.z);
+}
+
+__device__ __forceinline__ void advance(const GcRep& R, const GcParams& P, const float4& v, const tfloat3& f,
+                                        double& x, double& y, double& z, int img[3])
+{
+    double u[3] = {x, y, z};
+    advance_step(P, v, f, u);
+    wrap_once(R, P, u, img);
     x = u[0]; y = u[1]; z = u[2];
 }
 
@@ -222,6 +279,25 @@ __device__ __forceinline__ bool wall_moves(const GcParams& P, int id, int wmove)
 __device__ __forceinline__ float4 wall_v4(const GcParams& P, double y)
 {
     return make_float4((float)y >= P.wall_split ? P.wall_u_hi : P.wall_u_lo, 0.f, 0.f, 0.f);
+}
+
+// Histogram rank of key k for the wave's active lanes (a prefix of the
+// wave): each run of equal keys takes its ranks with one atomic.
+__device__ __forceinline__ void hist_rank(bool act, int k, int p, int* __restrict__ rank, int* __restrict__ cnt)
+{
+    const int lane = threadIdx.x & 63;
+    const int prev = __shfl_up(k, 1);
+    const bool head = act && (lane == 0 || prev != k);
+    const unsigned long long heads = __ballot(head);
+    const int nact = __popcll(__ballot(act));
+    const unsigned long long above = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+    const int next = above ? (int)__ffsll((long long)above) - 1 : nact;
+    int base = 0;
+    if (head) base = atomicAdd(&cnt[k], next - lane);
+    const unsigned long long upto = lane == 63 ? heads : heads & ((2ull << lane) - 1ull);
+    const int myhead = upto ? 63 - __clzll((long long)upto) : 0;
+    base = __shfl(base, myhead);
+    if (act) rank[p] = base + (lane - myhead);
 }
 
 // [1,2,3] key of the advanced position, histogram rank.  Slots are in the
@@ -258,26 +334,14 @@ __global__ __launch_bounds__(TBS) void k2_integrate(int n, GcRep R0, const GcRep
         key[p] = k;
     }
     });
-    // runs of equal keys among the active lanes (active lanes are a prefix)
-    const int lane = threadIdx.x & 63;
-    const int prev = __shfl_up(k, 1);
-    const bool head = act && (lane == 0 || prev != k);
-    const unsigned long long heads = __ballot(head);
-    const int nact = __popcll(__ballot(act));
-    const unsigned long long above = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
-    const int next = above ? (int)__ffsll((long long)above) - 1 : nact;
-    int base = 0;
-    if (head) base = atomicAdd(&cnt[k], next - lane);
-    const unsigned long long upto = lane == 63 ? heads : heads & ((2ull << lane) - 1ull);
-    const int myhead = upto ? 63 - __clzll((long long)upto) : 0;
-    base = __shfl(base, myhead);
-    if (act) rank[p] = base + (lane - myhead);
+    hist_rank(act, k, p, rank, cnt);
 }
 
 // [4] cube counts per tile of CUBE_TILE cubes
 __global__ __launch_bounds__(CUBE_SCAN_BS) void k2_tile_sums(int K, const int* __restrict__ cnt,
-                                                             int* __restrict__ tsum)
+                                                             int* __restrict__ tsum, const int* __restrict__ gate)
 {
+    if (gate && *gate == 0) return;   // skin mode: no rebuild this step
     __shared__ int red[CUBE_SCAN_BS];
     const int tid = threadIdx.x, base = blockIdx.x * CUBE_TILE + tid * CUBE_IPT;
     int t = 0;
@@ -298,8 +362,9 @@ __global__ __launch_bounds__(CUBE_SCAN_BS) void k2_tile_sums(int K, const int* _
 // reads the value before the advance).
 __global__ __launch_bounds__(CUBE_SCAN_BS) void k2_cube_offsets(int K, const int* __restrict__ cnt,
     const int* __restrict__ tsum, int* __restrict__ offs, unsigned long long* __restrict__ ctr, int bump,
-    int* __restrict__ nlover)
+    int* __restrict__ nlover, const int* __restrict__ gate)
 {
+    if (gate && *gate == 0) return;
     __shared__ int red[CUBE_SCAN_BS];
     const int b = blockIdx.x, tid = threadIdx.x;
     int pre = 0;
@@ -345,8 +410,10 @@ __global__ __launch_bounds__(CUBE_SCAN_BS) void k2_cube_offsets(int K, const int
 // (every non-empty cube has exactly one rank-0 member).
 __global__ __launch_bounds__(BS) void k2_scatter(int n, const int* __restrict__ key,
     const int* __restrict__ rank, const int* __restrict__ sp, const int* __restrict__ offs,
-    int* __restrict__ tk, int* __restrict__ tv, int* __restrict__ to, int* __restrict__ cnt)
+    int* __restrict__ tk, int* __restrict__ tv, int* __restrict__ to, int* __restrict__ cnt,
+    const int* __restrict__ gate)
 {
+    if (gate && *gate == 0) return;
     int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (p >= n) return;
     int k = key[p], r = rank[p];
@@ -363,7 +430,12 @@ __global__ __launch_bounds__(BS) void k2_scatter(int n, const int* __restrict__ 
 // advanced as in the key pass (move); predictor.  CurPos is kept for the
 // membrane vertices only (the membrane kernels read it; the whole map is
 // the inverse of SortPart).
-template <bool ENS>
+// SK (skin mode, a list rebuild): the positions were advanced and wrapped
+// in place by the advance pass; the wrap codes of a rebuilding replica are
+// cleared (its lists start from these positions), the other replicas keep
+// their slots and wrap codes; the predictor is recomputed from the same
+// inputs as in the advance pass (the same bits).
+template <bool ENS, bool SK>
 __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, int move, int mmove, int wmove, const int* __restrict__ tk,
     const int* __restrict__ tv, const int* __restrict__ to, const int* __restrict__ offs,
@@ -373,8 +445,10 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     const tint3* __restrict__ nt_old, tint3* __restrict__ nt_new, int* __restrict__ sp_new,
     int* __restrict__ key_new, float4* __restrict__ posid, tfloat3* __restrict__ coor,
     float4* __restrict__ velrho, tfloat3* __restrict__ velhalf, const int* __restrict__ wincl,
-    int* __restrict__ act, int* __restrict__ fault)
+    int* __restrict__ act, int* __restrict__ fault, const int* __restrict__ gate,
+    const int* __restrict__ rbflag, unsigned char* __restrict__ woff)
 {
+    if (SK && *gate == 0) return;   // skin mode: no rebuild this step
     int q = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (q >= n) return;
     int k = tk[q], id = tv[q];
@@ -388,7 +462,18 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     float4 v = vout[o];
     const bool mv = move && moves(R0.P, id, mmove);
     const bool wmv = move && wall_moves(R0.P, id, wmove);
-    with_rep<ENS>(R0, reps, rep_of, mv || wmv, id, [&](const GcRep& R) __attribute__((always_inline)) {
+    with_rep<ENS>(R0, reps, rep_of, SK || mv || wmv, id, [&](const GcRep& R) __attribute__((always_inline)) {
+    if constexpr (SK) {
+        const GcParams& P = R.P;
+        if (moves(P, id, mmove)) {
+            const tfloat3 f = force[o];
+            float h = 0.5f * P.dt * P.inv_m, l = P.lam * P.dt * P.inv_m;
+            velhalf[s] = tfloat3{v.x + h * f.x, v.y + h * f.y, v.z + h * f.z};
+            v.x += l * f.x; v.y += l * f.y; v.z += l * f.z;
+        } else {
+            velhalf[s] = id < P.nf ? tfloat3{0.f, 0.f, 0.f} : tfloat3{v.x, v.y, v.z};
+        }
+    } else {
     if (mv) {
         const GcParams& P = R.P;
         const tfloat3 f = force[o];
@@ -408,13 +493,19 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
             nt.x += img[0]; nt.y += img[1]; nt.z += img[2];
         }
     }
+    }
     });
     pos_new[s] = x; pos_new[n + s] = y; pos_new[2 * n + s] = z;
     // a non-finite position has no neighbours (every distance test fails)
     // and so a finite force: it is caught here
     if (!isfinite(x + y + z)) atomicOr(fault, GC_FAULT_NONFINITE);
     float fx = (float)x, fy = (float)y, fz = (float)z;
-    posid[s] = make_float4(fx, fy, fz, __int_as_float(id));
+    int wc = 0;
+    if (SK) {   // a replica that does not rebuild keeps its slots (s == o) and wrap codes
+        if (ENS && !rbflag[rep_of[id]]) wc = woff[o];
+        woff[s] = (unsigned char)wc;
+    }
+    posid[s] = make_float4(fx, fy, fz, id_word(id, wc));
     coor[s] = tfloat3{fx, fy, fz};
     nt_new[s] = nt;
     sp_new[s] = id;
@@ -498,32 +589,42 @@ __device__ __forceinline__ void wave_lds_sync()
 // lane skip the shift adds (x + 0 is x: the same bits).
 // The neighbour list of a wave is one block of 64 x NL_CAP entries, entry k
 // of lane l at 64 k + l (coalesced rows for the force pass).
-template <bool ENS, int TBS, int TSTAGE>
+// LB (skin mode, a list rebuild): the same scan writes the Verlet list of
+// every particle of a rebuilding replica (Bound2 walls included: every pair
+// within h + skin, VL_CAP entries per particle, blocks of VL_ROWS rows) and
+// computes no density.
+template <bool ENS, int TBS, int TSTAGE, bool LB = false>
 __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, const float4* __restrict__ posid, float4* __restrict__ velrho,
     const int* __restrict__ offs, const int* __restrict__ key, int* __restrict__ nl,
-    int* __restrict__ nlc, int* __restrict__ nlover)
+    int* __restrict__ nlc, int* __restrict__ nlover, const int* __restrict__ gate,
+    const int* __restrict__ rbflag, tfloat3* __restrict__ rbpos)
 {
+    constexpr int CAP = LB ? VL_CAP : NL_CAP;
+    constexpr int ROWS = LB ? VL_ROWS : NL_CAP;
+    if (LB && *gate == 0) return;   // no rebuild this step
     __shared__ float4 stage[TBS / 64][TSTAGE > 0 ? TSTAGE : 1];
     const int lane = threadIdx.x & 63;
     float4* st = stage[threadIdx.x >> 6];
     const int p = xcd_block((n + TBS - 1) / TBS) * TBS + threadIdx.x;
     const bool inb = p < n;
     const float4 pi = inb ? posid[p] : make_float4(0.f, 0.f, 0.f, 0.f);
-    const int id = inb ? __float_as_int(pi.w) : 0;
-    const bool act = inb && id >= R0.P.nb2;
+    const int id = inb ? pid_of(pi.w) : 0;
+    bool act = inb && id >= R0.P.nb2;
+    if (LB && ENS && act) act = rbflag[rep_of[id]] != 0;   // lists of the other replicas stay
     if (__ballot(act) == 0) return;   // wave-uniform
     with_rep<ENS>(R0, reps, rep_of, inb, id, [&](const GcRep& R) __attribute__((always_inline)) {
     const GcParams& P = R.P;
     int cx = 0, cy = 0, cz = 0;
     if (act) key_to_cc(key[p], R, cx, cy, cz);
-    const bool mov = act && id >= P.nf;
+    const bool mov = act && (LB || id >= P.nf);
     const float ih = 1.0f / P.h;
     float d = 0.0f;  // self term added below
     float sw = 0.f, svx = 0.f, svy = 0.f, svz = 0.f;
-    int* wl = nl + (size_t)__builtin_amdgcn_readfirstlane(p >> 6) * (NL_CAP * 64);
+    int* wl = nl + (size_t)__builtin_amdgcn_readfirstlane(p >> 6) * (ROWS * 64);
     unsigned woff = (unsigned)lane;   // = lane + 64 (entries so far)
-    const unsigned wlast = (unsigned)((NL_CAP - 1) * 64 + lane);   // overflow rewrites the last entry
+    const unsigned wlast = (unsigned)((CAP - 1) * 64 + lane);   // overflow rewrites the last entry
+    const float r2max = LB ? R.list_r2 : P.h2;
     auto hit = [&](int j, float r2, int idj, int code) {
         wl[min(woff, wlast)] = (j << 6) | code;
         woff += 64;
@@ -539,8 +640,13 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
             rx = pi.x - pj.x; ry = pi.y - pj.y; rz = pi.z - pj.z;
         }
         float r2 = rx * rx + ry * ry + rz * rz;
-        const int idj = __float_as_int(pj.w);
-        if (mov) {
+        const int idj = pid_of(pj.w);
+        if (LB) {
+            if (mov && r2 < r2max && j != p) {
+                wl[min(woff, wlast)] = (j << 6) | code;
+                woff += 64;
+            }
+        } else if (mov) {
             if (r2 < P.h2 && j != p) hit(j, r2, idj, code);
         } else if (r2 < P.h2 && idj >= P.nf) {
             float q = __builtin_amdgcn_sqrtf(r2) * ih, om = 1.0f - q;
@@ -585,7 +691,14 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
         if (__ballot(rr.ok && (rr.sy != 0.0f || rr.sz != 0.0f))) main_range(std::true_type{});
         else main_range(std::false_type{});
     }
-    if (mov) {
+    if (LB) {
+        if (mov) {
+            const int cnt = (int)((woff - (unsigned)lane) >> 6);
+            nlc[p] = cnt <= CAP ? cnt : -1;
+            if (cnt > CAP) atomicAdd(nlover, 1);
+            rbpos[p] = tfloat3{pi.x, pi.y, pi.z};   // displacements are measured from here
+        }
+    } else if (mov) {
         const int cnt = (int)((woff - (unsigned)lane) >> 6);
         velrho[p].w = P.m * P.wnorm * (d + 1.0f);
         nlc[p] = cnt <= NL_CAP ? cnt : -1;
@@ -602,7 +715,9 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
 // [10-13,17] SDPD pair force with the pressure from the EOS on the fly,
 // Bound1 repulsion, body force; corrector for fluid (membrane vertices get
 // it after their membrane force).
-template <bool ENS, int TBS>
+// VL (skin mode): the neighbours are the entries of the particle's Verlet
+// list whose bits the density pass set (r < h at this step), in list order.
+template <bool ENS, int TBS, bool VL = false>
 __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, int driveon, const int* __restrict__ act, const float4* __restrict__ posid,
     const float4* __restrict__ velrho, const int* __restrict__ offs, const int* __restrict__ key,
@@ -616,7 +731,8 @@ __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep*
     if (t >= nmov) return;
     const int p = act[t];
     float4 pi = posid[p];
-    int id = __float_as_int(pi.w);
+    int id = pid_of(pi.w);
+    const int oi = VL ? woff_of(pi.w) : 0;
     if (id < R0.P.nf) return;   // a wall slot (only with moving walls): no force
     with_rep<ENS>(R0, reps, rep_of, true, id, [&](const GcRep& R) __attribute__((always_inline)) {
     const GcParams& P = R.P;   // a reference: a copy of the batch record would live in scratch
@@ -642,16 +758,23 @@ __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep*
     auto pair = [&](int code) {
         int j = code >> 6;
         float4 pj = posid[j];
+        const int oj = VL ? woff_of(pj.w) : 0;
         float rx, ry, rz;
-        if (__ballot(code & 63)) {   // an image neighbour in the wave (x + 0 is x: same bits)
-            float sx = code_shift(code, L0), sy = code_shift(code >> 2, L1), sz = code_shift(code >> 4, L2);
+        if (__ballot((code & 63) | oj | oi)) {   // an image neighbour in the wave (x + 0 is x: same bits)
+            float sx, sy, sz;
+            if (VL) {
+                sx = eff_shift(code, oi, oj, 0, L0); sy = eff_shift(code, oi, oj, 1, L1);
+                sz = eff_shift(code, oi, oj, 2, L2);
+            } else {
+                sx = code_shift(code, L0); sy = code_shift(code >> 2, L1); sz = code_shift(code >> 4, L2);
+            }
             rx = pdiff(pi.x, pj.x, sx); ry = pdiff(pi.y, pj.y, sy); rz = pdiff(pi.z, pj.z, sz);
         } else {
             rx = pi.x - pj.x; ry = pi.y - pj.y; rz = pi.z - pj.z;
         }
         float r2 = rx * rx + ry * ry + rz * rz;
         if (r2 == 0.f) return;  // coincident particles exert no pair force
-        int idj = __float_as_int(pj.w);
+        int idj = pid_of(pj.w);
         float ir = __builtin_amdgcn_rsqf(r2);
         float r = r2 * ir;
         float om = 1.0f - r * ih;
@@ -689,7 +812,30 @@ __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep*
     // The density pass listed the neighbours (r < h, j != p) in the scan
     // order of the cubes, so the sums below run in that order either way.
     const int cnt = nlc[p];
-    if (cnt >= 0) {
+    if (cnt >= 0 && VL) {
+        const int* wl = nl + (size_t)(p >> 6) * (VL_ROWS * 64) + (p & 63);   // the list's block
+        const int nw = (cnt + 31) >> 5;
+        int w = 0;
+        unsigned m = nw > 0 ? (unsigned)wl[64 * VL_CAP] : 0u;
+        // the entry index of the next set bit, -1 after the last
+        auto next_k = [&]() -> int {
+            while (m == 0u) {
+                if (++w >= nw) return -1;
+                m = (unsigned)wl[64 * (VL_CAP + w)];
+            }
+            const int b = __builtin_ctz(m);
+            m &= m - 1u;
+            return 32 * w + b;
+        };
+        int k = next_k();
+        int code = k >= 0 ? wl[64 * k] : 0;   // the next entry is loaded during the current pair
+        while (k >= 0) {
+            const int cur = code;
+            k = next_k();
+            if (k >= 0) code = wl[64 * k];
+            pair(cur);
+        }
+    } else if (cnt >= 0) {
         const int* wl = nl + (size_t)(p >> 6) * (NL_CAP * 64) + (p & 63);   // the density lane's block
         int next = cnt > 0 ? wl[0] : 0;   // the next entry is loaded during the current pair
         for (int k = 0; k < cnt; ++k) {
@@ -697,11 +843,16 @@ __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep*
             if (k + 1 < cnt) next = wl[64 * (k + 1)];
             pair(code);
         }
-    } else {  // more than NL_CAP neighbours: scan the cubes again
+    } else {  // more than the list holds: scan the cubes again
         visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
             const int code = shift_code(sx, sy, sz);
             for (int j = jb; j < je; ++j) {
                 float4 pj = posid[j];
+                if (VL) {   // the cubes of the last rebuild, wraps since added
+                    const int oj = woff_of(pj.w);
+                    sx = eff_shift(code, oi, oj, 0, L0); sy = eff_shift(code, oi, oj, 1, L1);
+                    sz = eff_shift(code, oi, oj, 2, L2);
+                }
                 float rx = pdiff(pi.x, pj.x, sx), ry = pdiff(pi.y, pj.y, sy), rz = pdiff(pi.z, pj.z, sz);
                 float r2 = rx * rx + ry * ry + rz * rz;
                 if (r2 < P.h2 && j != p) pair((j << 6) | code);
@@ -720,22 +871,24 @@ __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep*
 }
 
 // [16] Morse between membrane vertices of different cells
-// Iterates the density pass's neighbour list (MaxCut <= CutRadius, so every
-// Morse partner is in it, in scan order); a vertex whose list overflowed
-// scans its cubes.
-template <bool ENS>
-// One thread per membrane vertex (its slot from CurPos).
+// Iterates the density pass's neighbour list, or in skin mode the Verlet
+// list (MaxCut <= CutRadius, so every Morse partner is in it, in scan
+// order); a vertex whose list overflowed scans its cubes.
+// One thread per membrane vertex (its slot from CurPos).  SK: skin mode
+// (Verlet lists, wrap codes).
+template <bool ENS, bool SK>
 __global__ __launch_bounds__(BS) void k2_agg(int nm, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, const int* __restrict__ curpos, const float4* __restrict__ posid,
     const int* __restrict__ offs, const int* __restrict__ key, const int* __restrict__ nl,
     const int* __restrict__ nlc, const int* __restrict__ celltab, tfloat3* __restrict__ force,
-    tfloat3* __restrict__ aggforce)
+    tfloat3* __restrict__ aggforce, int nl_stride)
 {
     const int v = xcd_block((nm + BS - 1) / BS) * BS + threadIdx.x;
     if (v >= nm) return;
     const int p = curpos[R0.P.nc + v];
     float4 pi = posid[p];
-    int id = __float_as_int(pi.w);
+    int id = pid_of(pi.w);
+    const int oi = SK ? woff_of(pi.w) : 0;
     with_rep<ENS>(R0, reps, rep_of, true, id, [&](const GcRep& R) __attribute__((always_inline)) {
     const GcParams& P = R.P;
     if (P.De == 0.0f) return;
@@ -744,10 +897,23 @@ __global__ __launch_bounds__(BS) void k2_agg(int nm, GcRep R0, const GcRep* __re
     key_to_cc(key[p], R, cx, cy, cz);
     const float tb = 2.0f * P.beta * P.De;
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    auto morse = [&](int j, float sx, float sy, float sz) {
+    // the image shift of the pair (list code or scanned cube), wraps since
+    // the last rebuild added in skin mode
+    auto shifts = [&](int code, float4 pj, float& sx, float& sy, float& sz) {
+        if (SK) {
+            const int oj = woff_of(pj.w);
+            sx = eff_shift(code, oi, oj, 0, R.L[0]); sy = eff_shift(code, oi, oj, 1, R.L[1]);
+            sz = eff_shift(code, oi, oj, 2, R.L[2]);
+        } else {
+            sx = code_shift(code, R.L[0]); sy = code_shift(code >> 2, R.L[1]); sz = code_shift(code >> 4, R.L[2]);
+        }
+    };
+    auto morse = [&](int j, int code) {
         float4 pj = posid[j];
-        int idj = __float_as_int(pj.w);
+        int idj = pid_of(pj.w);
         if (idj < P.nc) return;
+        float sx, sy, sz;
+        shifts(code, pj, sx, sy, sz);
         float rx = pdiff(pi.x, pj.x, sx), ry = pdiff(pi.y, pj.y, sy), rz = pdiff(pi.z, pj.z, sz);
         float r2 = rx * rx + ry * ry + rz * rz;
         if (r2 >= P.mcut2 || r2 == 0.f || celltab[idj - P.nc] == cell) return;
@@ -764,7 +930,7 @@ __global__ __launch_bounds__(BS) void k2_agg(int nm, GcRep R0, const GcRep* __re
         // cells are loaded from clamped indices before any is used (one entry
         // after another, each pair would wait out three dependent loads);
         // the pairs are then summed in list order
-        const int* wl = nl + (size_t)(p >> 6) * (NL_CAP * 64) + (p & 63);
+        const int* wl = nl + (size_t)(p >> 6) * nl_stride + (p & 63);
         constexpr int G = 8;
         for (int k0 = 0; k0 < cnt; k0 += G) {
             int code[G], cj[G];
@@ -774,14 +940,14 @@ __global__ __launch_bounds__(BS) void k2_agg(int nm, GcRep R0, const GcRep* __re
 #pragma unroll
             for (int u = 0; u < G; ++u) pj[u] = posid[code[u] >> 6];
 #pragma unroll
-            for (int u = 0; u < G; ++u) cj[u] = celltab[max(__float_as_int(pj[u].w) - P.nc, 0)];
+            for (int u = 0; u < G; ++u) cj[u] = celltab[max(pid_of(pj[u].w) - P.nc, 0)];
 #pragma unroll
             for (int u = 0; u < G; ++u) {
                 if (k0 + u >= cnt) break;
-                const int idj = __float_as_int(pj[u].w);
+                const int idj = pid_of(pj[u].w);
                 if (idj < P.nc) continue;
-                const float sx = code_shift(code[u], R.L[0]), sy = code_shift(code[u] >> 2, R.L[1]),
-                            sz = code_shift(code[u] >> 4, R.L[2]);
+                float sx, sy, sz;
+                shifts(code[u], pj[u], sx, sy, sz);
                 const float rx = pdiff(pi.x, pj[u].x, sx), ry = pdiff(pi.y, pj[u].y, sy),
                             rz = pdiff(pi.z, pj[u].z, sz);
                 const float r2 = rx * rx + ry * ry + rz * rz;
@@ -795,7 +961,8 @@ __global__ __launch_bounds__(BS) void k2_agg(int nm, GcRep R0, const GcRep* __re
         }
     } else {
         visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
-            for (int j = jb; j < je; ++j) morse(j, sx, sy, sz);
+            const int code = shift_code(sx, sy, sz);
+            for (int j = jb; j < je; ++j) morse(j, code);
         });
     }
     aggforce[id - P.nc] = tfloat3{fx, fy, fz};
@@ -805,8 +972,337 @@ __global__ __launch_bounds__(BS) void k2_agg(int nm, GcRep R0, const GcRep* __re
     });
 }
 
+// ---------------------------------------------------------------------------
+// Skin mode (Verlet lists).  Between rebuilds the slots, cube keys, CurPos
+// and the moving-slot map stay as the last rebuild left them, positions are
+// advanced in place and not wrapped (they stay continuous in that rebuild's
+// image frame, so the image codes of the lists stay valid; they are wrapped
+// at the next rebuild and on export), and the lists hold every pair within
+// h + skin at the rebuild.  Every particle then moved at most the summed
+// largest displacement since, so while that sum is <= skin / 2 every pair
+// within h is in the lists.
+
+__device__ __forceinline__ float wave_maxf(float v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// Largest squared displacement of a workgroup's particles into bmax[b] (and
+// its replica into brep[b], -1 when its particles belong to several: those
+// go to their replicas' words of disp2 by atomics, float bits of a
+// non-negative value ordering as unsigned integers).  Called by every
+// thread of the group; WAVES = waves per group.
+template <bool ENS, int WAVES>
+__device__ __forceinline__ void block_disp(bool live, float dd, int r, int b, float* __restrict__ bmax,
+                                           int* __restrict__ brep, unsigned* __restrict__ disp2)
+{
+    __shared__ float smax[WAVES];
+    __shared__ int srep[WAVES];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int r0 = 0;
+    bool mixed = false;
+    if constexpr (ENS) {
+        r0 = __builtin_amdgcn_readfirstlane(live ? r : -1);
+        if (r0 < 0) {   // the first lane is not live: take the first live lane's replica
+            const unsigned long long lv = __ballot(live);
+            r0 = lv ? __shfl(r, (int)__ffsll((long long)lv) - 1) : -1;
+        }
+        mixed = __ballot(live && r != r0) != 0;
+        if (mixed && live && dd > 0.0f) atomicMax(&disp2[r], __float_as_uint(dd));
+    }
+    const float m = wave_maxf(live && !mixed ? dd : 0.0f);
+    if (lane == 0) { smax[w] = m; srep[w] = mixed ? -1 : r0; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float bm = 0.0f;
+        int rep = -2;   // -2: no particle yet
+        bool split = false;
+        for (int k = 0; k < WAVES; ++k) {
+            bm = fmaxf(bm, smax[k]);
+            if (ENS && srep[k] >= 0) {
+                if (rep == -2) rep = srep[k];
+                else if (rep != srep[k]) split = true;
+            }
+        }
+        if (ENS && split) {   // several replicas in the group (rare): their own words
+            for (int k = 0; k < WAVES; ++k)
+                if (srep[k] >= 0 && smax[k] > 0.0f) atomicMax(&disp2[srep[k]], __float_as_uint(smax[k]));
+            bm = 0.0f;
+            rep = -1;
+        }
+        bmax[b] = bm;
+        if (ENS) brep[b] = rep < 0 ? -1 : rep;
+    }
+}
+
+// [1,2,5] skin mode, every step: r += dt v + dt^2/(2m) F in place (moving
+// particles and moving walls) with the periodic wrap and image counters of
+// the per-step gather (the same expressions: the same bits), the
+// Groot-Warren predictor (velhalf, velrho), the FP32 neighbour-loop copy of
+// the position (posid, with the particle's wraps since the last rebuild in
+// its wrap code; coor for membrane vertices, which the membrane kernels
+// read), and the largest squared displacement since the last rebuild (from
+// the position saved then) per workgroup.  Frozen walls change nothing.
+// Membrane vertices under substepping were moved and wrapped by the
+// previous step's substeps (which noted their displacement and wraps): they
+// take their velocity as is.
+template <bool ENS, int TBS>
+__global__ __launch_bounds__(TBS) void k3_advance(int n, GcRep R0, const GcRep* __restrict__ reps,
+    const int* __restrict__ rep_of, int mmove, int wmove, double* __restrict__ pos64,
+    const float4* __restrict__ vout, const tfloat3* __restrict__ force, const int* __restrict__ sp,
+    tint3* __restrict__ nt, unsigned char* __restrict__ woff, const tfloat3* __restrict__ rbpos,
+    float4* __restrict__ posid, tfloat3* __restrict__ coor, float4* __restrict__ velrho,
+    tfloat3* __restrict__ velhalf, float* __restrict__ bmax, int* __restrict__ brep,
+    unsigned* __restrict__ disp2)
+{
+    const int nb = (n + TBS - 1) / TBS;
+    const int b = xcd_block(nb);
+    if (b >= nb) return;   // a padding group of the XCD mapping (whole group)
+    const int p = b * TBS + threadIdx.x;
+    const bool inb = p < n;
+    const int id = inb ? sp[p] : 0;
+    float dd = 0.0f;
+    with_rep<ENS>(R0, reps, rep_of, inb, id, [&](const GcRep& R) __attribute__((always_inline)) {
+    if (!inb) return;
+    const GcParams& P = R.P;
+    const bool mv = moves(P, id, mmove), wmv = wall_moves(P, id, wmove);
+    if (!mv && !wmv && id < P.nf) return;   // a frozen wall
+    double u[3] = {pos64[p], pos64[n + p], pos64[2 * n + p]};
+    float4 v = vout[p];
+    int wc = woff[p];
+    if (mv || wmv) {
+        if (mv) {
+            const tfloat3 f = force[p];
+            advance_step(P, v, f, u);
+            float h = 0.5f * P.dt * P.inv_m, l = P.lam * P.dt * P.inv_m;
+            velhalf[p] = tfloat3{v.x + h * f.x, v.y + h * f.y, v.z + h * f.z};
+            v.x += l * f.x; v.y += l * f.y; v.z += l * f.z;
+            velrho[p] = v;
+        } else {
+            advance_step(P, wall_v4(P, u[1]), tfloat3{0.f, 0.f, 0.f}, u);
+        }
+        int img[3] = {0, 0, 0};
+        wrap_once(R, P, u, img);
+        pos64[p] = u[0]; pos64[n + p] = u[1]; pos64[2 * n + p] = u[2];
+        if (img[0] | img[1] | img[2]) {
+            tint3 t = nt[p];
+            t.x += img[0]; t.y += img[1]; t.z += img[2];
+            nt[p] = t;
+            wc = coef_code(code_coef(wc, 0) + img[0], code_coef(wc, 1) + img[1], code_coef(wc, 2) + img[2]);
+            woff[p] = (unsigned char)wc;
+        }
+        // displacement since the rebuild, in the rebuild's image frame
+        const tfloat3 r0 = rbpos[p];
+        const float dx = ((float)u[0] + (float)code_coef(wc, 0) * R.L[0]) - r0.x;
+        const float dy = ((float)u[1] + (float)code_coef(wc, 1) * R.L[1]) - r0.y;
+        const float dz = ((float)u[2] + (float)code_coef(wc, 2) * R.L[2]) - r0.z;
+        dd = dx * dx + dy * dy + dz * dz;
+        if (!(dd <= 3.0e38f)) dd = 3.0e38f;   // non-finite: forces a rebuild (which flags it)
+    } else {
+        velhalf[p] = tfloat3{v.x, v.y, v.z};
+        velrho[p] = v;
+    }
+    const float fx = (float)u[0], fy = (float)u[1], fz = (float)u[2];
+    posid[p] = make_float4(fx, fy, fz, id_word(id, wc));
+    if (id >= P.nc) coor[p] = tfloat3{fx, fy, fz};
+    });
+    block_disp<ENS, TBS / 64>(inb, dd, ENS && inb ? rep_of[id] : 0, b, bmax, brep, disp2);
+}
+
+// Skin mode, once per step after the advance: per replica the largest
+// displacement since its last rebuild (the advance's workgroup maxima, the
+// previous step's membrane substeps' from ms_off on, and the atomics of
+// groups spanning replicas); past half_skin (or non-finite) the replica's
+// lists are rebuilt this step.  Also advances the pair-noise counter (the
+// force pass reads the value before the advance).
+constexpr int DECIDE_BS = 1024, DECIDE_MAXREP = 1024;
+__global__ __launch_bounds__(DECIDE_BS) void k3_decide(int nrep, const GcRep* __restrict__ reps,
+    const float* __restrict__ bmax, const int* __restrict__ brep, int nb_adv, int ms_off, int nb_ms,
+    unsigned* __restrict__ disp2, int* __restrict__ rbflag, int* __restrict__ rbany, int* __restrict__ nrebuild,
+    unsigned long long* __restrict__ ctr)
+{
+    __shared__ unsigned smax[DECIDE_MAXREP];
+    __shared__ int any;
+    const int t = threadIdx.x;
+    const bool shared_ok = true;   // creation limits skin-mode ensembles to DECIDE_MAXREP replicas
+    for (int r = t; r < nrep && shared_ok; r += DECIDE_BS) smax[r] = 0u;
+    if (t == 0) { any = 0; *ctr += 1ull; }
+    __syncthreads();
+    float m = 0.0f;   // one replica: a plain maximum
+    auto take = [&](int k, float v) {
+        if (nrep == 1) { m = fmaxf(m, v); return; }
+        const int r = brep[k];
+        if (r < 0 || v <= 0.0f) return;
+        if (shared_ok) atomicMax(&smax[r], __float_as_uint(v));
+        else atomicMax(&disp2[r], __float_as_uint(v));
+    };
+    for (int k = t; k < nb_adv; k += DECIDE_BS) take(k, bmax[k]);
+    for (int k = t; k < nb_ms; k += DECIDE_BS) m = fmaxf(m, bmax[ms_off + k]);   // one suspension only
+    if (nrep == 1) {
+        m = wave_maxf(m);
+        if ((t & 63) == 0) atomicMax(&smax[0], __float_as_uint(m));
+    }
+    __syncthreads();
+    for (int r = t; r < nrep; r += DECIDE_BS) {
+        const float d2 = fmaxf(__uint_as_float(disp2[r]), shared_ok ? __uint_as_float(smax[r]) : 0.0f);
+        disp2[r] = 0u;
+        const int rb = !(sqrtf(d2) <= reps[r].half_skin);
+        rbflag[r] = rb;
+        if (rb) { nrebuild[r] += 1; atomicOr(&any, 1); }
+    }
+    __syncthreads();
+    if (t == 0) *rbany = any;
+}
+
+// [3] skin mode, a rebuild: cube key of each particle of a rebuilding
+// replica from its position; the other replicas keep their keys, hence
+// their slots.  Histogram rank as in the per-step key pass.
+template <bool ENS, int TBS>
+__global__ __launch_bounds__(TBS) void k3_keys(int n, GcRep R0, const GcRep* __restrict__ reps,
+    const int* __restrict__ rep_of, const int* __restrict__ gate, const int* __restrict__ rbflag,
+    const double* __restrict__ pos64, const int* __restrict__ sp, const int* __restrict__ key_old,
+    int* __restrict__ key, int* __restrict__ rank, int* __restrict__ cnt)
+{
+    if (*gate == 0) return;
+    const int p = xcd_block((n + TBS - 1) / TBS) * TBS + threadIdx.x;
+    const bool act = p < n;
+    if (__ballot(act) == 0) return;
+    int k = -1;
+    const int id = act ? sp[p] : 0;
+    with_rep<ENS>(R0, reps, rep_of, act, id, [&](const GcRep& R) __attribute__((always_inline)) {
+    if (!act) return;
+    if (ENS && !rbflag[rep_of[id]]) { k = key_old[p]; }
+    else {
+        const GcParams& P = R.P;
+        const double u[3] = {pos64[p], pos64[n + p], pos64[2 * n + p]};
+        int cx = cube_coord((float)u[0], R.bbox[0], P.off[0], R.invw[0], R.nc[0]);
+        int cy = cube_coord((float)u[1], R.bbox[2], P.off[1], R.invw[1], R.nc[1]);
+        int cz = cube_coord((float)u[2], R.bbox[4], P.off[2], R.invw[2], R.nc[2]);
+        k = R.cube_off + cx + R.nc[0] * (cy + R.nc[1] * cz);
+    }
+    key[p] = k;
+    });
+    hist_rank(act, k, p, rank, cnt);
+}
+
+// Skin mode, a rebuild: the gather wrote buffer 1; buffer 0 stays current.
+__global__ __launch_bounds__(BS) void k3_copyback(int n, const int* __restrict__ gate,
+    const double* __restrict__ pos1, double* __restrict__ pos0, const tint3* __restrict__ nt1,
+    tint3* __restrict__ nt0, const int* __restrict__ sp1, int* __restrict__ sp0)
+{
+    if (*gate == 0) return;
+    const int p = blockIdx.x * BS + threadIdx.x;
+    if (p >= n) return;
+    pos0[p] = pos1[p]; pos0[n + p] = pos1[n + p]; pos0[2 * n + p] = pos1[2 * n + p];
+    nt0[p] = nt1[p];
+    sp0[p] = sp1[p];
+}
+
+// [7,9] skin mode, every step: number density of moving particles and
+// velocity of Bound2 walls over their Verlet lists.  The wave runs through
+// the entries in lockstep (entry k of every lane is one coalesced row),
+// four entries' codes and then their positions loaded before use; r < h
+// selects this step's neighbours, whose bits go into one row per 32 entries
+// after the entries (read by the force pass).  The sums run in list order.
+// A particle whose list overflowed scans its cubes (the rebuild's cubes
+// are at least h + skin wide, so its neighbours are in the 27 around it).
+template <bool ENS, int TBS>
+__global__ __launch_bounds__(TBS) void k3_density(int n, GcRep R0, const GcRep* __restrict__ reps,
+    const int* __restrict__ rep_of, const float4* __restrict__ posid, float4* __restrict__ velrho,
+    const int* __restrict__ offs, const int* __restrict__ key, int* __restrict__ nl,
+    const int* __restrict__ nlc)
+{
+    const int lane = threadIdx.x & 63;
+    const int p = xcd_block((n + TBS - 1) / TBS) * TBS + threadIdx.x;
+    const bool inb = p < n;
+    const float4 pi = inb ? posid[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const int id = inb ? pid_of(pi.w) : 0;
+    const int oi = woff_of(pi.w);
+    const bool act = inb && id >= R0.P.nb2;
+    if (__ballot(act) == 0) return;   // wave-uniform
+    with_rep<ENS>(R0, reps, rep_of, inb, id, [&](const GcRep& R) __attribute__((always_inline)) {
+    const GcParams& P = R.P;
+    const bool mov = act && id >= P.nf;
+    const float ih = 1.0f / P.h;
+    const float L0 = R.L[0], L1 = R.L[1], L2 = R.L[2];
+    float d = 0.0f;  // self term added below
+    float sw = 0.f, svx = 0.f, svy = 0.f, svz = 0.f;
+    const int cnt = act ? nlc[p] : 0;
+    int* wl = nl + (size_t)__builtin_amdgcn_readfirstlane(p >> 6) * (VL_ROWS * 64) + lane;
+    auto take = [&](int j, float r2, int idj) {
+        float q = __builtin_amdgcn_sqrtf(r2) * ih, om = 1.0f - q;
+        float w = (1.0f + 3.0f * q) * om * om * om;
+        if (mov) {
+            d += idj >= P.nb2 ? w : 0.0f;
+        } else if (idj >= P.nf) {
+            float4 vj = velrho[j];
+            sw += w; svx += w * vj.x; svy += w * vj.y; svz += w * vj.z;
+        }
+    };
+    const int kmax = wave_max(cnt);
+    for (int k0 = 0; k0 < kmax; k0 += 32) {
+        unsigned m = 0u;
+        const int k1 = min(k0 + 32, kmax);
+        for (int k = k0; k < k1; k += 4) {
+            int c[4];
+            float4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) c[u] = cnt > 0 ? wl[64 * min(k + u, cnt - 1)] : 0;   // in bounds
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = posid[c[u] >> 6];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool ok = k + u < cnt;
+                const int oj = woff_of(q[u].w);
+                float rx, ry, rz;
+                if (__ballot(ok && ((c[u] & 63) | oi | oj))) {   // an image entry in the wave (x + 0 is x)
+                    const float sx = eff_shift(c[u], oi, oj, 0, L0), sy = eff_shift(c[u], oi, oj, 1, L1),
+                                sz = eff_shift(c[u], oi, oj, 2, L2);
+                    rx = pdiff(pi.x, q[u].x, sx); ry = pdiff(pi.y, q[u].y, sy); rz = pdiff(pi.z, q[u].z, sz);
+                } else {
+                    rx = pi.x - q[u].x; ry = pi.y - q[u].y; rz = pi.z - q[u].z;
+                }
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                if (ok && r2 < P.h2) {
+                    m |= 1u << (k + u - k0);
+                    take(c[u] >> 6, r2, pid_of(q[u].w));
+                }
+            }
+        }
+        if (mov && cnt > k0) wl[64 * (VL_CAP + (k0 >> 5))] = (int)m;
+    }
+    if (act && cnt < 0) {   // the list overflowed at the rebuild
+        int cx, cy, cz;
+        key_to_cc(key[p], R, cx, cy, cz);
+        visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
+            const int code = shift_code(sx, sy, sz);
+            for (int j = jb; j < je; ++j) {
+                const float4 pj = posid[j];
+                const int oj = woff_of(pj.w);   // the cubes of the last rebuild, wraps since added
+                const float ex = eff_shift(code, oi, oj, 0, L0), ey = eff_shift(code, oi, oj, 1, L1),
+                            ez = eff_shift(code, oi, oj, 2, L2);
+                const float rx = pdiff(pi.x, pj.x, ex), ry = pdiff(pi.y, pj.y, ey), rz = pdiff(pi.z, pj.z, ez);
+                const float r2 = rx * rx + ry * ry + rz * rz;
+                if (r2 < P.h2 && j != p) take(j, r2, pid_of(pj.w));
+            }
+        });
+    }
+    if (mov) {
+        velrho[p].w = P.m * P.wnorm * (d + 1.0f);
+    } else if (act && id < P.nf) {
+        const float3 u = wall_velocity(P, pi.y, sw, svx, svy, svz);
+        float4 v = velrho[p];
+        v.x = u.x; v.y = u.y; v.z = u.z;
+        velrho[p] = v;
+    }
+    });
+}
+
 // [18] wall velocity from the corrected velocities; walls' entry of vout
-template <bool ENS>
+// (SK: skin mode, the cubes of the last rebuild with the wraps since.)
+template <bool ENS, bool SK>
 __global__ __launch_bounds__(BS) void k2_wallvel(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, const float4* __restrict__ posid, float4* __restrict__ vout,
     const int* __restrict__ offs, const int* __restrict__ key)
@@ -814,7 +1310,8 @@ __global__ __launch_bounds__(BS) void k2_wallvel(int n, GcRep R0, const GcRep* _
     int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
     if (p >= n) return;
     float4 pi = posid[p];
-    int id = __float_as_int(pi.w);
+    int id = pid_of(pi.w);
+    const int oi = SK ? woff_of(pi.w) : 0;
     if (id >= R0.P.nf) return;
     const GcRep& R = rep_of_id<ENS>(R0, reps, rep_of, id);
     const GcParams& P = R.P;
@@ -824,11 +1321,17 @@ __global__ __launch_bounds__(BS) void k2_wallvel(int n, GcRep R0, const GcRep* _
     const float ih = 1.0f / P.h;
     float sw = 0.f, svx = 0.f, svy = 0.f, svz = 0.f;
     visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
+        const int code = shift_code(sx, sy, sz);
         for (int j = jb; j < je; ++j) {
             float4 pj = posid[j];
+            if (SK) {
+                const int oj = woff_of(pj.w);
+                sx = eff_shift(code, oi, oj, 0, R.L[0]); sy = eff_shift(code, oi, oj, 1, R.L[1]);
+                sz = eff_shift(code, oi, oj, 2, R.L[2]);
+            }
             float rx = pdiff(pi.x, pj.x, sx), ry = pdiff(pi.y, pj.y, sy), rz = pdiff(pi.z, pj.z, sz);
             float r2 = rx * rx + ry * ry + rz * rz;
-            if (r2 < P.h2 && __float_as_int(pj.w) >= P.nf) {
+            if (r2 < P.h2 && pid_of(pj.w) >= P.nf) {
                 float q = __builtin_amdgcn_sqrtf(r2) * ih, om = 1.0f - q;
                 float w = (1.0f + 3.0f * q) * om * om * om;
                 float4 vj = vout[j];
@@ -906,19 +1409,19 @@ int launch_build_t(gc_engine* e, int move, Prof* pr = nullptr)
     }
     GC_LAUNCH_CHECK("engine integrate");
     mark(pr, s, "integrate+key");
-    k2_tile_sums<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum);
+    k2_tile_sums<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, nullptr);
     k2_cube_offsets<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, e->d_offs, e->d_ctr, move,
-                                                      e->d_nlover);
+                                                      e->d_nlover, nullptr);
     GC_LAUNCH_CHECK("engine cube offsets");
     mark(pr, s, "cube offsets");
     k2_scatter<<<gb, BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[c], e->d_offs, e->d_tk, e->d_tv,
-                                 e->d_to, e->d_cnt);
+                                 e->d_to, e->d_cnt, nullptr);
     GC_LAUNCH_CHECK("engine scatter");
-    k2_sortgather<ENS><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, mfree, e->moving_walls,
+    k2_sortgather<ENS, false><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, mfree, e->moving_walls,
         e->d_tk, e->d_tv,
         e->d_to, e->d_offs, e->d_curpos, e->d_pos64[c], e->d_pos64[nx], e->d_vout, e->d_force, e->d_nt[c],
         e->d_nt[nx], e->d_sp[nx], e->d_key, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf,
-        e->moving_walls ? nullptr : e->d_wincl, e->d_act, e->d_fault);
+        e->moving_walls ? nullptr : e->d_wincl, e->d_act, e->d_fault, nullptr, nullptr, nullptr);
     GC_LAUNCH_CHECK("engine sort/gather");
     e->cur = nx;
     return GC_OK;
@@ -931,23 +1434,188 @@ int launch_build(gc_engine* e, int move)
     return e->ens() ? launch_build_t<true>(e, move) : launch_build_t<false>(e, move);
 }
 
+// [16] Morse (list iteration, or the cube scan for overflowed lists)
+template <bool ENS>
+void launch_agg(gc_engine* e)
+{
+    auto L = [&](auto kern) {
+        kern<<<xcd_grid(blocks_for(e->nm, BS)), BS, 0, e->s>>>(e->nm, e->hreps[0], e->d_reps, e->d_rep_of,
+            e->d_curpos, e->d_posid, e->d_offs, e->d_key, e->d_nl, e->d_nlc, e->d_celltab, e->d_force,
+            e->d_aggforce, nl_stride(e));
+    };
+    if (e->skin_mode()) L(k2_agg<ENS, true>);
+    else L(k2_agg<ENS, false>);
+}
+
+template <bool ENS>
+int skin_lists_t(gc_engine* e, const int* gate, const int* rbflag);
+
+// Skin mode: re-sort and list build of the replicas whose flag (rbflag)
+// is set, all kernels returning at once unless *gate is set (decided on the
+// device each step, or an all-ones flag for a host-driven rebuild).
+template <bool ENS>
+int skin_rebuild_t(gc_engine* e, const int* gate, const int* rbflag)
+{
+    const int N = e->N;
+    hipStream_t s = e->s;
+    const int gb = xcd_grid(blocks_for(N, BS));
+    const GcRep& R0 = e->hreps[0];
+    const int mfree = e->ms == nullptr;
+    k3_keys<ENS, 256><<<xcd_grid(blocks_for(N, 256)), 256, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, gate, rbflag,
+        e->d_pos64[0], e->d_sp[0], e->d_key, e->d_key0, e->d_rank, e->d_cnt);
+    k2_tile_sums<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, gate);
+    k2_cube_offsets<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, e->d_offs, e->d_ctr, 0,
+                                                      e->d_nlover, gate);
+    k2_scatter<<<gb, BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[0], e->d_offs, e->d_tk, e->d_tv,
+                                 e->d_to, e->d_cnt, gate);
+    k2_sortgather<ENS, true><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, 0, mfree, e->moving_walls,
+        e->d_tk, e->d_tv, e->d_to, e->d_offs, e->d_curpos, e->d_pos64[0], e->d_pos64[1], e->d_vout,
+        e->d_force, e->d_nt[0], e->d_nt[1], e->d_sp[1], e->d_key, e->d_posid, e->d_coor, e->d_velrho,
+        e->d_velhalf, e->moving_walls ? nullptr : e->d_wincl, e->d_act, e->d_fault, gate, rbflag, e->d_woff);
+    k3_copyback<<<blocks_for(N, BS), BS, 0, s>>>(N, gate, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
+                                                 e->d_nt[0], e->d_sp[1], e->d_sp[0]);
+    GC_LAUNCH_CHECK("engine skin re-sort");
+    return skin_lists_t<ENS>(e, gate, rbflag);
+}
+
+// Skin mode: the Verlet lists of the replicas whose flag is set (gated).
+template <bool ENS>
+int skin_lists_t(gc_engine* e, const int* gate, const int* rbflag)
+{
+    const int N = e->N;
+    hipStream_t s = e->s;
+    const GcRep& R0 = e->hreps[0];
+    {
+        auto L = [&](auto kern, int tbs) {
+            kern<<<xcd_grid(blocks_for(N, tbs)), tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid,
+                e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc, e->d_nlover, gate, rbflag, e->d_rbpos);
+        };
+        switch (e->tune.v[TUNE_DENSITY]) {   // block size x LDS row stage
+            case 0: L(k2_density<ENS, 64, 0, true>, 64); break;
+            case 1: L(k2_density<ENS, 64, 128, true>, 64); break;
+            case 2: L(k2_density<ENS, 64, 192, true>, 64); break;
+            case 3: L(k2_density<ENS, 64, 320, true>, 64); break;
+            case 4: L(k2_density<ENS, 128, 0, true>, 128); break;
+            case 5: L(k2_density<ENS, 128, 128, true>, 128); break;
+            case 6: L(k2_density<ENS, 128, 192, true>, 128); break;
+            case 7: L(k2_density<ENS, 128, 320, true>, 128); break;
+            case 8: L(k2_density<ENS, 256, 0, true>, 256); break;
+            case 9: L(k2_density<ENS, 256, 128, true>, 256); break;
+            case 11: L(k2_density<ENS, 256, 320, true>, 256); break;
+            default: L(k2_density<ENS, 256, 192, true>, 256); break;
+        }
+    }
+    GC_LAUNCH_CHECK("engine list build");
+    return GC_OK;
+}
+
+// Skin mode, after a host-driven build into buffer 1 (creation, SetState):
+// buffer 0 made current, every replica's lists built, the displacement
+// bounds cleared.
+int skin_init(gc_engine* e)
+{
+    if (!e->skin_mode()) return GC_OK;
+    const int N = e->N;
+    hipStream_t s = e->s;
+    if (e->cur == 1) {
+        k3_copyback<<<blocks_for(N, BS), BS, 0, s>>>(N, e->d_ones, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
+                                                     e->d_nt[0], e->d_sp[1], e->d_sp[0]);
+        GC_LAUNCH_CHECK("engine skin copy");
+        e->cur = 0;
+    }
+    int st = e->ens() ? skin_lists_t<true>(e, e->d_ones, e->d_ones) : skin_lists_t<false>(e, e->d_ones, e->d_ones);
+    if (st) return st;
+    const int nr = e->nrep;
+    if ((st = hip_check(hipMemsetAsync(e->d_woff, 0, (size_t)N, s), "skin init")) ||
+        (st = hip_check(hipMemsetAsync(e->d_disp2, 0, sizeof(unsigned) * nr, s), "skin init")) ||
+        (st = hip_check(hipMemsetAsync(e->d_bmax, 0, sizeof(float) * skin_bmax_size(e), s), "skin init")) ||
+        (st = hip_check(hipMemsetAsync(e->d_rbflag, 0, sizeof(int) * nr, s), "skin init")))
+        return st;
+    return hip_check(hipStreamSynchronize(s), "skin init sync");
+}
+
+// Skin mode: [1,2,5] advance and predictor, the rebuild decision, the
+// rebuild (gated), [7,9] density over the lists, [10-13,17] force over the
+// neighbour bits.
+template <bool ENS>
+int skin_fluid_t(gc_engine* e, int driveon, Prof* pr)
+{
+    const GcRep& R0 = e->hreps[0];
+    const int N = e->N;
+    hipStream_t s = e->s;
+    const int mfree = e->ms == nullptr;
+    int nb_adv = 0;
+    {
+        auto L = [&](auto kern, int tbs) {
+            kern<<<xcd_grid(blocks_for(N, tbs)), tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, mfree,
+                e->moving_walls, e->d_pos64[0], e->d_vout, e->d_force, e->d_sp[0], e->d_nt[0], e->d_woff,
+                e->d_rbpos, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf, e->d_bmax, e->d_brep, e->d_disp2);
+            nb_adv = blocks_for(N, tbs);
+        };
+        switch (e->tune.v[TUNE_INTEGRATE]) {
+            case 0: L(k3_advance<ENS, 64>, 64); break;
+            case 1: L(k3_advance<ENS, 128>, 128); break;
+            default: L(k3_advance<ENS, 256>, 256); break;
+        }
+    }
+    // the substeps of the previous step wrote their vertices' maxima after
+    // the advance's region (membrane substepping: one suspension)
+    const int nb_ms = e->ms ? blocks_for(e->nm, BS) : 0;
+    k3_decide<<<1, DECIDE_BS, 0, s>>>(e->nrep, e->d_reps, e->d_bmax, e->d_brep, nb_adv, skin_ms_off(e), nb_ms,
+                                      e->d_disp2, e->d_rbflag, e->d_rbany, e->d_nrebuild, e->d_ctr);
+    GC_LAUNCH_CHECK("engine advance");
+    mark(pr, s, "advance+predict");
+    int st = skin_rebuild_t<ENS>(e, e->d_rbany, e->d_rbflag);
+    if (st) return st;
+    if ((st = debug_check_slot_maps(e))) return st;
+    mark(pr, s, "list rebuild");
+    {
+        auto L = [&](auto kern, int tbs) {
+            kern<<<xcd_grid(blocks_for(N, tbs)), tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid,
+                e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc);
+        };
+        const int v = e->tune.v[TUNE_DENSITY];
+        if (v < 4) L(k3_density<ENS, 64>, 64);
+        else if (v < 8) L(k3_density<ENS, 128>, 128);
+        else L(k3_density<ENS, 256>, 256);
+    }
+    GC_LAUNCH_CHECK("engine density (lists)");
+    mark(pr, s, "density+wall velocity");
+    {
+        auto L = [&](auto kern, int tbs) {
+            if (e->nmov > 0)
+                kern<<<xcd_grid(blocks_for(e->nmov, tbs)), tbs, 0, s>>>(e->nmov, R0, e->d_reps, e->d_rep_of,
+                    driveon, e->d_act, e->d_posid, e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc,
+                    e->d_velhalf, e->d_force, e->d_vout, e->d_ctr, e->d_fault);
+        };
+        switch (e->tune.v[TUNE_FORCE]) {
+            case 0: L(k2_force<ENS, 64, true>, 64); break;
+            case 1: L(k2_force<ENS, 128, true>, 128); break;
+            default: L(k2_force<ENS, 256, true>, 256); break;
+        }
+    }
+    GC_LAUNCH_CHECK("engine force");
+    return GC_OK;
+}
+
 template <bool ENS>
 int launch_step_t(gc_engine* e, int driveon, Prof* pr)
 {
     const GcRep& R0 = e->hreps[0];
     const int N = e->N;
     hipStream_t s = e->s;
-    const int gb = xcd_grid(blocks_for(N, BS));
     mark(pr, s, "start");
-    int st = launch_build_t<ENS>(e, 1, pr);
-    if (st) return st;
+    int st;
+    if (e->skin_mode()) {
+        if ((st = skin_fluid_t<ENS>(e, driveon, pr))) return st;
+    } else {
+    if ((st = launch_build_t<ENS>(e, 1, pr))) return st;
     if ((st = debug_check_slot_maps(e))) return st;
     mark(pr, s, "sort+gather+predict");
-    const int c = e->cur;
     {
         auto L = [&](auto kern, int tbs) {
             kern<<<xcd_grid(blocks_for(N, tbs)), tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid,
-                e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc, e->d_nlover);
+                e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc, e->d_nlover, nullptr, nullptr, nullptr);
         };
         switch (e->tune.v[TUNE_DENSITY]) {   // block size x LDS row stage
             case 0: L(k2_density<ENS, 64, 0>, 64); break;
@@ -980,13 +1648,12 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
         }
     }
     GC_LAUNCH_CHECK("engine force");
+    }
     mark(pr, s, "sdpd force");
+    const int c = e->cur;
     if (e->ncell > 0 && e->ms) {
         if (e->any_morse) {
-            k2_agg<ENS><<<xcd_grid(blocks_for(e->nm, BS)), BS, 0, s>>>(e->nm, R0, e->d_reps, e->d_rep_of,
-                                          e->d_curpos, e->d_posid, e->d_offs,
-                                          e->d_key, e->d_nl, e->d_nlc, e->d_celltab, e->d_force,
-                                          e->d_aggforce);
+            launch_agg<ENS>(e);
             GC_LAUNCH_CHECK("engine morse");
         }
         mark(pr, s, "morse");
@@ -995,10 +1662,7 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
     } else if (e->ncell > 0) {
         const GcParams& P = R0.P;
         if (e->any_morse) {
-            k2_agg<ENS><<<xcd_grid(blocks_for(e->nm, BS)), BS, 0, s>>>(e->nm, R0, e->d_reps, e->d_rep_of,
-                                          e->d_curpos, e->d_posid, e->d_offs,
-                                          e->d_key, e->d_nl, e->d_nlc, e->d_celltab, e->d_force,
-                                          e->d_aggforce);
+            launch_agg<ENS>(e);
             GC_LAUNCH_CHECK("engine morse");
         }
         mark(pr, s, "morse");
@@ -1081,12 +1745,13 @@ int finish_walls(gc_engine* e)
 {
     if (!e->walls_dirty) return GC_OK;
     const int gb = xcd_grid(blocks_for(e->N, BS));
-    if (e->ens())
-        k2_wallvel<true><<<gb, BS, 0, e->s>>>(e->N, e->hreps[0], e->d_reps, e->d_rep_of, e->d_posid,
-                                             e->d_vout, e->d_offs, e->d_key);
-    else
-        k2_wallvel<false><<<gb, BS, 0, e->s>>>(e->N, e->hreps[0], e->d_reps, e->d_rep_of, e->d_posid,
-                                              e->d_vout, e->d_offs, e->d_key);
+    auto L = [&](auto kern) {
+        kern<<<gb, BS, 0, e->s>>>(e->N, e->hreps[0], e->d_reps, e->d_rep_of, e->d_posid, e->d_vout, e->d_offs,
+                                  e->d_key);
+    };
+    if (e->ens()) { if (e->skin_mode()) L(k2_wallvel<true, true>); else L(k2_wallvel<true, false>); }
+    else if (e->skin_mode()) L(k2_wallvel<false, true>);
+    else L(k2_wallvel<false, false>);
     GC_LAUNCH_CHECK("engine wall velocity");
     e->walls_dirty = false;
     return GC_OK;
@@ -1145,7 +1810,17 @@ int wall_tables(gc_engine* e)
     // moving walls change cubes: the force pass then runs over every slot
     e->nmov = e->moving_walls ? N : N - nwall;
     if ((st = upload(e->d_wincl, w.data(), (size_t)K, e->s))) return st;
-    return hip_check(hipStreamSynchronize(e->s), "wall table sync");   // w is freed on return
+    std::vector<int> act;
+    if (e->skin_mode()) {
+        // the build that preceded this table wrote the moving-slot map
+        // without it, and skin mode re-sorts only at a list rebuild: write
+        // the map here (the moving particles' slots in slot order)
+        act.reserve((size_t)e->nmov);
+        for (int s = 0; s < N; ++s)
+            if (e->moving_walls || sp[s] >= nf) act.push_back(s);
+        if ((st = upload(e->d_act, act.data(), act.size(), e->s))) return st;
+    }
+    return hip_check(hipStreamSynchronize(e->s), "wall table sync");   // w, act are freed on return
 }
 
 int validate_desc(const gc_case_desc* d)
@@ -1167,6 +1842,8 @@ int validate_desc(const gc_case_desc* d)
         return set_error(GC_ERR_GRID, "EngineCreate: MaxCut must not exceed CutRadius");
     if (!(d->ext.wlc_ratio > 1.0f))
         return set_error(GC_ERR_INVALID, "EngineCreate: wlc_ratio must be > 1");
+    if (!(d->skin == 0.0f || (d->skin >= 0.01f && d->skin <= 10.0f * c.CutRadius)))
+        return set_error(GC_ERR_INVALID, "EngineCreate: skin must be 0 or in [0.01, 10 CutRadius]");
     for (int cc = 0; cc < d->nCell; ++cc)
         if (d->beginendtri[cc].x < 0 || d->beginendtri[cc].y >= d->nAllTri ||
             d->beginendtri[cc].x > d->beginendtri[cc].y)
@@ -1185,6 +1862,15 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
     if (!descs || nrep < 1) { *status = set_error(GC_ERR_INVALID, "EngineCreate: no case"); return nullptr; }
     for (int r = 0; r < nrep; ++r)
         if ((*status = validate_desc(&descs[r]))) return nullptr;
+    if (descs[0].skin > 0.0f && nrep > 1024) {
+        *status = set_error(GC_ERR_INVALID, "EnsembleCreate: skin mode takes at most 1024 replicas");
+        return nullptr;
+    }
+    for (int r = 1; r < nrep; ++r)
+        if ((descs[r].skin > 0.0f) != (descs[0].skin > 0.0f)) {
+            *status = set_error(GC_ERR_INVALID, "EnsembleCreate: replicas must all use a skin or none");
+            return nullptr;
+        }
     for (int r = 1; r < nrep; ++r)
         if (descs[r].ext.noise_counter != descs[0].ext.noise_counter) {
             *status = set_error(GC_ERR_INVALID, "EnsembleCreate: replicas must share the noise counter");
@@ -1245,8 +1931,18 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
         }
         int Kr = 0;
         float outer[3];
-        if ((st = gc_ConfigDomain(d.bbox, d.consts.CutRadius, d.ext.periodic, R.nc, outer, R.invw, &Kr)))
+        // skin mode: cubes wider than the list radius h + skin (with a margin
+        // for the FP32 key rounding), padded by their width on walled axes
+        const float cut = d.skin > 0.0f ? (d.consts.CutRadius + d.skin) * 1.0001f + 1.0e-3f : d.consts.CutRadius;
+        if ((st = gc_ConfigDomain(d.bbox, cut, d.ext.periodic, R.nc, outer, R.invw, &Kr)))
             return fail(st);
+        if (d.skin > 0.0f) {
+            for (int q = 0; q < 3; ++q) R.P.off[q] = d.ext.periodic[q] ? 0.0f : cut;
+            const float rl = d.consts.CutRadius + d.skin;
+            R.list_r2 = rl * rl;
+            // FP32 positions and distances carry rounding errors well below 1e-3
+            R.half_skin = 0.5f * d.skin - 1.0e-3f;
+        }
         for (int q = 0; q < 3; ++q) R.L[q] = outer[q];
         memcpy(R.bbox, d.bbox, sizeof(R.bbox));
         R.cube_off = (int)koff;
@@ -1261,6 +1957,8 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
     }
     if (koff > (1L << 30)) return fail(set_error(GC_ERR_GRID, "EngineCreate: too many cubes"));
     e->K = (int)koff;
+    e->skin = descs[0].skin > 0.0f ? descs[0].skin : 0.0f;
+    e->nlcap = e->skin_mode() ? VL_CAP : NL_CAP;
     e->ntile = (e->K + CUBE_TILE - 1) / CUBE_TILE;
     if ((st = hip_check(hipStreamCreateWithFlags(&e->s, hipStreamNonBlocking), "engine stream")))
         return fail(st);
@@ -1281,7 +1979,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
         (st = e->alloc(&e->d_rank, N)) || (st = e->alloc(&e->d_cnt, K + 1)) ||
         (st = e->alloc(&e->d_offs, K + 1)) || (st = e->alloc(&e->d_tk, N)) ||
         (st = e->alloc(&e->d_tv, N)) || (st = e->alloc(&e->d_to, N)) || (st = e->alloc(&e->d_curpos, N)) ||
-        (st = e->alloc(&e->d_nl, (size_t)NL_CAP * (((size_t)N + 63) & ~(size_t)63))) || (st = e->alloc(&e->d_nlc, N)) ||
+        (st = e->alloc(&e->d_nl, (size_t)nl_stride(e) * (((size_t)N + 63) / 64))) || (st = e->alloc(&e->d_nlc, N)) ||
         (st = e->alloc(&e->d_ctr, 1)) || (st = e->alloc(&e->d_fault, 1)) || (st = e->alloc(&e->d_nlover, 1)) ||
         (st = hip_check(hipMemset(e->d_fault, 0, sizeof(int)), "engine fault word")) ||
         (st = hip_check(hipMemset(e->d_nlover, 0, sizeof(int)), "engine overflow counter")) ||
@@ -1290,6 +1988,19 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
         (st = hip_check(hipMemset(e->d_wincl, 0, sizeof(int) * (K + 1)), "engine wall table")) ||
         (st = hip_check(hipMemset(e->d_act, 0, sizeof(int) * N), "engine moving slots")))
         return fail(st);
+    if (e->skin_mode()) {
+        const size_t nr = (size_t)nrep;
+        std::vector<int> ones(nr, 1), zero(nr, 0);
+        if ((st = e->alloc(&e->d_disp2, nr)) || (st = e->alloc(&e->d_rbpos, (size_t)N)) ||
+            (st = e->alloc(&e->d_bmax, skin_bmax_size(e))) || (st = e->alloc(&e->d_brep, skin_bmax_size(e))) ||
+            (st = e->alloc(&e->d_rbflag, nr)) || (st = e->alloc(&e->d_rbany, 1)) ||
+            (st = e->alloc(&e->d_nrebuild, nr)) || (st = e->alloc(&e->d_ones, nr)) ||
+            (st = e->alloc(&e->d_woff, (size_t)N)) ||
+            (st = hip_check(hipMemset(e->d_woff, 0, (size_t)N), "engine wrap codes")) ||
+            (st = upload(e->d_ones, ones.data(), nr, e->s)) || (st = upload(e->d_nrebuild, zero.data(), nr, e->s)) ||
+            (st = upload(e->d_rbany, zero.data(), 1, e->s)))
+            return fail(st);
+    }
     hipStream_t s = e->s;
     std::vector<float4> vout_id(N);
     std::vector<tfloat3> vh_id(N), f_id(N, tfloat3{0.f, 0.f, 0.f});
@@ -1330,8 +2041,15 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
     }
     // first neighbour build (Particle::ConCubeList at init, Particle.cu:570)
     if ((st = launch_build(e, 0))) return fail(st);
+    if (e->skin_mode() && e->cur == 1) {   // buffer 0 stays current in skin mode
+        k3_copyback<<<blocks_for(e->N, BS), BS, 0, s>>>(e->N, e->d_ones, e->d_pos64[1], e->d_pos64[0],
+            e->d_nt[1], e->d_nt[0], e->d_sp[1], e->d_sp[0]);
+        if ((st = hip_check(hipGetLastError(), "engine skin copy"))) return fail(st);
+        e->cur = 0;
+    }
     if ((st = reorder_carried(e, vout_id, f_id, vh_id))) return fail(st);
     if ((st = wall_tables(e))) return fail(st);
+    if ((st = skin_init(e))) return fail(st);
 
     if (e->ncell > 0) {
         const size_t T = (size_t)e->ntri, E = (size_t)e->nedge, C = (size_t)e->ncell, M = (size_t)e->nm;
@@ -1584,6 +2302,14 @@ extern "C" int gc_EngineInfo(const gc_engine* e, gc_engine_info* info)
         (st = hip_check(hipMemcpy(&info->nl_overflow, e->d_nlover, sizeof(int), hipMemcpyDeviceToHost),
                         "EngineInfo D2H")))
         return st;
+    if (e->skin_mode()) {
+        std::vector<int> nrb((size_t)e->nrep);
+        if ((st = hip_check(hipMemcpy(nrb.data(), e->d_nrebuild, sizeof(int) * e->nrep, hipMemcpyDeviceToHost),
+                            "EngineInfo D2H")))
+            return st;
+        for (int v : nrb) info->nrebuild += v;
+    }
+    info->skin = e->skin;
     info->nTotalPar = e->N; info->nCellPar = e->nm; info->nCell = e->ncell; info->sizecube = e->K;
     for (int d = 0; d < 3; ++d) {
         info->ncube[d] = e->hreps[0].nc[d]; info->outerregion[d] = e->hreps[0].L[d];
@@ -1645,6 +2371,10 @@ int snapshot(gc_engine* e, Snapshot& S)
     S.vo.resize(N); S.vh.resize(N); S.f.resize(N); S.co.resize(N); S.nt.resize(N);
     for (int slot = 0; slot < N; ++slot) {
         int id = sp[slot];
+        // skin mode keeps the FP32 copy of moving particles in posid (coor
+        // only for membrane vertices): the same float of the FP64 position
+        if (e->skin_mode())
+            co[slot] = tfloat3{(float)pos[slot], (float)pos[(size_t)N + slot], (float)pos[2 * (size_t)N + slot]};
         for (int d = 0; d < 3; ++d) S.pos[3 * (size_t)id + d] = pos[(size_t)d * N + slot];
         // the force pass leaves wall slots alone: walls carry no force
         S.vo[id] = vo[slot]; S.vh[id] = vh[slot]; S.co[id] = co[slot];
@@ -1733,8 +2463,15 @@ int restore_state(gc_engine* e, const std::vector<double>& pos, const std::vecto
         (st = upload(e->d_ctr, &ctr, 1, s)) || (st = upload(e->d_fault, &zero, 1, s)))
         return st;
     if ((st = launch_build(e, 0))) return st;
+    if (e->skin_mode() && e->cur == 1) {   // buffer 0 stays current in skin mode
+        k3_copyback<<<blocks_for(N, BS), BS, 0, s>>>(N, e->d_ones, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
+                                                     e->d_nt[0], e->d_sp[1], e->d_sp[0]);
+        if ((st = hip_check(hipGetLastError(), "engine skin copy"))) return st;
+        e->cur = 0;
+    }
     if ((st = reorder_carried(e, vo, f, vh))) return st;
     if ((st = wall_tables(e))) return st;
+    if ((st = skin_init(e))) return st;
     e->walls_dirty = false;
     e->step = step;
     if (e->ms && (st = ms_reset(e))) return st;
@@ -1901,7 +2638,8 @@ extern "C" int gc_EngineAutotune(gc_engine* e, int steps, int driveon, gc_tune_r
             e->tune.v[f] = v;
             float us = 0.f;
             if ((st = launch_step(e, driveon, nullptr))) break;   // warm-up (code load)
-            if ((st = phase_us(e, steps, driveon, TUNE_PHASE[f], TUNE_PHASE2[f], &us))) break;
+            const char* ph = f == TUNE_INTEGRATE && e->skin_mode() ? "advance+predict" : TUNE_PHASE[f];
+            if ((st = phase_us(e, steps, driveon, ph, TUNE_PHASE2[f], &us))) break;
             if (us < best) { best = us; bestv = v; }
             if (out && row < 64) {
                 snprintf(out->family[row], sizeof(out->family[row]), "%s", TUNE_FAMILY[f]);

@@ -76,6 +76,22 @@ struct gc_engine {
     int* d_tsum = nullptr;   // cube counts per scan tile
     int ntile = 0;
     int *d_nl, *d_nlc;   // neighbour lists of moving particles (density -> force)
+    int nlcap = 64;      // list entries per particle (skin mode: 128)
+    // Verlet skin mode (skin > 0): slots, keys and lists change only at a
+    // rebuild; buffer 0 is always current (the rebuild gathers into buffer
+    // 1 and copies back).  Per replica: the largest squared displacement
+    // since the last rebuild from workgroups spanning replicas (float bits),
+    // the rebuild flag of this step and the rebuild count; rbany = some
+    // replica rebuilds this step (gates the rebuild kernels).
+    float skin = 0.0f;
+    unsigned* d_disp2 = nullptr;
+    tfloat3* d_rbpos = nullptr;   // per slot: FP32 position at the last rebuild
+    float* d_bmax = nullptr;      // per workgroup: largest squared displacement since the rebuild
+    int* d_brep = nullptr;        // per workgroup: its replica (-1: several)
+    int *d_rbflag = nullptr, *d_rbany = nullptr, *d_nrebuild = nullptr;
+    int* d_ones = nullptr;   // an all-ones flag array (host-driven rebuilds of every replica)
+    unsigned char* d_woff = nullptr;   // per slot: periodic wraps since the last rebuild (image code)
+    bool skin_mode() const { return skin > 0.0f; }
     unsigned long long* d_ctr;
     int* d_fault = nullptr;    // GC_FAULT_* bits raised by the kernels (sticky)
     int* d_nlover = nullptr;   // density-pass list overflows of the last step

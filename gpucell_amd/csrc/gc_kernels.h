@@ -23,6 +23,12 @@ struct GcRep {
     int cube_off;
     int id_off[4];      // first global id of the replica's Bound1, Bound2, fluid, membrane
     int loc_start[4];   // the same blocks' first ids in the replica's own numbering
+    // Verlet skin mode (skin > 0): neighbour lists hold every pair within
+    // h + skin at the last rebuild (list_r2 = (h + skin)^2); a rebuild is
+    // due once the sum over steps of the largest displacement exceeds
+    // half_skin (skin / 2 less a rounding margin).  0 in per-step mode.
+    float list_r2;
+    float half_skin;
 };
 
 // the replica's own id of global id `id` (used for the pair noise, so a

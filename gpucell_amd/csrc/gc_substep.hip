@@ -109,19 +109,29 @@ __global__ void k_ms_kick(int nm, float dts, float inv_m, tfloat3* __restrict__ 
 
 // membrane state -> engine slots: wrapped FP64 position, image counters,
 // FP32 coordinates, velocity, membrane force
-__global__ void k_ms_end(int nm, int nc, int N, const int* __restrict__ curpos, const double* __restrict__ mx,
+// skin mode (woff != NULL): the wraps since the last list rebuild join the
+// vertex's wrap code and its displacement since that rebuild (in the
+// rebuild's image frame, from rbpos) the group maximum bmax[blockIdx.x]
+__global__ __launch_bounds__(BS) void k_ms_end(int nm, int nc, int N, const int* __restrict__ curpos,
+                         const double* __restrict__ mx,
                          const tfloat3* __restrict__ mv, const tfloat3* __restrict__ fm,
                          const float* __restrict__ bbox, int3 per, double* __restrict__ pos64,
                          tint3* __restrict__ nt, tfloat3* __restrict__ coor, float4* __restrict__ vout,
-                         tfloat3* __restrict__ defforce, int* __restrict__ flag)
+                         tfloat3* __restrict__ defforce, int* __restrict__ flag,
+                         unsigned char* __restrict__ woff, const tfloat3* __restrict__ rbpos,
+                         float* __restrict__ bmax)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v == 0) flag[2] += 1;   // steps in this mode
-    if (v >= nm) return;
+    const bool live = v < nm;
+    float dd = 0.0f;
+    if (live) {
     int s = curpos[nc + v];
     double u[3] = {mx[v], mx[nm + v], mx[2 * nm + v]};
     int img[3] = {0, 0, 0};
     const int pp[3] = {per.x, per.y, per.z};
+    const tint3 n0 = nt[s];
+    const int im0[3] = {n0.x, n0.y, n0.z};
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         if (!pp[d]) continue;
@@ -133,6 +143,33 @@ __global__ void k_ms_end(int nm, int nc, int N, const int* __restrict__ curpos, 
         u[d] = w;
         img[d] = (int)k;
     }
+    if (woff) {
+        int c = woff[s];
+        if (img[0] != im0[0] || img[1] != im0[1] || img[2] != im0[2]) {
+            // wraps since the last list rebuild (image code, per axis -1..1)
+            int k3[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int b = (c >> (2 * d)) & 3;
+                int k = ((b & 2) ? 1 : -(b & 1)) + (img[d] - im0[d]);
+                k3[d] = k < -1 ? -1 : (k > 1 ? 1 : k);
+            }
+            auto enc = [](int k) { return k < 0 ? 1 : (k > 0 ? 2 : 0); };
+            c = enc(k3[0]) | (enc(k3[1]) << 2) | (enc(k3[2]) << 4);
+            woff[s] = (unsigned char)c;
+        }
+        const tfloat3 r0 = rbpos[s];
+        const float r0v[3] = {r0.x, r0.y, r0.z};
+        float d2 = 0.0f;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int b = (c >> (2 * d)) & 3;
+            const float k = (b & 2) ? 1.0f : ((b & 1) ? -1.0f : 0.0f);
+            const float dx = ((float)u[d] + k * (bbox[2 * d + 1] - bbox[2 * d])) - r0v[d];
+            d2 += dx * dx;
+        }
+        dd = d2 <= 3.0e38f ? d2 : 3.0e38f;
+    }
     pos64[s] = u[0]; pos64[N + s] = u[1]; pos64[2 * N + s] = u[2];
     nt[s] = tint3{img[0], img[1], img[2]};
     coor[s] = tfloat3{(float)u[0], (float)u[1], (float)u[2]};
@@ -140,6 +177,20 @@ __global__ void k_ms_end(int nm, int nc, int N, const int* __restrict__ curpos, 
     float4 o = vout[s];
     vout[s] = make_float4(q.x, q.y, q.z, o.w);
     defforce[v] = fm[v];
+    }
+    if (woff) {   // the group's largest squared displacement
+        __shared__ float sm[BS / 64];
+        float m = dd;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float b = 0.0f;
+            for (int k = 0; k < BS / 64; ++k) b = fmaxf(b, sm[k]);
+            bmax[blockIdx.x] = b;
+        }
+    }
 }
 
 // leaving substepping: the slot force becomes the total force again
@@ -664,7 +715,9 @@ int ms_step(gc_engine* e)
         }
     }
     k_ms_end<<<nb, BS, 0, s>>>(nm, nc, e->N, e->d_curpos, m->mx, m->mv, m->fm, e->d_bbox, per3(e),
-                               e->d_pos64[e->cur], e->d_nt[e->cur], e->d_coor, e->d_vout, e->d_defforce, m->flag);
+                               e->d_pos64[e->cur], e->d_nt[e->cur], e->d_coor, e->d_vout, e->d_defforce, m->flag,
+                               e->skin_mode() ? e->d_woff : nullptr, e->d_rbpos,
+                               e->skin_mode() ? e->d_bmax + (e->N + 63) / 64 : nullptr);
     GC_LAUNCH_CHECK("membrane write-back");
     return GC_OK;
 }

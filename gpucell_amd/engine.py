@@ -35,10 +35,16 @@ def _host_state(out: dict) -> abi.gc_host_state:
 
 
 class Engine:
-    def __init__(self, case):
+    """skin > 0: Verlet lists with that skin, rebuilt (with the re-sort)
+    only when the particles have moved far enough (gc_case_desc.skin);
+    0: the reference's per-step neighbour build."""
+
+    def __init__(self, case, skin: float = 0.0):
         self.lib = abi.load()
         self.case = case
         desc, self._keep = case.desc()
+        desc.skin = float(skin)
+        self.skin = float(skin)
         st = C.c_int(0)
         self.h = self.lib.gc_EngineCreate(C.byref(desc), C.byref(st))
         if not self.h:
@@ -177,13 +183,15 @@ class Ensemble:
     """Independent suspensions stepped together by one engine
     (gc_EnsembleCreate); replica r evolves bit for bit as Engine(cases[r])."""
 
-    def __init__(self, cases):
+    def __init__(self, cases, skin: float = 0.0):
         self.lib = abi.load()
         self.cases = list(cases)
+        self.skin = float(skin)
         descs = (abi.gc_case_desc * len(self.cases))()
         self._keep = []
         for r, c in enumerate(self.cases):
             d, keep = c.desc()
+            d.skin = float(skin)
             descs[r] = d
             self._keep.append(keep)
         st = C.c_int(0)

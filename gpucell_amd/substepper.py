@@ -110,9 +110,10 @@ def _cell_metrics(e, case):
 
 
 def compare(case, net, norm, threshold: float, steps: int, warmup: int = 2, graph: bool = True,
-            relax: int = 150):
+            relax: int = 150, skin: float = 0.0):
     """Plain (mode 1, k = 8) and surrogate (mode 2) runs of `steps` steps
-    from the same state; returns accuracy and wall time per step.  Both
+    from the same state (both engines with Verlet skin `skin`, 0 = per-step
+    neighbour build); returns accuracy and wall time per step.  Both
     start from the case relaxed for `relax` reference steps (mode 0): a
     freshly placed suspension changes its cell volumes by up to ~1-3 % over
     the first ~100 steps while the fluid around the membranes settles, and
@@ -124,7 +125,7 @@ def compare(case, net, norm, threshold: float, steps: int, warmup: int = 2, grap
     out = {}
     runs = {}
     for mode in (1, 2):
-        e = Engine(case)
+        e = Engine(case, skin=skin)
         if relax:
             e.step(relax, graph=graph)
         if mode == 1:

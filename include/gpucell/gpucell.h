@@ -258,6 +258,15 @@ typedef struct {
     float bbox[6];                /* BBox (main.cuh:12) */
     Structconst consts;           /* counts inside are recomputed */
     gc_ext_consts ext;
+    /* Verlet skin of the engine's neighbour lists (no reference
+     * counterpart).  0: the reference's per-step scheme (re-sort and cube
+     * scan every step; cube side CutRadius).  > 0: cube side CutRadius +
+     * skin, lists of every pair within CutRadius + skin rebuilt (with the
+     * re-sort) only when the summed largest displacement since the last
+     * rebuild exceeds skin / 2, decided on the device; the density and
+     * force passes iterate the lists.  Every replica of an ensemble must
+     * use the same mode (skin 0 for all, or > 0 for all). */
+    float skin;
 } gc_case_desc;
 
 typedef struct gc_engine gc_engine;
@@ -286,9 +295,13 @@ typedef struct {
     int ncube[3];
     float outerregion[3], invwid[3];
     int fault;                    /* GC_FAULT_* bits raised so far */
-    int nl_overflow;              /* particles of the last step with more
+    int nl_overflow;              /* particles of the last step (skin mode:
+                                     of the last list build) with more
                                      neighbours than the list holds (their
                                      force and Morse passes scan the cubes) */
+    int nrebuild;                 /* skin mode: list rebuilds so far, summed
+                                     over the replicas (0 in per-step mode) */
+    float skin;                   /* the engine's Verlet skin (0: per step) */
 } gc_engine_info;
 /* Synchronises; returns GC_OK whatever the fault bits (they are in info). */
 int gc_EngineInfo(const gc_engine* e, gc_engine_info* info);

@@ -64,3 +64,35 @@ def test_host_only_calls():
     assert lib.gc_ConstsDataUp(ctypes.byref(c)) == -1
     e = abi.default_ext()
     assert list(e.periodic) == [1, 0, 0] and abs(e.wlc_ratio - 2.2) < 1e-6
+
+
+def test_engine_struct_layouts_match_c(tmp_path):
+    """gc_case_desc and gc_engine_info as the C compiler lays them out
+    (the ctypes mirrors in gpucell_amd/abi.py must agree field by field)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc")
+    if cc is None:
+        import pytest
+        pytest.skip("gcc not found")
+    src = tmp_path / "layout.c"
+    src.write_text(r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "gpucell/gpucell.h"
+int main(void) {
+    printf("%zu %zu %zu %zu %zu %zu %zu\n", sizeof(gc_case_desc), offsetof(gc_case_desc, consts),
+           offsetof(gc_case_desc, ext), offsetof(gc_case_desc, skin), sizeof(gc_engine_info),
+           offsetof(gc_engine_info, nrebuild), offsetof(gc_engine_info, skin));
+    return 0;
+}
+""")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    want = [ctypes.sizeof(abi.gc_case_desc), abi.gc_case_desc.consts.offset, abi.gc_case_desc.ext.offset,
+            abi.gc_case_desc.skin.offset, ctypes.sizeof(abi.gc_engine_info),
+            abi.gc_engine_info.nrebuild.offset, abi.gc_engine_info.skin.offset]
+    assert got == want
