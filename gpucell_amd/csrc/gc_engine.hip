@@ -69,7 +69,10 @@ inline int nl_stride(const gc_engine* e) { return (e->skin_mode() ? VL_ROWS : NL
 // Skin mode: workgroup displacement maxima, the advance's groups (at most
 // one per 64 particles) first, the membrane substeps' groups after them.
 inline int skin_ms_off(const gc_engine* e) { return (e->N + 63) / 64; }
-inline size_t skin_bmax_size(const gc_engine* e) { return (size_t)skin_ms_off(e) + (e->nm + GC_BS - 1) / GC_BS + 1; }
+inline size_t skin_bmax_size(const gc_engine* e)
+{
+    return DSTAT_N * ((size_t)skin_ms_off(e) + (e->nm + GC_BS - 1) / GC_BS + 1);
+}
 constexpr int CUBE_SCAN_BS = 256, CUBE_IPT = 4, CUBE_TILE = CUBE_SCAN_BS * CUBE_IPT;
 
 // Image code of a range: 2 bits per axis, 1 = shift -L, 2 = shift +L.
@@ -989,16 +992,32 @@ __device__ __forceinline__ float wave_maxf(float v)
     return v;
 }
 
-// Largest squared displacement of a workgroup's particles into bmax[b] (and
-// its replica into brep[b], -1 when its particles belong to several: those
-// go to their replicas' words of disp2 by atomics, float bits of a
-// non-negative value ordering as unsigned integers).  Called by every
-// thread of the group; WAVES = waves per group.
-template <bool ENS, int WAVES>
-__device__ __forceinline__ void block_disp(bool live, float dd, int r, int b, float* __restrict__ bmax,
-                                           int* __restrict__ brep, unsigned* __restrict__ disp2)
+// Displacement statistics of a workgroup's particles since the last
+// rebuild, DSTAT non-negative values: the largest squared displacement
+// |D|^2 and, per axis, the largest max(D, 0) and max(-D, 0) (the bounding
+// box of the displacements, which holds D = 0 too).  Into bmax[DSTAT b ..]
+// (and the group's replica into brep[b], -1 when its particles belong to
+// several: those go to their replicas' words of disp2 by atomics, float
+// bits of a non-negative value ordering as unsigned integers).  Called by
+// every thread of the group; WAVES = waves per group.
+constexpr int DSTAT = DSTAT_N;
+__device__ __forceinline__ void disp_stats(float dx, float dy, float dz, float st[DSTAT])
 {
-    __shared__ float smax[WAVES];
+    float dd = dx * dx + dy * dy + dz * dz;
+    st[0] = dd <= 3.0e38f ? dd : 3.0e38f;   // non-finite: forces a rebuild (which flags it)
+    st[1] = fmaxf(dx, 0.0f); st[2] = fmaxf(-dx, 0.0f);
+    st[3] = fmaxf(dy, 0.0f); st[4] = fmaxf(-dy, 0.0f);
+    st[5] = fmaxf(dz, 0.0f); st[6] = fmaxf(-dz, 0.0f);
+    if (!(st[0] < 3.0e38f))
+        for (int k = 1; k < DSTAT; ++k) st[k] = 3.0e38f;
+}
+
+template <bool ENS, int WAVES>
+__device__ __forceinline__ void block_disp(bool live, const float st[DSTAT], int r, int b,
+                                           float* __restrict__ bmax, int* __restrict__ brep,
+                                           unsigned* __restrict__ disp2)
+{
+    __shared__ float smax[WAVES][DSTAT];
     __shared__ int srep[WAVES];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     int r0 = 0;
@@ -1010,29 +1029,38 @@ __device__ __forceinline__ void block_disp(bool live, float dd, int r, int b, fl
             r0 = lv ? __shfl(r, (int)__ffsll((long long)lv) - 1) : -1;
         }
         mixed = __ballot(live && r != r0) != 0;
-        if (mixed && live && dd > 0.0f) atomicMax(&disp2[r], __float_as_uint(dd));
+        if (mixed && live)
+            for (int k = 0; k < DSTAT; ++k)
+                if (st[k] > 0.0f) atomicMax(&disp2[DSTAT * r + k], __float_as_uint(st[k]));
     }
-    const float m = wave_maxf(live && !mixed ? dd : 0.0f);
-    if (lane == 0) { smax[w] = m; srep[w] = mixed ? -1 : r0; }
+#pragma unroll
+    for (int k = 0; k < DSTAT; ++k) {
+        const float m = wave_maxf(live && !mixed ? st[k] : 0.0f);
+        if (lane == 0) smax[w][k] = m;
+    }
+    if (lane == 0) srep[w] = mixed ? -1 : r0;
     __syncthreads();
     if (threadIdx.x == 0) {
-        float bm = 0.0f;
+        float bm[DSTAT];
+        for (int k = 0; k < DSTAT; ++k) bm[k] = 0.0f;
         int rep = -2;   // -2: no particle yet
         bool split = false;
-        for (int k = 0; k < WAVES; ++k) {
-            bm = fmaxf(bm, smax[k]);
-            if (ENS && srep[k] >= 0) {
-                if (rep == -2) rep = srep[k];
-                else if (rep != srep[k]) split = true;
+        for (int q = 0; q < WAVES; ++q) {
+            for (int k = 0; k < DSTAT; ++k) bm[k] = fmaxf(bm[k], smax[q][k]);
+            if (ENS && srep[q] >= 0) {
+                if (rep == -2) rep = srep[q];
+                else if (rep != srep[q]) split = true;
             }
         }
         if (ENS && split) {   // several replicas in the group (rare): their own words
-            for (int k = 0; k < WAVES; ++k)
-                if (srep[k] >= 0 && smax[k] > 0.0f) atomicMax(&disp2[srep[k]], __float_as_uint(smax[k]));
-            bm = 0.0f;
+            for (int q = 0; q < WAVES; ++q)
+                if (srep[q] >= 0)
+                    for (int k = 0; k < DSTAT; ++k)
+                        if (smax[q][k] > 0.0f) atomicMax(&disp2[DSTAT * srep[q] + k], __float_as_uint(smax[q][k]));
+            for (int k = 0; k < DSTAT; ++k) bm[k] = 0.0f;
             rep = -1;
         }
-        bmax[b] = bm;
+        for (int k = 0; k < DSTAT; ++k) bmax[DSTAT * b + k] = bm[k];
         if (ENS) brep[b] = rep < 0 ? -1 : rep;
     }
 }
@@ -1063,7 +1091,7 @@ __global__ __launch_bounds__(TBS) void k3_advance(int n, GcRep R0, const GcRep* 
     const int p = b * TBS + threadIdx.x;
     const bool inb = p < n;
     const int id = inb ? sp[p] : 0;
-    float dd = 0.0f;
+    float st[DSTAT] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     with_rep<ENS>(R0, reps, rep_of, inb, id, [&](const GcRep& R) __attribute__((always_inline)) {
     if (!inb) return;
     const GcParams& P = R.P;
@@ -1098,8 +1126,7 @@ __global__ __launch_bounds__(TBS) void k3_advance(int n, GcRep R0, const GcRep* 
         const float dx = ((float)u[0] + (float)code_coef(wc, 0) * R.L[0]) - r0.x;
         const float dy = ((float)u[1] + (float)code_coef(wc, 1) * R.L[1]) - r0.y;
         const float dz = ((float)u[2] + (float)code_coef(wc, 2) * R.L[2]) - r0.z;
-        dd = dx * dx + dy * dy + dz * dz;
-        if (!(dd <= 3.0e38f)) dd = 3.0e38f;   // non-finite: forces a rebuild (which flags it)
+        disp_stats(dx, dy, dz, st);
     } else {
         velhalf[p] = tfloat3{v.x, v.y, v.z};
         velrho[p] = v;
@@ -1108,52 +1135,83 @@ __global__ __launch_bounds__(TBS) void k3_advance(int n, GcRep R0, const GcRep* 
     posid[p] = make_float4(fx, fy, fz, id_word(id, wc));
     if (id >= P.nc) coor[p] = tfloat3{fx, fy, fz};
     });
-    block_disp<ENS, TBS / 64>(inb, dd, ENS && inb ? rep_of[id] : 0, b, bmax, brep, disp2);
+    block_disp<ENS, TBS / 64>(inb, st, ENS && inb ? rep_of[id] : 0, b, bmax, brep, disp2);
 }
 
-// Skin mode, once per step after the advance: per replica the largest
-// displacement since its last rebuild (the advance's workgroup maxima, the
-// previous step's membrane substeps' from ms_off on, and the atomics of
-// groups spanning replicas); past half_skin (or non-finite) the replica's
-// lists are rebuilt this step.  Also advances the pair-noise counter (the
-// force pass reads the value before the advance).
-constexpr int DECIDE_BS = 1024, DECIDE_MAXREP = 1024;
-__global__ __launch_bounds__(DECIDE_BS) void k3_decide(int nrep, const GcRep* __restrict__ reps,
-    const float* __restrict__ bmax, const int* __restrict__ brep, int nb_adv, int ms_off, int nb_ms,
+// Skin mode, once per step after the advance: the workgroup displacement
+// statistics (the advance's groups, then the previous step's membrane
+// substeps' from ms_off on) reduced per replica into disp2 (which also holds
+// the atomics of groups spanning replicas).  Each thread takes a contiguous
+// run of entries and flushes its maxima when the replica changes.
+constexpr int REDUCE_BS = 256, REDUCE_NB = 64;
+__global__ __launch_bounds__(REDUCE_BS) void k3_reduce(int nrep, const float* __restrict__ bmax,
+    const int* __restrict__ brep, int nb_adv, int ms_off, int nb_ms, unsigned* __restrict__ disp2)
+{
+    const int E = nb_adv + nb_ms, T = REDUCE_NB * REDUCE_BS;
+    const int C = (E + T - 1) / T;
+    const int t = blockIdx.x * REDUCE_BS + threadIdx.x;
+    float m[DSTAT];
+    for (int k = 0; k < DSTAT; ++k) m[k] = 0.0f;
+    int cur = -1;
+    auto flush = [&]() {
+        if (cur >= 0)
+            for (int k = 0; k < DSTAT; ++k)
+                if (m[k] > 0.0f) atomicMax(&disp2[DSTAT * cur + k], __float_as_uint(m[k]));
+        for (int k = 0; k < DSTAT; ++k) m[k] = 0.0f;
+    };
+    for (int e = t * C; e < min(E, (t + 1) * C); ++e) {
+        // entries past the advance's groups are the substeps' (one suspension)
+        const int q = e < nb_adv ? e : ms_off + (e - nb_adv);
+        const int r = nrep == 1 ? 0 : (e < nb_adv ? brep[q] : 0);
+        if (r < 0) continue;   // a group spanning replicas: already in disp2
+        if (r != cur) { flush(); cur = r; }
+        for (int k = 0; k < DSTAT; ++k) m[k] = fmaxf(m[k], bmax[DSTAT * q + k]);
+    }
+    if (nrep == 1) {   // one word set: reduce the group first
+        __shared__ float sm[REDUCE_BS / 64][DSTAT];
+        for (int k = 0; k < DSTAT; ++k) {
+            const float v = wave_maxf(m[k]);
+            if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6][k] = v;
+        }
+        __syncthreads();
+        if (threadIdx.x < DSTAT) {
+            float v = 0.0f;
+            for (int w = 0; w < REDUCE_BS / 64; ++w) v = fmaxf(v, sm[w][threadIdx.x]);
+            if (v > 0.0f) atomicMax(&disp2[threadIdx.x], __float_as_uint(v));
+        }
+    } else {
+        flush();
+    }
+}
+
+// Skin mode, per replica from its displacement statistics since its last
+// rebuild: the lists stay valid while every pair's relative displacement
+// |D_i - D_j| is at most the skin, which holds if every |D_i| <= skin/2 or
+// if the bounding box of all D (a uniform translation moves no pair) has a
+// diagonal <= skin; when both fail (or a value is non-finite) the replica's
+// lists are rebuilt this step.  Clears the statistics and advances the
+// pair-noise counter (the force pass reads the value before the advance).
+__global__ __launch_bounds__(64) void k3_decide(int nrep, const GcRep* __restrict__ reps,
     unsigned* __restrict__ disp2, int* __restrict__ rbflag, int* __restrict__ rbany, int* __restrict__ nrebuild,
     unsigned long long* __restrict__ ctr)
 {
-    __shared__ unsigned smax[DECIDE_MAXREP];
-    __shared__ int any;
-    const int t = threadIdx.x;
-    const bool shared_ok = true;   // creation limits skin-mode ensembles to DECIDE_MAXREP replicas
-    for (int r = t; r < nrep && shared_ok; r += DECIDE_BS) smax[r] = 0u;
-    if (t == 0) { any = 0; *ctr += 1ull; }
-    __syncthreads();
-    float m = 0.0f;   // one replica: a plain maximum
-    auto take = [&](int k, float v) {
-        if (nrep == 1) { m = fmaxf(m, v); return; }
-        const int r = brep[k];
-        if (r < 0 || v <= 0.0f) return;
-        if (shared_ok) atomicMax(&smax[r], __float_as_uint(v));
-        else atomicMax(&disp2[r], __float_as_uint(v));
-    };
-    for (int k = t; k < nb_adv; k += DECIDE_BS) take(k, bmax[k]);
-    for (int k = t; k < nb_ms; k += DECIDE_BS) m = fmaxf(m, bmax[ms_off + k]);   // one suspension only
-    if (nrep == 1) {
-        m = wave_maxf(m);
-        if ((t & 63) == 0) atomicMax(&smax[0], __float_as_uint(m));
-    }
-    __syncthreads();
-    for (int r = t; r < nrep; r += DECIDE_BS) {
-        const float d2 = fmaxf(__uint_as_float(disp2[r]), shared_ok ? __uint_as_float(smax[r]) : 0.0f);
-        disp2[r] = 0u;
-        const int rb = !(sqrtf(d2) <= reps[r].half_skin);
+    int any = 0;
+    for (int r = threadIdx.x; r < nrep; r += 64) {
+        float v[DSTAT];
+        for (int k = 0; k < DSTAT; ++k) {
+            v[k] = __uint_as_float(disp2[DSTAT * r + k]);
+            disp2[DSTAT * r + k] = 0u;
+        }
+        const float hs = reps[r].half_skin;
+        const float bx = v[1] + v[2], by = v[3] + v[4], bz = v[5] + v[6];
+        const bool each = sqrtf(v[0]) <= hs;                          // every |D_i| <= skin/2
+        const bool box = sqrtf(bx * bx + by * by + bz * bz) <= 2.0f * hs;   // diagonal <= skin
+        const int rb = !(each || box);
         rbflag[r] = rb;
-        if (rb) { nrebuild[r] += 1; atomicOr(&any, 1); }
+        if (rb) { nrebuild[r] += 1; any = 1; }
     }
-    __syncthreads();
-    if (t == 0) *rbany = any;
+    any = __ballot(any) != 0;
+    if (threadIdx.x == 0) { *rbany = any; *ctr += 1ull; }
 }
 
 // [3] skin mode, a rebuild: cube key of each particle of a rebuilding
@@ -1242,6 +1300,14 @@ __global__ __launch_bounds__(TBS) void k3_density(int n, GcRep R0, const GcRep* 
         }
     };
     const int kmax = wave_max(cnt);
+    // the codes of the next four entries are loaded while the current four
+    // entries' positions are in flight
+    int cn[4];
+    auto load_codes = [&](int k) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cn[u] = cnt > 0 ? wl[64 * min(k + u, cnt - 1)] : 0;   // in bounds
+    };
+    if (kmax > 0) load_codes(0);
     for (int k0 = 0; k0 < kmax; k0 += 32) {
         unsigned m = 0u;
         const int k1 = min(k0 + 32, kmax);
@@ -1249,9 +1315,10 @@ __global__ __launch_bounds__(TBS) void k3_density(int n, GcRep R0, const GcRep* 
             int c[4];
             float4 q[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) c[u] = cnt > 0 ? wl[64 * min(k + u, cnt - 1)] : 0;   // in bounds
+            for (int u = 0; u < 4; ++u) c[u] = cn[u];
 #pragma unroll
             for (int u = 0; u < 4; ++u) q[u] = posid[c[u] >> 6];
+            if (k + 4 < kmax) load_codes(k + 4);
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 const bool ok = k + u < cnt;
@@ -1527,7 +1594,7 @@ int skin_init(gc_engine* e)
     if (st) return st;
     const int nr = e->nrep;
     if ((st = hip_check(hipMemsetAsync(e->d_woff, 0, (size_t)N, s), "skin init")) ||
-        (st = hip_check(hipMemsetAsync(e->d_disp2, 0, sizeof(unsigned) * nr, s), "skin init")) ||
+        (st = hip_check(hipMemsetAsync(e->d_disp2, 0, sizeof(unsigned) * DSTAT_N * nr, s), "skin init")) ||
         (st = hip_check(hipMemsetAsync(e->d_bmax, 0, sizeof(float) * skin_bmax_size(e), s), "skin init")) ||
         (st = hip_check(hipMemsetAsync(e->d_rbflag, 0, sizeof(int) * nr, s), "skin init")))
         return st;
@@ -1561,8 +1628,9 @@ int skin_fluid_t(gc_engine* e, int driveon, Prof* pr)
     // the substeps of the previous step wrote their vertices' maxima after
     // the advance's region (membrane substepping: one suspension)
     const int nb_ms = e->ms ? blocks_for(e->nm, BS) : 0;
-    k3_decide<<<1, DECIDE_BS, 0, s>>>(e->nrep, e->d_reps, e->d_bmax, e->d_brep, nb_adv, skin_ms_off(e), nb_ms,
-                                      e->d_disp2, e->d_rbflag, e->d_rbany, e->d_nrebuild, e->d_ctr);
+    k3_reduce<<<REDUCE_NB, REDUCE_BS, 0, s>>>(e->nrep, e->d_bmax, e->d_brep, nb_adv, skin_ms_off(e), nb_ms,
+                                               e->d_disp2);
+    k3_decide<<<1, 64, 0, s>>>(e->nrep, e->d_reps, e->d_disp2, e->d_rbflag, e->d_rbany, e->d_nrebuild, e->d_ctr);
     GC_LAUNCH_CHECK("engine advance");
     mark(pr, s, "advance+predict");
     int st = skin_rebuild_t<ENS>(e, e->d_rbany, e->d_rbflag);
@@ -1862,10 +1930,6 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
     if (!descs || nrep < 1) { *status = set_error(GC_ERR_INVALID, "EngineCreate: no case"); return nullptr; }
     for (int r = 0; r < nrep; ++r)
         if ((*status = validate_desc(&descs[r]))) return nullptr;
-    if (descs[0].skin > 0.0f && nrep > 1024) {
-        *status = set_error(GC_ERR_INVALID, "EnsembleCreate: skin mode takes at most 1024 replicas");
-        return nullptr;
-    }
     for (int r = 1; r < nrep; ++r)
         if ((descs[r].skin > 0.0f) != (descs[0].skin > 0.0f)) {
             *status = set_error(GC_ERR_INVALID, "EnsembleCreate: replicas must all use a skin or none");
@@ -1991,7 +2055,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
     if (e->skin_mode()) {
         const size_t nr = (size_t)nrep;
         std::vector<int> ones(nr, 1), zero(nr, 0);
-        if ((st = e->alloc(&e->d_disp2, nr)) || (st = e->alloc(&e->d_rbpos, (size_t)N)) ||
+        if ((st = e->alloc(&e->d_disp2, DSTAT_N * nr)) || (st = e->alloc(&e->d_rbpos, (size_t)N)) ||
             (st = e->alloc(&e->d_bmax, skin_bmax_size(e))) || (st = e->alloc(&e->d_brep, skin_bmax_size(e))) ||
             (st = e->alloc(&e->d_rbflag, nr)) || (st = e->alloc(&e->d_rbany, 1)) ||
             (st = e->alloc(&e->d_nrebuild, nr)) || (st = e->alloc(&e->d_ones, nr)) ||
@@ -2635,6 +2699,9 @@ extern "C" int gc_EngineAutotune(gc_engine* e, int steps, int driveon, gc_tune_r
         const int row0 = row;
         for (int v = 0; v < TUNE_NVARIANT[f] && !st; ++v) {
             if (f == TUNE_MEMBRANE && v == 3 && !e->cell_fused) continue;   // cells too large
+            // skin mode: the stage only shapes the list build, which the timed
+            // steps rarely run; the density pass itself has no stage
+            if (f == TUNE_DENSITY && e->skin_mode() && v % 4 != 2) continue;
             e->tune.v[f] = v;
             float us = 0.f;
             if ((st = launch_step(e, driveon, nullptr))) break;   // warm-up (code load)

@@ -42,6 +42,11 @@ struct TuneCfg {
 };
 constexpr int TUNE_NVARIANT[TUNE_N] = {3, 12, 3, 4};
 
+// Skin mode: displacement statistics per workgroup since the last list
+// rebuild (the largest |D|^2, then per axis the largest max(D, 0) and
+// max(-D, 0)), written by the advance pass and the membrane substeps.
+constexpr int DSTAT_N = 7;
+
 // Membrane substepping / surrogate state (gc_substep.hip); NULL while the
 // membrane moves with the fluid step.
 struct MembraneSub;

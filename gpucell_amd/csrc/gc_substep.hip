@@ -110,8 +110,9 @@ __global__ void k_ms_kick(int nm, float dts, float inv_m, tfloat3* __restrict__ 
 // membrane state -> engine slots: wrapped FP64 position, image counters,
 // FP32 coordinates, velocity, membrane force
 // skin mode (woff != NULL): the wraps since the last list rebuild join the
-// vertex's wrap code and its displacement since that rebuild (in the
-// rebuild's image frame, from rbpos) the group maximum bmax[blockIdx.x]
+// vertex's wrap code, and the statistics of its displacement D since that
+// rebuild (in the rebuild's image frame, from rbpos) the group's maxima
+// bmax[DSTAT_N blockIdx.x ..]: |D|^2, then per axis max(D, 0), max(-D, 0)
 __global__ __launch_bounds__(BS) void k_ms_end(int nm, int nc, int N, const int* __restrict__ curpos,
                          const double* __restrict__ mx,
                          const tfloat3* __restrict__ mv, const tfloat3* __restrict__ fm,
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(BS) void k_ms_end(int nm, int nc, int N, const int*
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v == 0) flag[2] += 1;   // steps in this mode
     const bool live = v < nm;
-    float dd = 0.0f;
+    float st[DSTAT_N] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (live) {
     int s = curpos[nc + v];
     double u[3] = {mx[v], mx[nm + v], mx[2 * nm + v]};
@@ -167,8 +168,12 @@ __global__ __launch_bounds__(BS) void k_ms_end(int nm, int nc, int N, const int*
             const float k = (b & 2) ? 1.0f : ((b & 1) ? -1.0f : 0.0f);
             const float dx = ((float)u[d] + k * (bbox[2 * d + 1] - bbox[2 * d])) - r0v[d];
             d2 += dx * dx;
+            st[1 + 2 * d] = fmaxf(dx, 0.0f);
+            st[2 + 2 * d] = fmaxf(-dx, 0.0f);
         }
-        dd = d2 <= 3.0e38f ? d2 : 3.0e38f;
+        st[0] = d2 <= 3.0e38f ? d2 : 3.0e38f;
+        if (!(st[0] < 3.0e38f))
+            for (int k = 1; k < DSTAT_N; ++k) st[k] = 3.0e38f;
     }
     pos64[s] = u[0]; pos64[N + s] = u[1]; pos64[2 * N + s] = u[2];
     nt[s] = tint3{img[0], img[1], img[2]};
@@ -178,17 +183,20 @@ __global__ __launch_bounds__(BS) void k_ms_end(int nm, int nc, int N, const int*
     vout[s] = make_float4(q.x, q.y, q.z, o.w);
     defforce[v] = fm[v];
     }
-    if (woff) {   // the group's largest squared displacement
-        __shared__ float sm[BS / 64];
-        float m = dd;
+    if (woff) {   // the group's displacement statistics
+        __shared__ float sm[BS / 64][DSTAT_N];
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-        if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+        for (int k = 0; k < DSTAT_N; ++k) {
+            float m = st[k];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+            if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6][k] = m;
+        }
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (threadIdx.x < DSTAT_N) {
             float b = 0.0f;
-            for (int k = 0; k < BS / 64; ++k) b = fmaxf(b, sm[k]);
-            bmax[blockIdx.x] = b;
+            for (int q = 0; q < BS / 64; ++q) b = fmaxf(b, sm[q][threadIdx.x]);
+            bmax[DSTAT_N * blockIdx.x + threadIdx.x] = b;
         }
     }
 }
@@ -717,7 +725,7 @@ int ms_step(gc_engine* e)
     k_ms_end<<<nb, BS, 0, s>>>(nm, nc, e->N, e->d_curpos, m->mx, m->mv, m->fm, e->d_bbox, per3(e),
                                e->d_pos64[e->cur], e->d_nt[e->cur], e->d_coor, e->d_vout, e->d_defforce, m->flag,
                                e->skin_mode() ? e->d_woff : nullptr, e->d_rbpos,
-                               e->skin_mode() ? e->d_bmax + (e->N + 63) / 64 : nullptr);
+                               e->skin_mode() ? e->d_bmax + (size_t)DSTAT_N * ((e->N + 63) / 64) : nullptr);
     GC_LAUNCH_CHECK("membrane write-back");
     return GC_OK;
 }
