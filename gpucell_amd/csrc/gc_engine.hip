@@ -593,9 +593,9 @@ __device__ __forceinline__ void wave_lds_sync()
 // The neighbour list of a wave is one block of 64 x NL_CAP entries, entry k
 // of lane l at 64 k + l (coalesced rows for the force pass).
 // LB (skin mode, a list rebuild): the same scan writes the Verlet list of
-// every particle of a rebuilding replica (Bound2 walls included: every pair
-// within h + skin, VL_CAP entries per particle, blocks of VL_ROWS rows) and
-// computes no density.
+// every particle of a rebuilding replica (every pair within h + skin; for a
+// Bound2 wall its moving partners only; VL_CAP entries per particle, blocks
+// of VL_ROWS rows) and computes no density.
 template <bool ENS, int TBS, int TSTAGE, bool LB = false>
 __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, const float4* __restrict__ posid, float4* __restrict__ velrho,
@@ -645,7 +645,9 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
         float r2 = rx * rx + ry * ry + rz * rz;
         const int idj = pid_of(pj.w);
         if (LB) {
-            if (mov && r2 < r2max && j != p) {
+            // a Bound2 wall lists its moving neighbours only (its velocity
+            // is their weighted mean; wall-wall pairs enter no sum)
+            if (mov && r2 < r2max && j != p && (id >= P.nf || idj >= P.nf)) {
                 wl[min(woff, wlast)] = (j << 6) | code;
                 woff += 64;
             }
