@@ -1603,6 +1603,69 @@ int skin_init(gc_engine* e)
     return hip_check(hipStreamSynchronize(s), "skin init sync");
 }
 
+// Skin mode, host-driven builds (creation, SetState): the given positions
+// may lie outside a periodic face (the per-step scheme wraps them in its
+// first step).  The build then runs on the wrapped positions (the lists'
+// frame, in `frame`, id order, SoA), and afterwards the stored positions are
+// put back unchanged with their offset to that frame as the wrap code (the
+// first step's advance wraps them and clears it).  pos: id order, SoA.
+void skin_frame(const gc_engine* e, const std::vector<int>& rep_of, const std::vector<double>& pos,
+                std::vector<double>& frame, std::vector<unsigned char>& code)
+{
+    const size_t N = (size_t)e->N;
+    frame = pos;
+    code.assign(N, 0);
+    for (size_t i = 0; i < N; ++i) {
+        const GcRep& R = e->hreps[rep_of.empty() ? 0 : rep_of[i]];
+        int k[3] = {0, 0, 0};
+        for (int d = 0; d < 3; ++d) {
+            if (!R.P.per[d]) continue;
+            double& u = frame[(size_t)d * N + i];
+            const double lo = R.bbox[2 * d], hi = R.bbox[2 * d + 1];
+            const double L = (double)(R.bbox[2 * d + 1] - R.bbox[2 * d]);
+            if (u < lo) { u += L; k[d] = 1; }          // frame = stored + L
+            else if (u >= hi) { u -= L; k[d] = -1; }   // frame = stored - L
+        }
+        auto enc = [](int v) { return v < 0 ? 1 : (v > 0 ? 2 : 0); };   // image code: stored + k L = frame
+        code[i] = (unsigned char)(enc(k[0]) | (enc(k[1]) << 2) | (enc(k[2]) << 4));
+    }
+}
+
+// After the build and the lists: the stored positions back in slot order,
+// their FP32 copies and wrap codes.
+int skin_restore_stored(gc_engine* e, const std::vector<double>& pos, const std::vector<unsigned char>& code)
+{
+    const int N = e->N;
+    bool any = false;
+    for (unsigned char c : code) any = any || c;
+    if (!any) return GC_OK;   // every position was inside the box: nothing to put back
+    std::vector<int> sp(N);
+    int st = hip_check(hipStreamSynchronize(e->s), "skin frame sync");
+    if (!st) st = hip_check(hipMemcpy(sp.data(), e->d_sp[0], sizeof(int) * N, hipMemcpyDeviceToHost), "skin frame D2H");
+    if (st) return st;
+    std::vector<double> p64(3 * (size_t)N);
+    std::vector<float4> pid(N);
+    std::vector<tfloat3> co(N);
+    std::vector<unsigned char> wc(N);
+    for (int s = 0; s < N; ++s) {
+        const int id = sp[s];
+        const double x = pos[id], y = pos[(size_t)N + id], z = pos[2 * (size_t)N + id];
+        p64[s] = x; p64[(size_t)N + s] = y; p64[2 * (size_t)N + s] = z;
+        const int w = id | ((int)code[id] << 25);
+        float wf;
+        memcpy(&wf, &w, sizeof(wf));
+        pid[s] = make_float4((float)x, (float)y, (float)z, wf);
+        co[s] = tfloat3{(float)x, (float)y, (float)z};
+        wc[s] = code[id];
+    }
+    if ((st = upload(e->d_pos64[0], p64.data(), 3 * (size_t)N, e->s)) ||
+        (st = upload(e->d_posid, pid.data(), (size_t)N, e->s)) ||
+        (st = upload(e->d_coor, co.data(), (size_t)N, e->s)) ||
+        (st = upload(e->d_woff, wc.data(), (size_t)N, e->s)))
+        return st;
+    return hip_check(hipStreamSynchronize(e->s), "skin frame sync");
+}
+
 // Skin mode: [1,2,5] advance and predictor, the rebuild decision, the
 // rebuild (gated), [7,9] density over the lists, [10-13,17] force over the
 // neighbour bits.
@@ -2070,6 +2133,8 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
     hipStream_t s = e->s;
     std::vector<float4> vout_id(N);
     std::vector<tfloat3> vh_id(N), f_id(N, tfloat3{0.f, 0.f, 0.f});
+    std::vector<double> skin_pos;            // skin mode: the given positions (id order, SoA)
+    std::vector<unsigned char> skin_code;    // and their offsets to the build's frame
     {
         // Particle::SetParticle (Particle.cu:485-546): rho = DenInit, F = 0
         std::vector<double> pos(3 * N);
@@ -2093,6 +2158,10 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
                 }
         }
         for (size_t i = 0; i < N; ++i) idn[i] = (int)i;
+        if (e->skin_mode()) {   // the build runs on the wrapped positions (skin_frame)
+            skin_pos = pos;
+            skin_frame(e, rep_of, skin_pos, pos, skin_code);
+        }
         unsigned long long ctr = e->ext.noise_counter;
         if ((st = upload(e->d_reps, e->hreps.data(), (size_t)nrep, s)) ||
             (st = upload(e->d_bbox, e->hreps[0].bbox, 6, s)) ||
@@ -2116,6 +2185,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
     if ((st = reorder_carried(e, vout_id, f_id, vh_id))) return fail(st);
     if ((st = wall_tables(e))) return fail(st);
     if ((st = skin_init(e))) return fail(st);
+    if (e->skin_mode() && (st = skin_restore_stored(e, skin_pos, skin_code))) return fail(st);
 
     if (e->ncell > 0) {
         const size_t T = (size_t)e->ntri, E = (size_t)e->nedge, C = (size_t)e->ncell, M = (size_t)e->nm;
@@ -2517,6 +2587,12 @@ int restore_state(gc_engine* e, const std::vector<double>& pos, const std::vecto
     std::vector<double> pos_soa(3 * (size_t)N);
     for (int i = 0; i < N; ++i)
         for (int d = 0; d < 3; ++d) pos_soa[(size_t)d * N + i] = pos[3 * (size_t)i + d];
+    std::vector<double> skin_pos;
+    std::vector<unsigned char> skin_code;
+    if (e->skin_mode()) {   // the build runs on the wrapped positions (skin_frame)
+        skin_pos = pos_soa;
+        skin_frame(e, {}, skin_pos, pos_soa, skin_code);
+    }
     std::vector<int> idn(N);
     for (int i = 0; i < N; ++i) idn[i] = i;
     e->cur = 0;
@@ -2538,6 +2614,7 @@ int restore_state(gc_engine* e, const std::vector<double>& pos, const std::vecto
     if ((st = reorder_carried(e, vo, f, vh))) return st;
     if ((st = wall_tables(e))) return st;
     if ((st = skin_init(e))) return st;
+    if (e->skin_mode() && (st = skin_restore_stored(e, skin_pos, skin_code))) return st;
     e->walls_dirty = false;
     e->step = step;
     if (e->ms && (st = ms_reset(e))) return st;

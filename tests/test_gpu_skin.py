@@ -1,8 +1,8 @@
 """The engine's Verlet skin mode (gc_case_desc.skin > 0): neighbour lists of
-every pair within h + skin, re-sorted and rebuilt only when the summed
-largest displacement since the last rebuild passes skin / 2 (decided on the
-device), density over the lists, force over the neighbour bits the density
-pass sets.
+every pair within h + skin, re-sorted and rebuilt only when some pair could
+have moved apart by more than the skin since the last rebuild (decided on
+the device from the displacements), density over the lists, force over the
+neighbour bits the density pass sets.
 
 * one step from an exported state against the FP64 oracle on every engine
   case, with a skin that rebuilds every few steps (so the checked step comes
@@ -15,7 +15,9 @@ pass sets.
 * each replica of a batch == its solo run, bit for bit (each replica decides
   its own rebuilds);
 * membrane substepping under the skin mode (its displacements join the
-  rebuild bound)."""
+  rebuild bound);
+* particles that start outside a periodic face (the first lists are built
+  from the wrapped positions)."""
 import numpy as np
 import pytest
 
@@ -142,3 +144,29 @@ def test_skin_membrane_substeps(gpu, mode):
     dpos = np.abs(Sa["pos"] - Sb["pos"]).max()
     print(f"substeps k=4, 20 steps: per-step vs skin max |dx| {dpos:.2e}, rebuilds {b.info().nrebuild}")
     assert dpos < 1e-5
+
+
+def test_skin_start_outside_periodic_face(gpu):
+    """Moving particles that start just outside the periodic x faces (as
+    generated cells crossing a face do): the first lists must come from the
+    wrapped positions, while the exported start state keeps the given ones."""
+    case = CASES["channel_morse"]()
+    mov = np.arange(case.N) >= case.consts.nFluidStart
+    L = float(case.bbox[1] - case.bbox[0])
+    x = case.coor[:, 0]
+    shift = np.where(mov & (x < 0.7), -0.75, np.where(mov & (x > L - 0.7), 0.75, 0.0)).astype(np.float32)
+    case.coor[:, 0] = x + shift
+    n_out = int(np.sum((case.coor[:, 0] < 0) | (case.coor[:, 0] >= L)))
+    assert n_out > 50
+    e = Engine(case, skin=0.3)
+    S0 = e.state()
+    assert np.array_equal(S0["coor"], case.coor)   # the start state as given
+    o = orc.Oracle(case)
+    o.initial_state()
+    for k in range(2):
+        S, info = e.state(), e.info()
+        e.step(1)
+        S1 = e.state()
+        emax, _, _ = gu.step_parity(o, case, S, S1, info, label=f"start outside the box, step {k + 1}")
+        print(f"{n_out} particles start outside the box: step {k + 1} force error {emax:.3e}")
+    assert e.info().nrebuild == 0
