@@ -258,21 +258,25 @@ def main():
     # measured HBM bytes per launch from the committed counter profile of this
     # case, used only when it was taken on the launch variant this run uses
     traffic, traffic_variant = None, None
-    pmc = os.path.join(ROOT, "profiles", f"traffic_{knames[dom]}_{a.case}.json")
+    mode = "_skin" if skin > 0 else ""
+    pmc = os.path.join(ROOT, "profiles", f"traffic_{knames[dom]}_{a.case}{mode}.json")
     used = None if tune_rows is None else {r["family"]: r["shape"] for r in tune_rows if r["chosen"]}
     if os.path.exists(pmc):
         tj = json.load(open(pmc))
-        traffic_variant = tj.get("variant")
         fam = KERNEL_FAMILY.get(dom)
-        if used is None or fam is None or used.get(fam) == tj.get("shape"):
+        # only a profile of the same mode (skin) and launch variant
+        if abs(float(tj.get("skin", 0.0)) - skin) < 1e-6 and (
+                used is None or fam is None or used.get(fam) == tj.get("shape")):
             traffic = tj.get("bytes_per_launch")
+            traffic_variant = tj.get("variant")
     # instruction-issue roofline of the dominant kernel: VALU instructions per
     # wave from the committed counter profile of this case (rocprofv3 SQ
     # counters), 4 cycles per wave64 instruction on each of 1024 SIMDs
     valu = None
-    vp = os.path.join(ROOT, "profiles", f"valu_{a.case}.json")
-    if os.path.exists(vp):
-        vj = json.load(open(vp)).get("kernels", {}).get(knames[dom])
+    vp = os.path.join(ROOT, "profiles", f"valu_{a.case}{mode}.json")
+    vdoc = json.load(open(vp)) if os.path.exists(vp) else {}
+    if vdoc and abs(float(vdoc.get("skin", 0.0)) - skin) < 1e-6:
+        vj = vdoc.get("kernels", {}).get(knames[dom])
         if vj:
             # waves the kernel launches on this case (the force pass runs one
             # lane per moving particle), as recorded with the counters

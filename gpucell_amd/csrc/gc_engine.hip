@@ -66,6 +66,10 @@ constexpr int NL_CAP = 64;
 constexpr int VL_CAP = 128, VL_ROWS = VL_CAP + VL_CAP / 32;
 // ints per wave block of the neighbour lists
 inline int nl_stride(const gc_engine* e) { return (e->skin_mode() ? VL_ROWS : NL_CAP) * 64; }
+// Skin mode: grid of the gated rebuild kernels (they stride over their
+// logical groups; an empty launch then dispatches at most this many)
+constexpr int GATED_MAX_GROUPS = 2048;
+inline int gated_grid(int nb) { return std::min(xcd_grid(nb), GATED_MAX_GROUPS); }
 // Skin mode: workgroup displacement maxima, the advance's groups (at most
 // one per 64 particles) first, the membrane substeps' groups after them.
 inline int skin_ms_off(const gc_engine* e) { return (e->N + 63) / 64; }
@@ -130,6 +134,19 @@ __device__ __forceinline__ void scan_range(const float4* __restrict__ posid, int
         fn(j, a); fn(j + 1, b); fn(j + 2, c); fn(j + 3, d);
     }
     for (; j < je; ++j) fn(j, posid[j]);
+}
+
+// Logical groups [0, nb) of TBS lanes over a grid of gridDim.x groups (a
+// multiple of 8, renumbered so each XCD takes contiguous runs): body(lb) for
+// lb = xcd_block(gridDim.x) + k gridDim.x.  With gridDim.x = xcd_grid(nb)
+// each group runs one logical group, numbered as xcd_block(nb) does; the
+// skin mode's gated rebuild kernels run on a small fixed grid, so that an
+// empty launch (no rebuild this step) dispatches few groups.
+template <class Body>
+__device__ __forceinline__ void for_groups(int nb, Body&& body)
+{
+    const int G = gridDim.x;   // the renumbering is a permutation for multiples of 8 only
+    for (int lb = G % GC_NXCD == 0 ? xcd_block(G) : (int)blockIdx.x; lb < nb; lb += G) body(lb);
 }
 
 __device__ __forceinline__ void key_to_cc(int k, const GcRep& R, int& cx, int& cy, int& cz)
@@ -417,7 +434,8 @@ __global__ __launch_bounds__(BS) void k2_scatter(int n, const int* __restrict__ 
     const int* __restrict__ gate)
 {
     if (gate && *gate == 0) return;
-    int p = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
+    for_groups((n + BS - 1) / BS, [&](int lb) {
+    int p = lb * BS + threadIdx.x;
     if (p >= n) return;
     int k = key[p], r = rank[p];
     int q = offs[k] + r;
@@ -426,6 +444,7 @@ __global__ __launch_bounds__(BS) void k2_scatter(int n, const int* __restrict__ 
     tv[q] = sp[p];
     to[q] = p;
     if (r == 0) cnt[k] = 0;
+    });
 }
 
 // [1,2,4,5,6,8] final slot = cube start + number of smaller ids in the
@@ -452,7 +471,8 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     const int* __restrict__ rbflag, unsigned char* __restrict__ woff)
 {
     if (SK && *gate == 0) return;   // skin mode: no rebuild this step
-    int q = xcd_block((n + BS - 1) / BS) * BS + threadIdx.x;
+    for_groups((n + BS - 1) / BS, [&](int lb) {
+    int q = lb * BS + threadIdx.x;
     if (q >= n) return;
     int k = tk[q], id = tv[q];
     int b = max(offs[k], 0), e = min(offs[k + 1], n);
@@ -523,6 +543,7 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
         const int ai = s - wincl[k];
         if ((unsigned)ai < (unsigned)n) act[ai] = s;
     }
+    });
 }
 
 // [7,9] density of moving particles; velocity of Bound2 walls from the
@@ -1226,7 +1247,8 @@ __global__ __launch_bounds__(TBS) void k3_keys(int n, GcRep R0, const GcRep* __r
     int* __restrict__ key, int* __restrict__ rank, int* __restrict__ cnt)
 {
     if (*gate == 0) return;
-    const int p = xcd_block((n + TBS - 1) / TBS) * TBS + threadIdx.x;
+    for_groups((n + TBS - 1) / TBS, [&](int lgb) {
+    const int p = lgb * TBS + threadIdx.x;
     const bool act = p < n;
     if (__ballot(act) == 0) return;
     int k = -1;
@@ -1245,6 +1267,7 @@ __global__ __launch_bounds__(TBS) void k3_keys(int n, GcRep R0, const GcRep* __r
     key[p] = k;
     });
     hist_rank(act, k, p, rank, cnt);
+    });
 }
 
 // Skin mode, a rebuild: the gather wrote buffer 1; buffer 0 stays current.
@@ -1253,11 +1276,13 @@ __global__ __launch_bounds__(BS) void k3_copyback(int n, const int* __restrict__
     tint3* __restrict__ nt0, const int* __restrict__ sp1, int* __restrict__ sp0)
 {
     if (*gate == 0) return;
-    const int p = blockIdx.x * BS + threadIdx.x;
+    for_groups((n + BS - 1) / BS, [&](int lb) {
+    const int p = lb * BS + threadIdx.x;
     if (p >= n) return;
     pos0[p] = pos1[p]; pos0[n + p] = pos1[n + p]; pos0[2 * n + p] = pos1[2 * n + p];
     nt0[p] = nt1[p];
     sp0[p] = sp1[p];
+    });
 }
 
 // [7,9] skin mode, every step: number density of moving particles and
@@ -1530,18 +1555,19 @@ int skin_rebuild_t(gc_engine* e, const int* gate, const int* rbflag)
     const int gb = xcd_grid(blocks_for(N, BS));
     const GcRep& R0 = e->hreps[0];
     const int mfree = e->ms == nullptr;
-    k3_keys<ENS, 256><<<xcd_grid(blocks_for(N, 256)), 256, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, gate, rbflag,
+    k3_keys<ENS, 256><<<gated_grid(blocks_for(N, 256)), 256, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, gate, rbflag,
         e->d_pos64[0], e->d_sp[0], e->d_key, e->d_key0, e->d_rank, e->d_cnt);
     k2_tile_sums<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, gate);
     k2_cube_offsets<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, e->d_offs, e->d_ctr, 0,
                                                       e->d_nlover, gate);
-    k2_scatter<<<gb, BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[0], e->d_offs, e->d_tk, e->d_tv,
-                                 e->d_to, e->d_cnt, gate);
-    k2_sortgather<ENS, true><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, 0, mfree, e->moving_walls,
+    k2_scatter<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[0], e->d_offs,
+                                                            e->d_tk, e->d_tv, e->d_to, e->d_cnt, gate);
+    k2_sortgather<ENS, true><<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, 0, mfree,
+        e->moving_walls,
         e->d_tk, e->d_tv, e->d_to, e->d_offs, e->d_curpos, e->d_pos64[0], e->d_pos64[1], e->d_vout,
         e->d_force, e->d_nt[0], e->d_nt[1], e->d_sp[1], e->d_key, e->d_posid, e->d_coor, e->d_velrho,
         e->d_velhalf, e->moving_walls ? nullptr : e->d_wincl, e->d_act, e->d_fault, gate, rbflag, e->d_woff);
-    k3_copyback<<<blocks_for(N, BS), BS, 0, s>>>(N, gate, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
+    k3_copyback<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, gate, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
                                                  e->d_nt[0], e->d_sp[1], e->d_sp[0]);
     GC_LAUNCH_CHECK("engine skin re-sort");
     return skin_lists_t<ENS>(e, gate, rbflag);
@@ -1587,7 +1613,7 @@ int skin_init(gc_engine* e)
     const int N = e->N;
     hipStream_t s = e->s;
     if (e->cur == 1) {
-        k3_copyback<<<blocks_for(N, BS), BS, 0, s>>>(N, e->d_ones, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
+        k3_copyback<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, e->d_ones, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
                                                      e->d_nt[0], e->d_sp[1], e->d_sp[0]);
         GC_LAUNCH_CHECK("engine skin copy");
         e->cur = 0;
@@ -2177,7 +2203,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
     // first neighbour build (Particle::ConCubeList at init, Particle.cu:570)
     if ((st = launch_build(e, 0))) return fail(st);
     if (e->skin_mode() && e->cur == 1) {   // buffer 0 stays current in skin mode
-        k3_copyback<<<blocks_for(e->N, BS), BS, 0, s>>>(e->N, e->d_ones, e->d_pos64[1], e->d_pos64[0],
+        k3_copyback<<<gated_grid(blocks_for(e->N, BS)), BS, 0, s>>>(e->N, e->d_ones, e->d_pos64[1], e->d_pos64[0],
             e->d_nt[1], e->d_nt[0], e->d_sp[1], e->d_sp[0]);
         if ((st = hip_check(hipGetLastError(), "engine skin copy"))) return fail(st);
         e->cur = 0;
@@ -2606,7 +2632,7 @@ int restore_state(gc_engine* e, const std::vector<double>& pos, const std::vecto
         return st;
     if ((st = launch_build(e, 0))) return st;
     if (e->skin_mode() && e->cur == 1) {   // buffer 0 stays current in skin mode
-        k3_copyback<<<blocks_for(N, BS), BS, 0, s>>>(N, e->d_ones, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
+        k3_copyback<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, e->d_ones, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
                                                      e->d_nt[0], e->d_sp[1], e->d_sp[0]);
         if ((st = hip_check(hipGetLastError(), "engine skin copy"))) return st;
         e->cur = 0;
