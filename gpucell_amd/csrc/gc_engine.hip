@@ -428,22 +428,27 @@ __global__ __launch_bounds__(CUBE_SCAN_BS) void k2_cube_offsets(int K, const int
 
 // [4] (cube, id) into cube segments; clears the histogram for the next step
 // (every non-empty cube has exactly one rank-0 member).
+// Skin mode, a batch (rbflag set): the particles of replicas that do not
+// rebuild keep their slots and are not scattered (their counts are cleared
+// all the same).
 __global__ __launch_bounds__(BS) void k2_scatter(int n, const int* __restrict__ key,
     const int* __restrict__ rank, const int* __restrict__ sp, const int* __restrict__ offs,
     int* __restrict__ tk, int* __restrict__ tv, int* __restrict__ to, int* __restrict__ cnt,
-    const int* __restrict__ gate)
+    const int* __restrict__ gate, const int* __restrict__ rep_of, const int* __restrict__ rbflag)
 {
     if (gate && *gate == 0) return;
     for_groups((n + BS - 1) / BS, [&](int lb) {
     int p = lb * BS + threadIdx.x;
     if (p >= n) return;
     int k = key[p], r = rank[p];
+    if (r == 0) cnt[k] = 0;
+    const int id = sp[p];
+    if (rbflag && !rbflag[rep_of[id]]) return;
     int q = offs[k] + r;
     if (q < 0 || q >= n) return;   // only if the counts were inconsistent
     tk[q] = k;
-    tv[q] = sp[p];
+    tv[q] = id;
     to[q] = p;
-    if (r == 0) cnt[k] = 0;
     });
 }
 
@@ -454,9 +459,9 @@ __global__ __launch_bounds__(BS) void k2_scatter(int n, const int* __restrict__ 
 // the inverse of SortPart).
 // SK (skin mode, a list rebuild): the positions were advanced and wrapped
 // in place by the advance pass; the wrap codes of a rebuilding replica are
-// cleared (its lists start from these positions), the other replicas keep
-// their slots and wrap codes; the predictor is recomputed from the same
-// inputs as in the advance pass (the same bits).
+// cleared (its lists start from these positions), the slots of the other
+// replicas of a batch are not touched; the predictor is recomputed from the
+// same inputs as in the advance pass (the same bits).
 template <bool ENS, bool SK>
 __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, int move, int mmove, int wmove, const int* __restrict__ tk,
@@ -468,12 +473,16 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     int* __restrict__ key_new, float4* __restrict__ posid, tfloat3* __restrict__ coor,
     float4* __restrict__ velrho, tfloat3* __restrict__ velhalf, const int* __restrict__ wincl,
     int* __restrict__ act, int* __restrict__ fault, const int* __restrict__ gate,
-    const int* __restrict__ rbflag, unsigned char* __restrict__ woff)
+    const int* __restrict__ rbflag, unsigned char* __restrict__ woff, const int* __restrict__ sp_old)
 {
     if (SK && *gate == 0) return;   // skin mode: no rebuild this step
     for_groups((n + BS - 1) / BS, [&](int lb) {
     int q = lb * BS + threadIdx.x;
     if (q >= n) return;
+    // a batch: the slots of a replica that does not rebuild are left as the
+    // advance pass wrote them (a replica's slot range is fixed, as its
+    // particle count is)
+    if (SK && ENS && !rbflag[rep_of[sp_old[q]]]) return;
     int k = tk[q], id = tv[q];
     int b = max(offs[k], 0), e = min(offs[k + 1], n);
     int r = 0;
@@ -523,12 +532,8 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     // and so a finite force: it is caught here
     if (!isfinite(x + y + z)) atomicOr(fault, GC_FAULT_NONFINITE);
     float fx = (float)x, fy = (float)y, fz = (float)z;
-    int wc = 0;
-    if (SK) {   // a replica that does not rebuild keeps its slots (s == o) and wrap codes
-        if (ENS && !rbflag[rep_of[id]]) wc = woff[o];
-        woff[s] = (unsigned char)wc;
-    }
-    posid[s] = make_float4(fx, fy, fz, id_word(id, wc));
+    if (SK) woff[s] = 0;   // the new lists start from these positions
+    posid[s] = make_float4(fx, fy, fz, __int_as_float(id));
     coor[s] = tfloat3{fx, fy, fz};
     nt_new[s] = nt;
     sp_new[s] = id;
@@ -1273,12 +1278,14 @@ __global__ __launch_bounds__(TBS) void k3_keys(int n, GcRep R0, const GcRep* __r
 // Skin mode, a rebuild: the gather wrote buffer 1; buffer 0 stays current.
 __global__ __launch_bounds__(BS) void k3_copyback(int n, const int* __restrict__ gate,
     const double* __restrict__ pos1, double* __restrict__ pos0, const tint3* __restrict__ nt1,
-    tint3* __restrict__ nt0, const int* __restrict__ sp1, int* __restrict__ sp0)
+    tint3* __restrict__ nt0, const int* __restrict__ sp1, int* __restrict__ sp0,
+    const int* __restrict__ rep_of, const int* __restrict__ rbflag)
 {
     if (*gate == 0) return;
     for_groups((n + BS - 1) / BS, [&](int lb) {
     const int p = lb * BS + threadIdx.x;
     if (p >= n) return;
+    if (rbflag && !rbflag[rep_of[sp0[p]]]) return;   // a batch: the rebuilt replicas' slots only
     pos0[p] = pos1[p]; pos0[n + p] = pos1[n + p]; pos0[2 * n + p] = pos1[2 * n + p];
     nt0[p] = nt1[p];
     sp0[p] = sp1[p];
@@ -1509,13 +1516,13 @@ int launch_build_t(gc_engine* e, int move, Prof* pr = nullptr)
     GC_LAUNCH_CHECK("engine cube offsets");
     mark(pr, s, "cube offsets");
     k2_scatter<<<gb, BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[c], e->d_offs, e->d_tk, e->d_tv,
-                                 e->d_to, e->d_cnt, nullptr);
+                                 e->d_to, e->d_cnt, nullptr, nullptr, nullptr);
     GC_LAUNCH_CHECK("engine scatter");
     k2_sortgather<ENS, false><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, mfree, e->moving_walls,
         e->d_tk, e->d_tv,
         e->d_to, e->d_offs, e->d_curpos, e->d_pos64[c], e->d_pos64[nx], e->d_vout, e->d_force, e->d_nt[c],
         e->d_nt[nx], e->d_sp[nx], e->d_key, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf,
-        e->moving_walls ? nullptr : e->d_wincl, e->d_act, e->d_fault, nullptr, nullptr, nullptr);
+        e->moving_walls ? nullptr : e->d_wincl, e->d_act, e->d_fault, nullptr, nullptr, nullptr, nullptr);
     GC_LAUNCH_CHECK("engine sort/gather");
     e->cur = nx;
     return GC_OK;
@@ -1560,15 +1567,19 @@ int skin_rebuild_t(gc_engine* e, const int* gate, const int* rbflag)
     k2_tile_sums<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, gate);
     k2_cube_offsets<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, e->d_offs, e->d_ctr, 0,
                                                       e->d_nlover, gate);
+    // a batch re-sorts the rebuilding replicas only (the others keep their slots)
+    const int* rbf = ENS ? rbflag : nullptr;
     k2_scatter<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[0], e->d_offs,
-                                                            e->d_tk, e->d_tv, e->d_to, e->d_cnt, gate);
+                                                            e->d_tk, e->d_tv, e->d_to, e->d_cnt, gate,
+                                                            e->d_rep_of, rbf);
     k2_sortgather<ENS, true><<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, 0, mfree,
         e->moving_walls,
         e->d_tk, e->d_tv, e->d_to, e->d_offs, e->d_curpos, e->d_pos64[0], e->d_pos64[1], e->d_vout,
         e->d_force, e->d_nt[0], e->d_nt[1], e->d_sp[1], e->d_key, e->d_posid, e->d_coor, e->d_velrho,
-        e->d_velhalf, e->moving_walls ? nullptr : e->d_wincl, e->d_act, e->d_fault, gate, rbflag, e->d_woff);
+        e->d_velhalf, e->moving_walls ? nullptr : e->d_wincl, e->d_act, e->d_fault, gate, rbflag, e->d_woff,
+        e->d_sp[0]);
     k3_copyback<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, gate, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
-                                                 e->d_nt[0], e->d_sp[1], e->d_sp[0]);
+                                                 e->d_nt[0], e->d_sp[1], e->d_sp[0], e->d_rep_of, rbf);
     GC_LAUNCH_CHECK("engine skin re-sort");
     return skin_lists_t<ENS>(e, gate, rbflag);
 }
@@ -1614,7 +1625,7 @@ int skin_init(gc_engine* e)
     hipStream_t s = e->s;
     if (e->cur == 1) {
         k3_copyback<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, e->d_ones, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
-                                                     e->d_nt[0], e->d_sp[1], e->d_sp[0]);
+                                                     e->d_nt[0], e->d_sp[1], e->d_sp[0], nullptr, nullptr);
         GC_LAUNCH_CHECK("engine skin copy");
         e->cur = 0;
     }
@@ -2204,7 +2215,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
     if ((st = launch_build(e, 0))) return fail(st);
     if (e->skin_mode() && e->cur == 1) {   // buffer 0 stays current in skin mode
         k3_copyback<<<gated_grid(blocks_for(e->N, BS)), BS, 0, s>>>(e->N, e->d_ones, e->d_pos64[1], e->d_pos64[0],
-            e->d_nt[1], e->d_nt[0], e->d_sp[1], e->d_sp[0]);
+            e->d_nt[1], e->d_nt[0], e->d_sp[1], e->d_sp[0], nullptr, nullptr);
         if ((st = hip_check(hipGetLastError(), "engine skin copy"))) return fail(st);
         e->cur = 0;
     }
@@ -2633,7 +2644,7 @@ int restore_state(gc_engine* e, const std::vector<double>& pos, const std::vecto
     if ((st = launch_build(e, 0))) return st;
     if (e->skin_mode() && e->cur == 1) {   // buffer 0 stays current in skin mode
         k3_copyback<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, e->d_ones, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
-                                                     e->d_nt[0], e->d_sp[1], e->d_sp[0]);
+                                                     e->d_nt[0], e->d_sp[1], e->d_sp[0], nullptr, nullptr);
         if ((st = hip_check(hipGetLastError(), "engine skin copy"))) return st;
         e->cur = 0;
     }
