@@ -149,6 +149,31 @@ __device__ __forceinline__ void for_groups(int nb, Body&& body)
     for (int lb = G % GC_NXCD == 0 ? xcd_block(G) : (int)blockIdx.x; lb < nb; lb += G) body(lb);
 }
 
+// Skin mode, a batch: replica r holds slots [sb[r], sb[r + 1]) (and, with
+// the cube bases, cubes [cb[r], cb[r + 1])).  The replica of slot (or cube)
+// x, by bisection over the nrep + 1 bases.
+__device__ __forceinline__ int base_rep(const int* __restrict__ sb, int nrep, int x)
+{
+    int lo = 0, hi = nrep;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (sb[mid] <= x) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// true if slots (or cubes) [x0, x1] lie in replicas that do not rebuild:
+// the group has nothing to do (x0, x1 wave-uniform: scalar loads)
+__device__ __forceinline__ bool range_idle(const int* __restrict__ sb, int nrep, const int* __restrict__ rbflag,
+                                           int x0, int x1)
+{
+    const int r1 = base_rep(sb, nrep, x1);
+    for (int r = base_rep(sb, nrep, x0); r <= r1; ++r)
+        if (rbflag[r]) return false;
+    return true;
+}
+
 __device__ __forceinline__ void key_to_cc(int k, const GcRep& R, int& cx, int& cy, int& cz)
 {
     k -= R.cube_off;
@@ -307,15 +332,17 @@ __device__ __forceinline__ void hist_rank(bool act, int k, int p, int* __restric
 {
     const int lane = threadIdx.x & 63;
     const int prev = __shfl_up(k, 1);
-    const bool head = act && (lane == 0 || prev != k);
-    const unsigned long long heads = __ballot(head);
-    const int nact = __popcll(__ballot(act));
+    // runs of equal keys; an inactive lane ends a run (the active lanes
+    // need not be a prefix of the wave)
+    const bool brk = !act || lane == 0 || prev != k;
+    const unsigned long long heads = __ballot(brk);
     const unsigned long long above = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
-    const int next = above ? (int)__ffsll((long long)above) - 1 : nact;
+    const int next = above ? (int)__ffsll((long long)above) - 1 : 64;
     int base = 0;
-    if (head) base = atomicAdd(&cnt[k], next - lane);
+    if (act && brk) base = atomicAdd(&cnt[k], next - lane);
+    // the nearest break at or below an active lane is its run's head
     const unsigned long long upto = lane == 63 ? heads : heads & ((2ull << lane) - 1ull);
-    const int myhead = upto ? 63 - __clzll((long long)upto) : 0;
+    const int myhead = 63 - __clzll((long long)upto);   // lane 0 always breaks
     base = __shfl(base, myhead);
     if (act) rank[p] = base + (lane - myhead);
 }
@@ -429,21 +456,23 @@ __global__ __launch_bounds__(CUBE_SCAN_BS) void k2_cube_offsets(int K, const int
 // [4] (cube, id) into cube segments; clears the histogram for the next step
 // (every non-empty cube has exactly one rank-0 member).
 // Skin mode, a batch (rbflag set): the particles of replicas that do not
-// rebuild keep their slots and are not scattered (their counts are cleared
-// all the same).
+// rebuild keep their slots and are not scattered, and the counts persist
+// (cleared per replica before its key pass).
 __global__ __launch_bounds__(BS) void k2_scatter(int n, const int* __restrict__ key,
     const int* __restrict__ rank, const int* __restrict__ sp, const int* __restrict__ offs,
     int* __restrict__ tk, int* __restrict__ tv, int* __restrict__ to, int* __restrict__ cnt,
-    const int* __restrict__ gate, const int* __restrict__ rep_of, const int* __restrict__ rbflag)
+    const int* __restrict__ gate, const int* __restrict__ rep_of, const int* __restrict__ rbflag,
+    const int* __restrict__ slotbase, int nrep)
 {
     if (gate && *gate == 0) return;
     for_groups((n + BS - 1) / BS, [&](int lb) {
+    if (rbflag && range_idle(slotbase, nrep, rbflag, lb * BS, min(lb * BS + BS, n) - 1)) return;
     int p = lb * BS + threadIdx.x;
     if (p >= n) return;
-    int k = key[p], r = rank[p];
-    if (r == 0) cnt[k] = 0;
     const int id = sp[p];
     if (rbflag && !rbflag[rep_of[id]]) return;
+    int k = key[p], r = rank[p];
+    if (!rbflag && r == 0) cnt[k] = 0;
     int q = offs[k] + r;
     if (q < 0 || q >= n) return;   // only if the counts were inconsistent
     tk[q] = k;
@@ -473,10 +502,12 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     int* __restrict__ key_new, float4* __restrict__ posid, tfloat3* __restrict__ coor,
     float4* __restrict__ velrho, tfloat3* __restrict__ velhalf, const int* __restrict__ wincl,
     int* __restrict__ act, int* __restrict__ fault, const int* __restrict__ gate,
-    const int* __restrict__ rbflag, unsigned char* __restrict__ woff, const int* __restrict__ sp_old)
+    const int* __restrict__ rbflag, unsigned char* __restrict__ woff, const int* __restrict__ sp_old,
+    const int* __restrict__ slotbase, int nrep)
 {
     if (SK && *gate == 0) return;   // skin mode: no rebuild this step
     for_groups((n + BS - 1) / BS, [&](int lb) {
+    if (SK && ENS && range_idle(slotbase, nrep, rbflag, lb * BS, min(lb * BS + BS, n) - 1)) return;
     int q = lb * BS + threadIdx.x;
     if (q >= n) return;
     // a batch: the slots of a replica that does not rebuild are left as the
@@ -627,7 +658,7 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
     const int* __restrict__ rep_of, const float4* __restrict__ posid, float4* __restrict__ velrho,
     const int* __restrict__ offs, const int* __restrict__ key, int* __restrict__ nl,
     int* __restrict__ nlc, int* __restrict__ nlover, const int* __restrict__ gate,
-    const int* __restrict__ rbflag, tfloat3* __restrict__ rbpos)
+    const int* __restrict__ rbflag, tfloat3* __restrict__ rbpos, const int* __restrict__ slotbase, int nrep)
 {
     constexpr int CAP = LB ? VL_CAP : NL_CAP;
     constexpr int ROWS = LB ? VL_ROWS : NL_CAP;
@@ -635,7 +666,13 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
     __shared__ float4 stage[TBS / 64][TSTAGE > 0 ? TSTAGE : 1];
     const int lane = threadIdx.x & 63;
     float4* st = stage[threadIdx.x >> 6];
-    const int p = xcd_block((n + TBS - 1) / TBS) * TBS + threadIdx.x;
+    // one group of TBS slots; a batch's list build strides over a fixed grid
+    // (the rebuilding replicas' slots are contiguous runs, which a grid of
+    // one group per TBS slots would place on one or two XCDs)
+    auto group = [&](int lb) {
+    const int p0 = lb * TBS;
+    if (LB && ENS && range_idle(slotbase, nrep, rbflag, p0, min(p0 + TBS, n) - 1)) return;
+    const int p = p0 + threadIdx.x;
     const bool inb = p < n;
     const float4 pi = inb ? posid[p] : make_float4(0.f, 0.f, 0.f, 0.f);
     const int id = inb ? pid_of(pi.w) : 0;
@@ -741,6 +778,9 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
         velrho[p] = v;
     }
     });
+    };
+    if (LB && ENS) for_groups((n + TBS - 1) / TBS, group);
+    else group(xcd_block((n + TBS - 1) / TBS));
 }
 
 // [10-13,17] SDPD pair force with the pressure from the EOS on the fly,
@@ -1175,7 +1215,7 @@ constexpr int REDUCE_BS = 256, REDUCE_NB = 64;
 __global__ __launch_bounds__(REDUCE_BS) void k3_reduce(int nrep, const float* __restrict__ bmax,
     const int* __restrict__ brep, int nb_adv, int ms_off, int nb_ms, unsigned* __restrict__ disp2)
 {
-    const int E = nb_adv + nb_ms, T = REDUCE_NB * REDUCE_BS;
+    const int E = nb_adv + nb_ms, T = (int)gridDim.x * REDUCE_BS;
     const int C = (E + T - 1) / T;
     const int t = blockIdx.x * REDUCE_BS + threadIdx.x;
     float m[DSTAT];
@@ -1208,7 +1248,18 @@ __global__ __launch_bounds__(REDUCE_BS) void k3_reduce(int nrep, const float* __
             if (v > 0.0f) atomicMax(&disp2[threadIdx.x], __float_as_uint(v));
         }
     } else {
-        flush();
+        // a wave whose lanes end on one replica (nearly all) reduces first:
+        // one atomic per statistic instead of 64 on the same word
+        const int c0 = __builtin_amdgcn_readfirstlane(cur);
+        if (__ballot(cur >= 0 && cur != c0) == 0) {
+            for (int k = 0; k < DSTAT; ++k) {
+                const float v = wave_maxf(cur >= 0 ? m[k] : 0.0f);
+                if ((threadIdx.x & 63) == 0 && c0 >= 0 && v > 0.0f)
+                    atomicMax(&disp2[DSTAT * c0 + k], __float_as_uint(v));
+            }
+        } else {
+            flush();
+        }
     }
 }
 
@@ -1243,46 +1294,68 @@ __global__ __launch_bounds__(64) void k3_decide(int nrep, const GcRep* __restric
 }
 
 // [3] skin mode, a rebuild: cube key of each particle of a rebuilding
-// replica from its position; the other replicas keep their keys, hence
-// their slots.  Histogram rank as in the per-step key pass.
+// replica from its position, histogram rank as in the per-step key pass.
+// In a batch the other replicas keep their slots and their cube counts
+// (which persist between rebuilds) and are skipped.
 template <bool ENS, int TBS>
 __global__ __launch_bounds__(TBS) void k3_keys(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, const int* __restrict__ gate, const int* __restrict__ rbflag,
-    const double* __restrict__ pos64, const int* __restrict__ sp, const int* __restrict__ key_old,
+    const int* __restrict__ slotbase, int nrep, const double* __restrict__ pos64, const int* __restrict__ sp,
     int* __restrict__ key, int* __restrict__ rank, int* __restrict__ cnt)
 {
     if (*gate == 0) return;
     for_groups((n + TBS - 1) / TBS, [&](int lgb) {
+    if (ENS && range_idle(slotbase, nrep, rbflag, lgb * TBS, min(lgb * TBS + TBS, n) - 1)) return;
     const int p = lgb * TBS + threadIdx.x;
-    const bool act = p < n;
+    bool act = p < n;
+    const int id = act ? sp[p] : 0;
+    if (ENS && act) act = rbflag[rep_of[id]] != 0;
     if (__ballot(act) == 0) return;
     int k = -1;
-    const int id = act ? sp[p] : 0;
     with_rep<ENS>(R0, reps, rep_of, act, id, [&](const GcRep& R) __attribute__((always_inline)) {
     if (!act) return;
-    if (ENS && !rbflag[rep_of[id]]) { k = key_old[p]; }
-    else {
-        const GcParams& P = R.P;
-        const double u[3] = {pos64[p], pos64[n + p], pos64[2 * n + p]};
-        int cx = cube_coord((float)u[0], R.bbox[0], P.off[0], R.invw[0], R.nc[0]);
-        int cy = cube_coord((float)u[1], R.bbox[2], P.off[1], R.invw[1], R.nc[1]);
-        int cz = cube_coord((float)u[2], R.bbox[4], P.off[2], R.invw[2], R.nc[2]);
-        k = R.cube_off + cx + R.nc[0] * (cy + R.nc[1] * cz);
-    }
+    const GcParams& P = R.P;
+    const double u[3] = {pos64[p], pos64[n + p], pos64[2 * n + p]};
+    int cx = cube_coord((float)u[0], R.bbox[0], P.off[0], R.invw[0], R.nc[0]);
+    int cy = cube_coord((float)u[1], R.bbox[2], P.off[1], R.invw[1], R.nc[1]);
+    int cz = cube_coord((float)u[2], R.bbox[4], P.off[2], R.invw[2], R.nc[2]);
+    k = R.cube_off + cx + R.nc[0] * (cy + R.nc[1] * cz);
     key[p] = k;
     });
     hist_rank(act, k, p, rank, cnt);
     });
 }
 
+// Skin mode, a batch rebuild: the rebuilding replicas' cube counts cleared
+// before their key pass (the others' stay as their last rebuild left them).
+__global__ __launch_bounds__(BS) void k3_clear_counts(int K, const int* __restrict__ gate,
+    const int* __restrict__ rbflag, const int* __restrict__ cubebase, int nrep, int* __restrict__ cnt)
+{
+    if (*gate == 0) return;
+    for_groups((K + BS - 1) / BS, [&](int lb) {
+    if (range_idle(cubebase, nrep, rbflag, lb * BS, min(lb * BS + BS, K) - 1)) return;
+    const int k = lb * BS + threadIdx.x;
+    if (k < K && rbflag[base_rep(cubebase, nrep, k)]) cnt[k] = 0;
+    });
+}
+
+// Skin mode, a batch, after a host-driven build (which clears the counts):
+// the cube counts restored from the offsets.
+__global__ __launch_bounds__(BS) void k3_counts(int K, const int* __restrict__ offs, int* __restrict__ cnt)
+{
+    const int k = blockIdx.x * BS + threadIdx.x;
+    if (k < K) cnt[k] = offs[k + 1] - offs[k];
+}
+
 // Skin mode, a rebuild: the gather wrote buffer 1; buffer 0 stays current.
 __global__ __launch_bounds__(BS) void k3_copyback(int n, const int* __restrict__ gate,
     const double* __restrict__ pos1, double* __restrict__ pos0, const tint3* __restrict__ nt1,
     tint3* __restrict__ nt0, const int* __restrict__ sp1, int* __restrict__ sp0,
-    const int* __restrict__ rep_of, const int* __restrict__ rbflag)
+    const int* __restrict__ rep_of, const int* __restrict__ rbflag, const int* __restrict__ slotbase, int nrep)
 {
     if (*gate == 0) return;
     for_groups((n + BS - 1) / BS, [&](int lb) {
+    if (rbflag && range_idle(slotbase, nrep, rbflag, lb * BS, min(lb * BS + BS, n) - 1)) return;
     const int p = lb * BS + threadIdx.x;
     if (p >= n) return;
     if (rbflag && !rbflag[rep_of[sp0[p]]]) return;   // a batch: the rebuilt replicas' slots only
@@ -1516,13 +1589,14 @@ int launch_build_t(gc_engine* e, int move, Prof* pr = nullptr)
     GC_LAUNCH_CHECK("engine cube offsets");
     mark(pr, s, "cube offsets");
     k2_scatter<<<gb, BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[c], e->d_offs, e->d_tk, e->d_tv,
-                                 e->d_to, e->d_cnt, nullptr, nullptr, nullptr);
+                                 e->d_to, e->d_cnt, nullptr, nullptr, nullptr, nullptr, 0);
     GC_LAUNCH_CHECK("engine scatter");
     k2_sortgather<ENS, false><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, mfree, e->moving_walls,
         e->d_tk, e->d_tv,
         e->d_to, e->d_offs, e->d_curpos, e->d_pos64[c], e->d_pos64[nx], e->d_vout, e->d_force, e->d_nt[c],
         e->d_nt[nx], e->d_sp[nx], e->d_key, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf,
-        e->moving_walls ? nullptr : e->d_wincl, e->d_act, e->d_fault, nullptr, nullptr, nullptr, nullptr);
+        e->moving_walls ? nullptr : e->d_wincl, e->d_act, e->d_fault, nullptr, nullptr, nullptr, nullptr,
+        nullptr, 0);
     GC_LAUNCH_CHECK("engine sort/gather");
     e->cur = nx;
     return GC_OK;
@@ -1562,8 +1636,11 @@ int skin_rebuild_t(gc_engine* e, const int* gate, const int* rbflag)
     const int gb = xcd_grid(blocks_for(N, BS));
     const GcRep& R0 = e->hreps[0];
     const int mfree = e->ms == nullptr;
+    if (ENS)   // the counts of the other replicas persist
+        k3_clear_counts<<<gated_grid(blocks_for(e->K, BS)), BS, 0, s>>>(e->K, gate, rbflag, e->d_cubebase, e->nrep,
+                                                                        e->d_cnt);
     k3_keys<ENS, 256><<<gated_grid(blocks_for(N, 256)), 256, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, gate, rbflag,
-        e->d_pos64[0], e->d_sp[0], e->d_key, e->d_key0, e->d_rank, e->d_cnt);
+        e->d_slotbase, e->nrep, e->d_pos64[0], e->d_sp[0], e->d_key0, e->d_rank, e->d_cnt);
     k2_tile_sums<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, gate);
     k2_cube_offsets<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, e->d_offs, e->d_ctr, 0,
                                                       e->d_nlover, gate);
@@ -1571,15 +1648,16 @@ int skin_rebuild_t(gc_engine* e, const int* gate, const int* rbflag)
     const int* rbf = ENS ? rbflag : nullptr;
     k2_scatter<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[0], e->d_offs,
                                                             e->d_tk, e->d_tv, e->d_to, e->d_cnt, gate,
-                                                            e->d_rep_of, rbf);
+                                                            e->d_rep_of, rbf, e->d_slotbase, e->nrep);
     k2_sortgather<ENS, true><<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, 0, mfree,
         e->moving_walls,
         e->d_tk, e->d_tv, e->d_to, e->d_offs, e->d_curpos, e->d_pos64[0], e->d_pos64[1], e->d_vout,
         e->d_force, e->d_nt[0], e->d_nt[1], e->d_sp[1], e->d_key, e->d_posid, e->d_coor, e->d_velrho,
         e->d_velhalf, e->moving_walls ? nullptr : e->d_wincl, e->d_act, e->d_fault, gate, rbflag, e->d_woff,
-        e->d_sp[0]);
+        e->d_sp[0], e->d_slotbase, e->nrep);
     k3_copyback<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, gate, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
-                                                 e->d_nt[0], e->d_sp[1], e->d_sp[0], e->d_rep_of, rbf);
+                                                 e->d_nt[0], e->d_sp[1], e->d_sp[0], e->d_rep_of, rbf,
+                                                 e->d_slotbase, e->nrep);
     GC_LAUNCH_CHECK("engine skin re-sort");
     return skin_lists_t<ENS>(e, gate, rbflag);
 }
@@ -1593,8 +1671,12 @@ int skin_lists_t(gc_engine* e, const int* gate, const int* rbflag)
     const GcRep& R0 = e->hreps[0];
     {
         auto L = [&](auto kern, int tbs) {
-            kern<<<xcd_grid(blocks_for(N, tbs)), tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid,
-                e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc, e->d_nlover, gate, rbflag, e->d_rbpos);
+            // a batch: a fixed grid (2048 groups' worth of 256 lanes) striding over the groups
+            const int grid = ENS ? std::min(xcd_grid(blocks_for(N, tbs)), GATED_MAX_GROUPS * (256 / tbs))
+                                 : xcd_grid(blocks_for(N, tbs));
+            kern<<<grid, tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid,
+                e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc, e->d_nlover, gate, rbflag, e->d_rbpos,
+                e->d_slotbase, e->nrep);
         };
         switch (e->tune.v[TUNE_DENSITY]) {   // block size x LDS row stage
             case 0: L(k2_density<ENS, 64, 0, true>, 64); break;
@@ -1625,9 +1707,13 @@ int skin_init(gc_engine* e)
     hipStream_t s = e->s;
     if (e->cur == 1) {
         k3_copyback<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, e->d_ones, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
-                                                     e->d_nt[0], e->d_sp[1], e->d_sp[0], nullptr, nullptr);
+                                                     e->d_nt[0], e->d_sp[1], e->d_sp[0], nullptr, nullptr, nullptr, 0);
         GC_LAUNCH_CHECK("engine skin copy");
         e->cur = 0;
+    }
+    if (e->ens()) {   // the cube counts persist between a batch's rebuilds
+        k3_counts<<<blocks_for(e->K, BS), BS, 0, s>>>(e->K, e->d_offs, e->d_cnt);
+        GC_LAUNCH_CHECK("engine skin counts");
     }
     int st = e->ens() ? skin_lists_t<true>(e, e->d_ones, e->d_ones) : skin_lists_t<false>(e, e->d_ones, e->d_ones);
     if (st) return st;
@@ -1730,7 +1816,10 @@ int skin_fluid_t(gc_engine* e, int driveon, Prof* pr)
     // the substeps of the previous step wrote their vertices' maxima after
     // the advance's region (membrane substepping: one suspension)
     const int nb_ms = e->ms ? blocks_for(e->nm, BS) : 0;
-    k3_reduce<<<REDUCE_NB, REDUCE_BS, 0, s>>>(e->nrep, e->d_bmax, e->d_brep, nb_adv, skin_ms_off(e), nb_ms,
+    // one word set: few groups, each reducing a long run first; a batch: one
+    // entry per thread, so that the loads are not a chain of dependent ones
+    const int nb_red = e->ens() ? blocks_for(nb_adv + nb_ms, REDUCE_BS) : REDUCE_NB;
+    k3_reduce<<<nb_red, REDUCE_BS, 0, s>>>(e->nrep, e->d_bmax, e->d_brep, nb_adv, skin_ms_off(e), nb_ms,
                                                e->d_disp2);
     k3_decide<<<1, 64, 0, s>>>(e->nrep, e->d_reps, e->d_disp2, e->d_rbflag, e->d_rbany, e->d_nrebuild, e->d_ctr);
     GC_LAUNCH_CHECK("engine advance");
@@ -1785,7 +1874,8 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
     {
         auto L = [&](auto kern, int tbs) {
             kern<<<xcd_grid(blocks_for(N, tbs)), tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid,
-                e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc, e->d_nlover, nullptr, nullptr, nullptr);
+                e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc, e->d_nlover, nullptr, nullptr, nullptr,
+                nullptr, 0);
         };
         switch (e->tune.v[TUNE_DENSITY]) {   // block size x LDS row stage
             case 0: L(k2_density<ENS, 64, 0>, 64); break;
@@ -2166,6 +2256,20 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
             (st = upload(e->d_ones, ones.data(), nr, e->s)) || (st = upload(e->d_nrebuild, zero.data(), nr, e->s)) ||
             (st = upload(e->d_rbany, zero.data(), 1, e->s)))
             return fail(st);
+        if (e->ens()) {
+            std::vector<int> sb(nr + 1, 0), cb(nr + 1, 0);
+            for (int r = 0; r < nrep; ++r) {
+                const RepMeta& m = e->meta[r];
+                sb[r + 1] = sb[r] + m.nb1 + m.nb2 + m.nfl + m.nm;
+                cb[r] = e->hreps[r].cube_off;
+            }
+            cb[nr] = K;
+            if (sb[nr] != N) return fail(set_error(GC_ERR_INVALID, "EngineCreate: replica sizes"));
+            if ((st = e->alloc(&e->d_slotbase, nr + 1)) || (st = e->alloc(&e->d_cubebase, nr + 1)) ||
+                (st = upload(e->d_slotbase, sb.data(), nr + 1, e->s)) ||
+                (st = upload(e->d_cubebase, cb.data(), nr + 1, e->s)))
+                return fail(st);
+        }
     }
     hipStream_t s = e->s;
     std::vector<float4> vout_id(N);
@@ -2215,7 +2319,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
     if ((st = launch_build(e, 0))) return fail(st);
     if (e->skin_mode() && e->cur == 1) {   // buffer 0 stays current in skin mode
         k3_copyback<<<gated_grid(blocks_for(e->N, BS)), BS, 0, s>>>(e->N, e->d_ones, e->d_pos64[1], e->d_pos64[0],
-            e->d_nt[1], e->d_nt[0], e->d_sp[1], e->d_sp[0], nullptr, nullptr);
+            e->d_nt[1], e->d_nt[0], e->d_sp[1], e->d_sp[0], nullptr, nullptr, nullptr, 0);
         if ((st = hip_check(hipGetLastError(), "engine skin copy"))) return fail(st);
         e->cur = 0;
     }
@@ -2641,10 +2745,14 @@ int restore_state(gc_engine* e, const std::vector<double>& pos, const std::vecto
         (st = upload(e->d_sp[0], idn.data(), N, s)) || (st = upload(e->d_curpos, idn.data(), N, s)) ||
         (st = upload(e->d_ctr, &ctr, 1, s)) || (st = upload(e->d_fault, &zero, 1, s)))
         return st;
+    // the build's histogram starts from zero counts (a batch in skin mode keeps them)
+    if (e->skin_mode() && e->ens() &&
+        (st = hip_check(hipMemsetAsync(e->d_cnt, 0, sizeof(int) * ((size_t)e->K + 1), s), "reset counts")))
+        return st;
     if ((st = launch_build(e, 0))) return st;
     if (e->skin_mode() && e->cur == 1) {   // buffer 0 stays current in skin mode
         k3_copyback<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, e->d_ones, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
-                                                     e->d_nt[0], e->d_sp[1], e->d_sp[0], nullptr, nullptr);
+                                                     e->d_nt[0], e->d_sp[1], e->d_sp[0], nullptr, nullptr, nullptr, 0);
         if ((st = hip_check(hipGetLastError(), "engine skin copy"))) return st;
         e->cur = 0;
     }

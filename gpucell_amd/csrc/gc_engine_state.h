@@ -96,6 +96,10 @@ struct gc_engine {
     int *d_rbflag = nullptr, *d_rbany = nullptr, *d_nrebuild = nullptr;
     int* d_ones = nullptr;   // an all-ones flag array (host-driven rebuilds of every replica)
     unsigned char* d_woff = nullptr;   // per slot: periodic wraps since the last rebuild (image code)
+    // a batch: first slot and first cube of each replica (nrep + 1 entries,
+    // the last = N or K); a replica's slot range is fixed (its particle
+    // count is), so a rebuild re-sorts the rebuilding replicas only
+    int *d_slotbase = nullptr, *d_cubebase = nullptr;
     bool skin_mode() const { return skin > 0.0f; }
     unsigned long long* d_ctr;
     int* d_fault = nullptr;    // GC_FAULT_* bits raised by the kernels (sticky)
