@@ -70,6 +70,10 @@ inline int nl_stride(const gc_engine* e) { return (e->skin_mode() ? VL_ROWS : NL
 // logical groups; an empty launch then dispatches at most this many)
 constexpr int GATED_MAX_GROUPS = 2048;
 inline int gated_grid(int nb) { return std::min(xcd_grid(nb), GATED_MAX_GROUPS); }
+// a batch's rebuild: activity bytes per 64-wide chunk of slots (then of
+// cubes), each table padded to whole 32-bit words
+inline int chunks64(int n) { return (n + 63) / 64; }
+inline size_t act_bytes(int n) { return ((size_t)chunks64(n) + 3) / 4 * 4; }
 // Skin mode: workgroup displacement maxima, the advance's groups (at most
 // one per 64 particles) first, the membrane substeps' groups after them.
 inline int skin_ms_off(const gc_engine* e) { return (e->N + 63) / 64; }
@@ -163,15 +167,37 @@ __device__ __forceinline__ int base_rep(const int* __restrict__ sb, int nrep, in
     return lo;
 }
 
-// true if slots (or cubes) [x0, x1] lie in replicas that do not rebuild:
-// the group has nothing to do (x0, x1 wave-uniform: scalar loads)
-__device__ __forceinline__ bool range_idle(const int* __restrict__ sb, int nrep, const int* __restrict__ rbflag,
-                                           int x0, int x1)
+// Activity of a group of TBS slots (or cubes) from the bytes of its 64-wide
+// chunks (1: a rebuilding replica owns part of the chunk), one load.
+template <int TBS>
+__device__ __forceinline__ bool group_active(const unsigned char* __restrict__ act, int lb)
 {
-    const int r1 = base_rep(sb, nrep, x1);
-    for (int r = base_rep(sb, nrep, x0); r <= r1; ++r)
-        if (rbflag[r]) return false;
-    return true;
+    static_assert(TBS == 64 || TBS == 128 || TBS == 256, "group of 1, 2 or 4 chunks");
+    if constexpr (TBS == 256) return reinterpret_cast<const unsigned*>(act)[lb] != 0u;
+    else if constexpr (TBS == 128) return reinterpret_cast<const unsigned short*>(act)[lb] != 0;
+    else return act[lb] != 0;
+}
+
+// for_groups over the active groups only (a batch's rebuild: the groups of
+// the rebuilding replicas).  The activity of all of a workgroup's logical
+// groups is loaded at once, one per lane, so an idle group costs no load of
+// its own; act == nullptr: every group.
+template <int TBS, class Body>
+__device__ __forceinline__ void for_groups_of(int nb, const unsigned char* __restrict__ act, Body&& body)
+{
+    if (!act) { for_groups(nb, body); return; }
+    const int G = gridDim.x;
+    const int first = G % GC_NXCD == 0 ? xcd_block(G) : (int)blockIdx.x;
+    const int lane = threadIdx.x & 63;
+    for (int base = first; base < nb; base += 64 * G) {
+        const int lb = base + lane * G;
+        unsigned long long m = __ballot(lb < nb && group_active<TBS>(act, lb));
+        while (m) {
+            const int k = (int)__ffsll((long long)m) - 1;
+            m &= m - 1;
+            body(base + k * G);
+        }
+    }
 }
 
 __device__ __forceinline__ void key_to_cc(int k, const GcRep& R, int& cx, int& cy, int& cz)
@@ -462,11 +488,10 @@ __global__ __launch_bounds__(BS) void k2_scatter(int n, const int* __restrict__ 
     const int* __restrict__ rank, const int* __restrict__ sp, const int* __restrict__ offs,
     int* __restrict__ tk, int* __restrict__ tv, int* __restrict__ to, int* __restrict__ cnt,
     const int* __restrict__ gate, const int* __restrict__ rep_of, const int* __restrict__ rbflag,
-    const int* __restrict__ slotbase, int nrep)
+    const unsigned char* __restrict__ gact)
 {
     if (gate && *gate == 0) return;
-    for_groups((n + BS - 1) / BS, [&](int lb) {
-    if (rbflag && range_idle(slotbase, nrep, rbflag, lb * BS, min(lb * BS + BS, n) - 1)) return;
+    for_groups_of<BS>((n + BS - 1) / BS, gact, [&](int lb) {
     int p = lb * BS + threadIdx.x;
     if (p >= n) return;
     const int id = sp[p];
@@ -503,11 +528,10 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     float4* __restrict__ velrho, tfloat3* __restrict__ velhalf, const int* __restrict__ wincl,
     int* __restrict__ act, int* __restrict__ fault, const int* __restrict__ gate,
     const int* __restrict__ rbflag, unsigned char* __restrict__ woff, const int* __restrict__ sp_old,
-    const int* __restrict__ slotbase, int nrep)
+    const unsigned char* __restrict__ gact)
 {
     if (SK && *gate == 0) return;   // skin mode: no rebuild this step
-    for_groups((n + BS - 1) / BS, [&](int lb) {
-    if (SK && ENS && range_idle(slotbase, nrep, rbflag, lb * BS, min(lb * BS + BS, n) - 1)) return;
+    for_groups_of<BS>((n + BS - 1) / BS, SK && ENS ? gact : nullptr, [&](int lb) {
     int q = lb * BS + threadIdx.x;
     if (q >= n) return;
     // a batch: the slots of a replica that does not rebuild are left as the
@@ -565,7 +589,8 @@ __global__ __launch_bounds__(BS) void k2_sortgather(int n, GcRep R0, const GcRep
     float fx = (float)x, fy = (float)y, fz = (float)z;
     if (SK) woff[s] = 0;   // the new lists start from these positions
     posid[s] = make_float4(fx, fy, fz, __int_as_float(id));
-    coor[s] = tfloat3{fx, fy, fz};
+    // skin mode: coor is read for membrane vertices only (posid holds the rest)
+    if (!SK || id >= R0.P.nc) coor[s] = tfloat3{fx, fy, fz};
     nt_new[s] = nt;
     sp_new[s] = id;
     key_new[s] = k;
@@ -658,7 +683,7 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
     const int* __restrict__ rep_of, const float4* __restrict__ posid, float4* __restrict__ velrho,
     const int* __restrict__ offs, const int* __restrict__ key, int* __restrict__ nl,
     int* __restrict__ nlc, int* __restrict__ nlover, const int* __restrict__ gate,
-    const int* __restrict__ rbflag, tfloat3* __restrict__ rbpos, const int* __restrict__ slotbase, int nrep)
+    const int* __restrict__ rbflag, tfloat3* __restrict__ rbpos, const unsigned char* __restrict__ gact)
 {
     constexpr int CAP = LB ? VL_CAP : NL_CAP;
     constexpr int ROWS = LB ? VL_ROWS : NL_CAP;
@@ -670,9 +695,7 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
     // (the rebuilding replicas' slots are contiguous runs, which a grid of
     // one group per TBS slots would place on one or two XCDs)
     auto group = [&](int lb) {
-    const int p0 = lb * TBS;
-    if (LB && ENS && range_idle(slotbase, nrep, rbflag, p0, min(p0 + TBS, n) - 1)) return;
-    const int p = p0 + threadIdx.x;
+    const int p = lb * TBS + threadIdx.x;
     const bool inb = p < n;
     const float4 pi = inb ? posid[p] : make_float4(0.f, 0.f, 0.f, 0.f);
     const int id = inb ? pid_of(pi.w) : 0;
@@ -779,7 +802,7 @@ __global__ __launch_bounds__(TBS) void k2_density(int n, GcRep R0, const GcRep* 
     }
     });
     };
-    if (LB && ENS) for_groups((n + TBS - 1) / TBS, group);
+    if (LB && ENS) for_groups_of<TBS>((n + TBS - 1) / TBS, gact, group);
     else group(xcd_block((n + TBS - 1) / TBS));
 }
 
@@ -1215,7 +1238,7 @@ constexpr int REDUCE_BS = 256, REDUCE_NB = 64;
 __global__ __launch_bounds__(REDUCE_BS) void k3_reduce(int nrep, const float* __restrict__ bmax,
     const int* __restrict__ brep, int nb_adv, int ms_off, int nb_ms, unsigned* __restrict__ disp2)
 {
-    const int E = nb_adv + nb_ms, T = (int)gridDim.x * REDUCE_BS;
+    const int E = nb_adv + nb_ms, T = REDUCE_NB * REDUCE_BS;
     const int C = (E + T - 1) / T;
     const int t = blockIdx.x * REDUCE_BS + threadIdx.x;
     float m[DSTAT];
@@ -1263,6 +1286,35 @@ __global__ __launch_bounds__(REDUCE_BS) void k3_reduce(int nrep, const float* __
     }
 }
 
+// Skin mode, a batch: workgroup r reduces the statistics of replica r's
+// advance groups (its slots are contiguous: groups slotbase[r] / gsz to
+// (slotbase[r + 1] - 1) / gsz, gsz the advance's group size); the groups
+// spanning two replicas put theirs into disp2 directly.  One atomic per
+// statistic and replica.
+__global__ __launch_bounds__(REDUCE_BS) void k3_reduce_rep(const float* __restrict__ bmax,
+    const int* __restrict__ brep, const int* __restrict__ slotbase, int gsz, unsigned* __restrict__ disp2)
+{
+    const int r = blockIdx.x;
+    const int g0 = slotbase[r] / gsz, g1 = (slotbase[r + 1] - 1) / gsz;
+    float m[DSTAT];
+    for (int k = 0; k < DSTAT; ++k) m[k] = 0.0f;
+    for (int q = g0 + (int)threadIdx.x; q <= g1; q += REDUCE_BS) {
+        if (brep[q] != r) continue;
+        for (int k = 0; k < DSTAT; ++k) m[k] = fmaxf(m[k], bmax[DSTAT * q + k]);
+    }
+    __shared__ float sm[REDUCE_BS / 64][DSTAT];
+    for (int k = 0; k < DSTAT; ++k) {
+        const float v = wave_maxf(m[k]);
+        if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < DSTAT) {
+        float v = 0.0f;
+        for (int w = 0; w < REDUCE_BS / 64; ++w) v = fmaxf(v, sm[w][threadIdx.x]);
+        if (v > 0.0f) atomicMax(&disp2[DSTAT * r + threadIdx.x], __float_as_uint(v));
+    }
+}
+
 // Skin mode, per replica from its displacement statistics since its last
 // rebuild: the lists stay valid while every pair's relative displacement
 // |D_i - D_j| is at most the skin, which holds if every |D_i| <= skin/2 or
@@ -1300,12 +1352,11 @@ __global__ __launch_bounds__(64) void k3_decide(int nrep, const GcRep* __restric
 template <bool ENS, int TBS>
 __global__ __launch_bounds__(TBS) void k3_keys(int n, GcRep R0, const GcRep* __restrict__ reps,
     const int* __restrict__ rep_of, const int* __restrict__ gate, const int* __restrict__ rbflag,
-    const int* __restrict__ slotbase, int nrep, const double* __restrict__ pos64, const int* __restrict__ sp,
+    const unsigned char* __restrict__ gact, const double* __restrict__ pos64, const int* __restrict__ sp,
     int* __restrict__ key, int* __restrict__ rank, int* __restrict__ cnt)
 {
     if (*gate == 0) return;
-    for_groups((n + TBS - 1) / TBS, [&](int lgb) {
-    if (ENS && range_idle(slotbase, nrep, rbflag, lgb * TBS, min(lgb * TBS + TBS, n) - 1)) return;
+    for_groups_of<TBS>((n + TBS - 1) / TBS, ENS ? gact : nullptr, [&](int lgb) {
     const int p = lgb * TBS + threadIdx.x;
     bool act = p < n;
     const int id = act ? sp[p] : 0;
@@ -1329,13 +1380,32 @@ __global__ __launch_bounds__(TBS) void k3_keys(int n, GcRep R0, const GcRep* __r
 // Skin mode, a batch rebuild: the rebuilding replicas' cube counts cleared
 // before their key pass (the others' stay as their last rebuild left them).
 __global__ __launch_bounds__(BS) void k3_clear_counts(int K, const int* __restrict__ gate,
-    const int* __restrict__ rbflag, const int* __restrict__ cubebase, int nrep, int* __restrict__ cnt)
+    const int* __restrict__ rbflag, const int* __restrict__ cubebase, int nrep,
+    const unsigned char* __restrict__ cact, int* __restrict__ cnt)
 {
     if (*gate == 0) return;
-    for_groups((K + BS - 1) / BS, [&](int lb) {
-    if (range_idle(cubebase, nrep, rbflag, lb * BS, min(lb * BS + BS, K) - 1)) return;
+    for_groups_of<BS>((K + BS - 1) / BS, cact, [&](int lb) {
     const int k = lb * BS + threadIdx.x;
     if (k < K && rbflag[base_rep(cubebase, nrep, k)]) cnt[k] = 0;
+    });
+}
+
+// Skin mode, a batch rebuild: the activity bytes of the 64-slot chunks
+// (entries [0, nsc)) and of the 64-cube chunks (then ncc) from the replicas
+// each chunk spans (chunkrep: first and last) and their rebuild flags.
+__global__ __launch_bounds__(BS) void k3_mark(int nsc, int ncc, const int* __restrict__ gate,
+    const int* __restrict__ rbflag, const int2* __restrict__ chunkrep, unsigned char* __restrict__ sact,
+    unsigned char* __restrict__ cact)
+{
+    if (*gate == 0) return;
+    for_groups((nsc + ncc + BS - 1) / BS, [&](int lb) {
+    const int c = lb * BS + threadIdx.x;
+    if (c >= nsc + ncc) return;
+    const int2 rr = chunkrep[c];
+    int a = 0;
+    for (int r = rr.x; r <= rr.y; ++r) a |= rbflag[r];
+    if (c < nsc) sact[c] = a != 0;
+    else cact[c - nsc] = a != 0;
     });
 }
 
@@ -1351,11 +1421,10 @@ __global__ __launch_bounds__(BS) void k3_counts(int K, const int* __restrict__ o
 __global__ __launch_bounds__(BS) void k3_copyback(int n, const int* __restrict__ gate,
     const double* __restrict__ pos1, double* __restrict__ pos0, const tint3* __restrict__ nt1,
     tint3* __restrict__ nt0, const int* __restrict__ sp1, int* __restrict__ sp0,
-    const int* __restrict__ rep_of, const int* __restrict__ rbflag, const int* __restrict__ slotbase, int nrep)
+    const int* __restrict__ rep_of, const int* __restrict__ rbflag, const unsigned char* __restrict__ gact)
 {
     if (*gate == 0) return;
-    for_groups((n + BS - 1) / BS, [&](int lb) {
-    if (rbflag && range_idle(slotbase, nrep, rbflag, lb * BS, min(lb * BS + BS, n) - 1)) return;
+    for_groups_of<BS>((n + BS - 1) / BS, gact, [&](int lb) {
     const int p = lb * BS + threadIdx.x;
     if (p >= n) return;
     if (rbflag && !rbflag[rep_of[sp0[p]]]) return;   // a batch: the rebuilt replicas' slots only
@@ -1589,14 +1658,14 @@ int launch_build_t(gc_engine* e, int move, Prof* pr = nullptr)
     GC_LAUNCH_CHECK("engine cube offsets");
     mark(pr, s, "cube offsets");
     k2_scatter<<<gb, BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[c], e->d_offs, e->d_tk, e->d_tv,
-                                 e->d_to, e->d_cnt, nullptr, nullptr, nullptr, nullptr, 0);
+                                 e->d_to, e->d_cnt, nullptr, nullptr, nullptr, nullptr);
     GC_LAUNCH_CHECK("engine scatter");
     k2_sortgather<ENS, false><<<gb, BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, move, mfree, e->moving_walls,
         e->d_tk, e->d_tv,
         e->d_to, e->d_offs, e->d_curpos, e->d_pos64[c], e->d_pos64[nx], e->d_vout, e->d_force, e->d_nt[c],
         e->d_nt[nx], e->d_sp[nx], e->d_key, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf,
         e->moving_walls ? nullptr : e->d_wincl, e->d_act, e->d_fault, nullptr, nullptr, nullptr, nullptr,
-        nullptr, 0);
+        nullptr);
     GC_LAUNCH_CHECK("engine sort/gather");
     e->cur = nx;
     return GC_OK;
@@ -1623,7 +1692,7 @@ void launch_agg(gc_engine* e)
 }
 
 template <bool ENS>
-int skin_lists_t(gc_engine* e, const int* gate, const int* rbflag);
+int skin_lists_t(gc_engine* e, const int* gate, const int* rbflag, const unsigned char* gact);
 
 // Skin mode: re-sort and list build of the replicas whose flag (rbflag)
 // is set, all kernels returning at once unless *gate is set (decided on the
@@ -1636,11 +1705,16 @@ int skin_rebuild_t(gc_engine* e, const int* gate, const int* rbflag)
     const int gb = xcd_grid(blocks_for(N, BS));
     const GcRep& R0 = e->hreps[0];
     const int mfree = e->ms == nullptr;
-    if (ENS)   // the counts of the other replicas persist
+    const unsigned char* gact = ENS ? e->d_gact : nullptr;   // activity of the slot chunks
+    if (ENS) {   // the chunks' activity; the counts of the other replicas persist
+        const int nsc = chunks64(N), ncc = chunks64(e->K);
+        k3_mark<<<gated_grid(blocks_for(nsc + ncc, BS)), BS, 0, s>>>(nsc, ncc, gate, rbflag, e->d_chunkrep,
+                                                                    e->d_gact, e->d_gact + act_bytes(N));
         k3_clear_counts<<<gated_grid(blocks_for(e->K, BS)), BS, 0, s>>>(e->K, gate, rbflag, e->d_cubebase, e->nrep,
-                                                                        e->d_cnt);
+                                                                        e->d_gact + act_bytes(N), e->d_cnt);
+    }
     k3_keys<ENS, 256><<<gated_grid(blocks_for(N, 256)), 256, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, gate, rbflag,
-        e->d_slotbase, e->nrep, e->d_pos64[0], e->d_sp[0], e->d_key0, e->d_rank, e->d_cnt);
+        gact, e->d_pos64[0], e->d_sp[0], e->d_key0, e->d_rank, e->d_cnt);
     k2_tile_sums<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, gate);
     k2_cube_offsets<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, e->d_offs, e->d_ctr, 0,
                                                       e->d_nlover, gate);
@@ -1648,23 +1722,23 @@ int skin_rebuild_t(gc_engine* e, const int* gate, const int* rbflag)
     const int* rbf = ENS ? rbflag : nullptr;
     k2_scatter<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[0], e->d_offs,
                                                             e->d_tk, e->d_tv, e->d_to, e->d_cnt, gate,
-                                                            e->d_rep_of, rbf, e->d_slotbase, e->nrep);
+                                                            e->d_rep_of, rbf, gact);
     k2_sortgather<ENS, true><<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, 0, mfree,
         e->moving_walls,
         e->d_tk, e->d_tv, e->d_to, e->d_offs, e->d_curpos, e->d_pos64[0], e->d_pos64[1], e->d_vout,
         e->d_force, e->d_nt[0], e->d_nt[1], e->d_sp[1], e->d_key, e->d_posid, e->d_coor, e->d_velrho,
         e->d_velhalf, e->moving_walls ? nullptr : e->d_wincl, e->d_act, e->d_fault, gate, rbflag, e->d_woff,
-        e->d_sp[0], e->d_slotbase, e->nrep);
+        e->d_sp[0], gact);
     k3_copyback<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, gate, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
                                                  e->d_nt[0], e->d_sp[1], e->d_sp[0], e->d_rep_of, rbf,
-                                                 e->d_slotbase, e->nrep);
+                                                 gact);
     GC_LAUNCH_CHECK("engine skin re-sort");
-    return skin_lists_t<ENS>(e, gate, rbflag);
+    return skin_lists_t<ENS>(e, gate, rbflag, gact);
 }
 
 // Skin mode: the Verlet lists of the replicas whose flag is set (gated).
 template <bool ENS>
-int skin_lists_t(gc_engine* e, const int* gate, const int* rbflag)
+int skin_lists_t(gc_engine* e, const int* gate, const int* rbflag, const unsigned char* gact)
 {
     const int N = e->N;
     hipStream_t s = e->s;
@@ -1676,7 +1750,7 @@ int skin_lists_t(gc_engine* e, const int* gate, const int* rbflag)
                                  : xcd_grid(blocks_for(N, tbs));
             kern<<<grid, tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid,
                 e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc, e->d_nlover, gate, rbflag, e->d_rbpos,
-                e->d_slotbase, e->nrep);
+                gact);
         };
         switch (e->tune.v[TUNE_DENSITY]) {   // block size x LDS row stage
             case 0: L(k2_density<ENS, 64, 0, true>, 64); break;
@@ -1707,7 +1781,7 @@ int skin_init(gc_engine* e)
     hipStream_t s = e->s;
     if (e->cur == 1) {
         k3_copyback<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, e->d_ones, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
-                                                     e->d_nt[0], e->d_sp[1], e->d_sp[0], nullptr, nullptr, nullptr, 0);
+                                                     e->d_nt[0], e->d_sp[1], e->d_sp[0], nullptr, nullptr, nullptr);
         GC_LAUNCH_CHECK("engine skin copy");
         e->cur = 0;
     }
@@ -1715,7 +1789,8 @@ int skin_init(gc_engine* e)
         k3_counts<<<blocks_for(e->K, BS), BS, 0, s>>>(e->K, e->d_offs, e->d_cnt);
         GC_LAUNCH_CHECK("engine skin counts");
     }
-    int st = e->ens() ? skin_lists_t<true>(e, e->d_ones, e->d_ones) : skin_lists_t<false>(e, e->d_ones, e->d_ones);
+    int st = e->ens() ? skin_lists_t<true>(e, e->d_ones, e->d_ones, nullptr)
+                      : skin_lists_t<false>(e, e->d_ones, e->d_ones, nullptr);
     if (st) return st;
     const int nr = e->nrep;
     if ((st = hip_check(hipMemsetAsync(e->d_woff, 0, (size_t)N, s), "skin init")) ||
@@ -1799,13 +1874,14 @@ int skin_fluid_t(gc_engine* e, int driveon, Prof* pr)
     const int N = e->N;
     hipStream_t s = e->s;
     const int mfree = e->ms == nullptr;
-    int nb_adv = 0;
+    int nb_adv = 0, adv_bs = 0;
     {
         auto L = [&](auto kern, int tbs) {
             kern<<<xcd_grid(blocks_for(N, tbs)), tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, mfree,
                 e->moving_walls, e->d_pos64[0], e->d_vout, e->d_force, e->d_sp[0], e->d_nt[0], e->d_woff,
                 e->d_rbpos, e->d_posid, e->d_coor, e->d_velrho, e->d_velhalf, e->d_bmax, e->d_brep, e->d_disp2);
             nb_adv = blocks_for(N, tbs);
+            adv_bs = tbs;
         };
         switch (e->tune.v[TUNE_INTEGRATE]) {
             case 0: L(k3_advance<ENS, 64>, 64); break;
@@ -1816,11 +1892,11 @@ int skin_fluid_t(gc_engine* e, int driveon, Prof* pr)
     // the substeps of the previous step wrote their vertices' maxima after
     // the advance's region (membrane substepping: one suspension)
     const int nb_ms = e->ms ? blocks_for(e->nm, BS) : 0;
-    // one word set: few groups, each reducing a long run first; a batch: one
-    // entry per thread, so that the loads are not a chain of dependent ones
-    const int nb_red = e->ens() ? blocks_for(nb_adv + nb_ms, REDUCE_BS) : REDUCE_NB;
-    k3_reduce<<<nb_red, REDUCE_BS, 0, s>>>(e->nrep, e->d_bmax, e->d_brep, nb_adv, skin_ms_off(e), nb_ms,
-                                               e->d_disp2);
+    if (e->ens() && nb_ms == 0)
+        k3_reduce_rep<<<e->nrep, REDUCE_BS, 0, s>>>(e->d_bmax, e->d_brep, e->d_slotbase, adv_bs, e->d_disp2);
+    else
+        k3_reduce<<<REDUCE_NB, REDUCE_BS, 0, s>>>(e->nrep, e->d_bmax, e->d_brep, nb_adv, skin_ms_off(e), nb_ms,
+                                                  e->d_disp2);
     k3_decide<<<1, 64, 0, s>>>(e->nrep, e->d_reps, e->d_disp2, e->d_rbflag, e->d_rbany, e->d_nrebuild, e->d_ctr);
     GC_LAUNCH_CHECK("engine advance");
     mark(pr, s, "advance+predict");
@@ -1875,7 +1951,7 @@ int launch_step_t(gc_engine* e, int driveon, Prof* pr)
         auto L = [&](auto kern, int tbs) {
             kern<<<xcd_grid(blocks_for(N, tbs)), tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid,
                 e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc, e->d_nlover, nullptr, nullptr, nullptr,
-                nullptr, 0);
+                nullptr);
         };
         switch (e->tune.v[TUNE_DENSITY]) {   // block size x LDS row stage
             case 0: L(k2_density<ENS, 64, 0>, 64); break;
@@ -2263,9 +2339,23 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
                 sb[r + 1] = sb[r] + m.nb1 + m.nb2 + m.nfl + m.nm;
                 cb[r] = e->hreps[r].cube_off;
             }
-            cb[nr] = K;
-            if (sb[nr] != N) return fail(set_error(GC_ERR_INVALID, "EngineCreate: replica sizes"));
+            cb[nr] = (int)K;
+            if (sb[nr] != (int)N) return fail(set_error(GC_ERR_INVALID, "EngineCreate: replica sizes"));
+            // first and last replica of every 64-slot chunk, then of every 64-cube chunk
+            const int nsc = chunks64((int)N), ncc = chunks64((int)K);
+            std::vector<int2> cr((size_t)nsc + ncc);
+            auto rep_at = [](const std::vector<int>& b, int x) {
+                return (int)(std::upper_bound(b.begin(), b.end(), x) - b.begin()) - 1;
+            };
+            for (int c = 0; c < nsc; ++c)
+                cr[c] = make_int2(rep_at(sb, 64 * c), rep_at(sb, std::min(64 * c + 63, (int)N - 1)));
+            for (int c = 0; c < ncc; ++c)
+                cr[(size_t)nsc + c] = make_int2(rep_at(cb, 64 * c), rep_at(cb, std::min(64 * c + 63, (int)K - 1)));
+            const size_t nact = act_bytes((int)N) + act_bytes((int)K);
             if ((st = e->alloc(&e->d_slotbase, nr + 1)) || (st = e->alloc(&e->d_cubebase, nr + 1)) ||
+                (st = e->alloc(&e->d_chunkrep, cr.size())) || (st = e->alloc(&e->d_gact, nact)) ||
+                (st = hip_check(hipMemset(e->d_gact, 0, nact), "engine chunk activity")) ||
+                (st = upload(e->d_chunkrep, cr.data(), cr.size(), e->s)) ||
                 (st = upload(e->d_slotbase, sb.data(), nr + 1, e->s)) ||
                 (st = upload(e->d_cubebase, cb.data(), nr + 1, e->s)))
                 return fail(st);
@@ -2319,7 +2409,7 @@ gc_engine* create(const gc_case_desc* descs, int nrep, int* status)
     if ((st = launch_build(e, 0))) return fail(st);
     if (e->skin_mode() && e->cur == 1) {   // buffer 0 stays current in skin mode
         k3_copyback<<<gated_grid(blocks_for(e->N, BS)), BS, 0, s>>>(e->N, e->d_ones, e->d_pos64[1], e->d_pos64[0],
-            e->d_nt[1], e->d_nt[0], e->d_sp[1], e->d_sp[0], nullptr, nullptr, nullptr, 0);
+            e->d_nt[1], e->d_nt[0], e->d_sp[1], e->d_sp[0], nullptr, nullptr, nullptr);
         if ((st = hip_check(hipGetLastError(), "engine skin copy"))) return fail(st);
         e->cur = 0;
     }
@@ -2752,7 +2842,7 @@ int restore_state(gc_engine* e, const std::vector<double>& pos, const std::vecto
     if ((st = launch_build(e, 0))) return st;
     if (e->skin_mode() && e->cur == 1) {   // buffer 0 stays current in skin mode
         k3_copyback<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, e->d_ones, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
-                                                     e->d_nt[0], e->d_sp[1], e->d_sp[0], nullptr, nullptr, nullptr, 0);
+                                                     e->d_nt[0], e->d_sp[1], e->d_sp[0], nullptr, nullptr, nullptr);
         if ((st = hip_check(hipGetLastError(), "engine skin copy"))) return st;
         e->cur = 0;
     }

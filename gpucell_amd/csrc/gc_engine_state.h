@@ -100,6 +100,8 @@ struct gc_engine {
     // the last = N or K); a replica's slot range is fixed (its particle
     // count is), so a rebuild re-sorts the rebuilding replicas only
     int *d_slotbase = nullptr, *d_cubebase = nullptr;
+    int2* d_chunkrep = nullptr;          // per 64-slot chunk, then 64-cube chunk: first, last replica
+    unsigned char* d_gact = nullptr;     // per chunk, set each rebuild: 1 if a rebuilding replica owns part
     bool skin_mode() const { return skin > 0.0f; }
     unsigned long long* d_ctr;
     int* d_fault = nullptr;    // GC_FAULT_* bits raised by the kernels (sticky)
