@@ -262,10 +262,12 @@ typedef struct {
      * counterpart).  0: the reference's per-step scheme (re-sort and cube
      * scan every step; cube side CutRadius).  > 0: cube side CutRadius +
      * skin, lists of every pair within CutRadius + skin rebuilt (with the
-     * re-sort) only when the summed largest displacement since the last
-     * rebuild exceeds skin / 2, decided on the device; the density and
-     * force passes iterate the lists.  Every replica of an ensemble must
-     * use the same mode (skin 0 for all, or > 0 for all). */
+     * re-sort) only when some pair may have moved apart by more than skin
+     * since the last rebuild (a displacement beyond skin / 2, and a
+     * displacement bounding box wider than skin), decided on the device per
+     * replica; the density and force passes iterate the lists.  Every
+     * replica of an ensemble must use the same mode (skin 0 for all, or > 0
+     * for all). */
     float skin;
 } gc_case_desc;
 
