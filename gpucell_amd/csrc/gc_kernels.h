@@ -21,6 +21,11 @@ struct GcRep {
     float bbox[6];
     float invw[3];
     int cube_off;
+    // cube order: x fastest within a (y, z) row; the rows are grouped into
+    // tiles of 2^tys consecutive y (z, then y inside a tile; tiles in y), so
+    // that the rows above and below a cube lie ~2^tys rows away in slot
+    // order instead of ny (2^tys >= ny: the plain x, y, z order)
+    int tys;
     int id_off[4];      // first global id of the replica's Bound1, Bound2, fluid, membrane
     int loc_start[4];   // the same blocks' first ids in the replica's own numbering
     // Verlet skin mode (skin > 0): neighbour lists hold every pair within
@@ -30,6 +35,29 @@ struct GcRep {
     float list_r2;
     float half_skin;
 };
+
+// Index of row (y, z) of a replica's cubes (the key is cube_off + x +
+// nx * row): tile t0 = the tile's first y, th its height (the last tile
+// may be shorter).
+__host__ __device__ inline int cube_row(const GcRep& R, int y, int z)
+{
+    const int t0 = (y >> R.tys) << R.tys;
+    const int th = (R.nc[1] - t0) < (1 << R.tys) ? (R.nc[1] - t0) : (1 << R.tys);
+    return t0 * R.nc[2] + z * th + (y - t0);
+}
+
+// Cube coordinates of key k (global key, cube_off included).
+__host__ __device__ inline void cube_coords(const GcRep& R, int k, int& cx, int& cy, int& cz)
+{
+    k -= R.cube_off;
+    cx = k % R.nc[0];
+    const int r = k / R.nc[0];
+    const int t0 = (r / (R.nc[2] << R.tys)) << R.tys;
+    const int rem = r - t0 * R.nc[2];
+    const int th = (R.nc[1] - t0) < (1 << R.tys) ? (R.nc[1] - t0) : (1 << R.tys);
+    cz = rem / th;
+    cy = t0 + (rem - cz * th);
+}
 
 // the replica's own id of global id `id` (used for the pair noise, so a
 // replica draws the same numbers as when it runs alone).  Selects on

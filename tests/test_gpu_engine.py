@@ -65,20 +65,41 @@ def couette_case():
     return casegen.couette(n=16, n_rbc=1, rbc_diameter=7.0, shear_rate=0.01, seed=13, physics=ph)
 
 
+def long_channel_case():
+    """A 480-long duct (14 wide): its cube grid is long enough in x that the engine
+    groups its rows of cubes into tiles (at most 4096 cubes each), in both
+    the per-step and the skin mode's cubes, so its slot order is not the
+    reference's cube order."""
+    ph = util.replace_physics(kT=0.01, fext=0.05)
+    return casegen.channel(480, 14, 6, rbc_diameter=5.0, subdiv=2, seed=9, physics=ph)
+
+
+def tiled(ncube) -> bool:
+    """Whether a grid of ncube cubes is tiled: 2^s rows of nx cubes hold at
+    most 4096 cubes, with s as large as that allows, and fewer than ny."""
+    s = 0
+    while s < 20 and (2 << s) * int(ncube[0]) <= 4096:
+        s += 1
+    return (1 << s) < int(ncube[1])
+
+
 CASES = {"channel": lambda: channel_case(), "channel_morse": morse_case, "dense_morse": dense_morse_case,
          "box": box_case, "channel_bound1": bound1_case, "box_big_cell": big_cell_case,
-         "couette": couette_case}
+         "couette": couette_case, "long_channel": long_channel_case}
 # cases whose particles exceed the 64-entry neighbour list (the force and
 # Morse passes then scan the cubes)
 OVERFLOW = ("dense_morse", "box_big_cell")
 
 
-@pytest.mark.parametrize("name", ["channel", "box"])
+@pytest.mark.parametrize("name", ["channel", "box", "long_channel"])
 def test_engine_cells_bit_exact(gpu, name):
+    """The exported cell list is the reference's, also where the engine's
+    own cube order is tiled (long_channel)."""
     case = CASES[name]()
     e = Engine(case)
     e.step(3)
     st, cl, info = e.state(), e.cells(), e.info()
+    assert tiled(info.ncube) == (name == "long_channel")
     nc = np.array(info.ncube)
     inv = np.array(info.invwid, np.float32)
     cc_o, key_o, _ = orc.preconlist(st["coor"], np.float32(case.consts.CutRadius),
@@ -106,7 +127,7 @@ def _oracle_state(o, case, S, info):
 
 
 @pytest.mark.parametrize("name", ["channel", "channel_morse", "dense_morse", "box", "channel_bound1",
-                                  "box_big_cell", "couette"])
+                                  "box_big_cell", "couette", "long_channel"])
 def test_engine_step_parity(gpu, name):
     case = CASES[name]()
     e = Engine(case)

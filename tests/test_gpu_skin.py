@@ -54,6 +54,9 @@ def test_skin_step_parity(gpu, name, skin):
     print(f"{name} skin {skin}: max per-particle force error {emax:.3e}, relative L2 {l2:.3e}, "
           f"rebuilds {i1.nrebuild}, list overflows {i1.nl_overflow}")
     assert abs(i1.skin - skin) < 1e-7
+    if name == "long_channel":   # the skin mode's wider cubes are tiled too
+        from tests.test_gpu_engine import tiled
+        assert tiled(i1.ncube)
     if skin == 0.02:
         assert i1.nrebuild >= 1
     assert i1.fault == 0
