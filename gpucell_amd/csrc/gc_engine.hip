@@ -864,9 +864,8 @@ __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep*
     key_to_cc(key[p], R, cx, cy, cz);
     const float ih = 1.0f / P.h;
     float fx = 0.f, fy = 0.f, fz = 0.f;
-    auto pair = [&](int code) {
+    auto pair = [&](int code, const float4 pj, const float4 vj) {
         int j = code >> 6;
-        float4 pj = posid[j];
         const int oj = VL ? woff_of(pj.w) : 0;
         float rx, ry, rz;
         if (__ballot((code & 63) | oj | oi)) {   // an image neighbour in the wave (x + 0 is x: same bits)
@@ -893,7 +892,6 @@ __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep*
             fx += kk * rx; fy += kk * ry; fz += kk * rz;
             return;
         }
-        float4 vj = velrho[j];
         float dj = vj.w * P.inv_m;
         float rdj = __builtin_amdgcn_rcpf(dj);
         float cC = (pi_d2 + (idj < P.nc ? P.c1sq : P.c2sq) * P.m * rdj) * F;
@@ -936,13 +934,22 @@ __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep*
             m &= m - 1u;
             return 32 * w + b;
         };
+        // the next pair's entry, position and velocity are loaded during the
+        // current pair (10M: 1,248 -> 1,205 us; also loading the entry after
+        // next was slower, 1,232)
         int k = next_k();
-        int code = k >= 0 ? wl[64 * k] : 0;   // the next entry is loaded during the current pair
+        int code = k >= 0 ? wl[64 * k] : 0;
+        float4 pjn = posid[code >> 6], vjn = velrho[code >> 6];
         while (k >= 0) {
             const int cur = code;
+            const float4 pj = pjn, vj = vjn;
             k = next_k();
-            if (k >= 0) code = wl[64 * k];
-            pair(cur);
+            if (k >= 0) {
+                code = wl[64 * k];
+                pjn = posid[code >> 6];
+                vjn = velrho[code >> 6];
+            }
+            pair(cur, pj, vj);
         }
     } else if (cnt >= 0) {
         const int* wl = nl + (size_t)(p >> 6) * (NL_CAP * 64) + (p & 63);   // the density lane's block
@@ -950,7 +957,7 @@ __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep*
         for (int k = 0; k < cnt; ++k) {
             const int code = next;
             if (k + 1 < cnt) next = wl[64 * (k + 1)];
-            pair(code);
+            pair(code, posid[code >> 6], velrho[code >> 6]);
         }
     } else {  // more than the list holds: scan the cubes again
         visit2(R, offs, cx, cy, cz, [&](int jb, int je, float sx, float sy, float sz) {
@@ -964,7 +971,7 @@ __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep*
                 }
                 float rx = pdiff(pi.x, pj.x, sx), ry = pdiff(pi.y, pj.y, sy), rz = pdiff(pi.z, pj.z, sz);
                 float r2 = rx * rx + ry * ry + rz * rz;
-                if (r2 < P.h2 && j != p) pair((j << 6) | code);
+                if (r2 < P.h2 && j != p) pair((j << 6) | code, pj, velrho[j]);
             }
         });
     }
@@ -1452,7 +1459,7 @@ __global__ __launch_bounds__(BS) void k3_copyback(int n, const int* __restrict__
 // [7,9] skin mode, every step: number density of moving particles and
 // velocity of Bound2 walls over their Verlet lists.  The wave runs through
 // the entries in lockstep (entry k of every lane is one coalesced row),
-// four entries' codes and then their positions loaded before use; r < h
+// eight entries' codes and then their positions loaded before use; r < h
 // selects this step's neighbours, whose bits go into one row per 32 entries
 // after the entries (read by the force pass).  The sums run in list order.
 // A particle whose list overflowed scans its cubes (the rebuild's cubes
@@ -1491,27 +1498,29 @@ __global__ __launch_bounds__(TBS) void k3_density(int n, GcRep R0, const GcRep* 
         }
     };
     const int kmax = wave_max(cnt);
-    // the codes of the next four entries are loaded while the current four
-    // entries' positions are in flight
-    int cn[4];
+    // the codes of the next U entries are loaded while the current U
+    // entries' positions are in flight (U = 8: 10M step 586 -> 504 us
+    // against U = 4; U = 2 and U = 16 were slower)
+    constexpr int U = 8;   // divides the 32 entries of a mask row
+    int cn[U];
     auto load_codes = [&](int k) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) cn[u] = cnt > 0 ? wl[64 * min(k + u, cnt - 1)] : 0;   // in bounds
+        for (int u = 0; u < U; ++u) cn[u] = cnt > 0 ? wl[64 * min(k + u, cnt - 1)] : 0;   // in bounds
     };
     if (kmax > 0) load_codes(0);
     for (int k0 = 0; k0 < kmax; k0 += 32) {
         unsigned m = 0u;
         const int k1 = min(k0 + 32, kmax);
-        for (int k = k0; k < k1; k += 4) {
-            int c[4];
-            float4 q[4];
+        for (int k = k0; k < k1; k += U) {
+            int c[U];
+            float4 q[U];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) c[u] = cn[u];
+            for (int u = 0; u < U; ++u) c[u] = cn[u];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) q[u] = posid[c[u] >> 6];
-            if (k + 4 < kmax) load_codes(k + 4);
+            for (int u = 0; u < U; ++u) q[u] = posid[c[u] >> 6];
+            if (k + U < kmax) load_codes(k + U);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 const bool ok = k + u < cnt;
                 const int oj = woff_of(q[u].w);
                 float rx, ry, rz;
