@@ -70,6 +70,10 @@ inline int nl_stride(const gc_engine* e) { return (e->skin_mode() ? VL_ROWS : NL
 // logical groups; an empty launch then dispatches at most this many)
 constexpr int GATED_MAX_GROUPS = 2048;
 inline int gated_grid(int nb) { return std::min(xcd_grid(nb), GATED_MAX_GROUPS); }
+// A batch rebuilds a few replicas per step: half the grid (64 sheared
+// replicas: rebuild 252 -> 239 us per step; a full 10M rebuild is faster
+// with the whole grid, 126 against 130 us per step amortised).
+inline int gated_grid(int nb, bool batch) { return std::min(xcd_grid(nb), batch ? GATED_MAX_GROUPS / 2 : GATED_MAX_GROUPS); }
 // Cube order: rows of nx cubes grouped into tiles of 2^tys y-rows holding
 // at most CUBE_TILE_CUBES cubes, so that a cube's neighbours in the rows at
 // z - 1 and z + 1 are at most that many cubes away in slot order instead of
@@ -1732,28 +1736,28 @@ int skin_rebuild_t(gc_engine* e, const int* gate, const int* rbflag)
     const unsigned char* gact = ENS ? e->d_gact : nullptr;   // activity of the slot chunks
     if (ENS) {   // the chunks' activity; the counts of the other replicas persist
         const int nsc = chunks64(N), ncc = chunks64(e->K);
-        k3_mark<<<gated_grid(blocks_for(nsc + ncc, BS)), BS, 0, s>>>(nsc, ncc, gate, rbflag, e->d_chunkrep,
+        k3_mark<<<gated_grid(blocks_for(nsc + ncc, BS), ENS), BS, 0, s>>>(nsc, ncc, gate, rbflag, e->d_chunkrep,
                                                                     e->d_gact, e->d_gact + act_bytes(N));
-        k3_clear_counts<<<gated_grid(blocks_for(e->K, BS)), BS, 0, s>>>(e->K, gate, rbflag, e->d_cubebase, e->nrep,
+        k3_clear_counts<<<gated_grid(blocks_for(e->K, BS), ENS), BS, 0, s>>>(e->K, gate, rbflag, e->d_cubebase, e->nrep,
                                                                         e->d_gact + act_bytes(N), e->d_cnt);
     }
-    k3_keys<ENS, 256><<<gated_grid(blocks_for(N, 256)), 256, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, gate, rbflag,
+    k3_keys<ENS, 256><<<gated_grid(blocks_for(N, 256), ENS), 256, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, gate, rbflag,
         gact, e->d_pos64[0], e->d_sp[0], e->d_key0, e->d_rank, e->d_cnt);
     k2_tile_sums<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, gate);
     k2_cube_offsets<<<e->ntile, CUBE_SCAN_BS, 0, s>>>(e->K, e->d_cnt, e->d_tsum, e->d_offs, e->d_ctr, 0,
                                                       e->d_nlover, gate);
     // a batch re-sorts the rebuilding replicas only (the others keep their slots)
     const int* rbf = ENS ? rbflag : nullptr;
-    k2_scatter<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[0], e->d_offs,
+    k2_scatter<<<gated_grid(blocks_for(N, BS), ENS), BS, 0, s>>>(N, e->d_key0, e->d_rank, e->d_sp[0], e->d_offs,
                                                             e->d_tk, e->d_tv, e->d_to, e->d_cnt, gate,
                                                             e->d_rep_of, rbf, gact);
-    k2_sortgather<ENS, true><<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, 0, mfree,
+    k2_sortgather<ENS, true><<<gated_grid(blocks_for(N, BS), ENS), BS, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, 0, mfree,
         e->moving_walls,
         e->d_tk, e->d_tv, e->d_to, e->d_offs, e->d_curpos, e->d_pos64[0], e->d_pos64[1], e->d_vout,
         e->d_force, e->d_nt[0], e->d_nt[1], e->d_sp[1], e->d_key, e->d_posid, e->d_coor, e->d_velrho,
         e->d_velhalf, e->moving_walls ? nullptr : e->d_wincl, e->d_act, e->d_fault, gate, rbflag, e->d_woff,
         e->d_sp[0], gact);
-    k3_copyback<<<gated_grid(blocks_for(N, BS)), BS, 0, s>>>(N, gate, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
+    k3_copyback<<<gated_grid(blocks_for(N, BS), ENS), BS, 0, s>>>(N, gate, e->d_pos64[1], e->d_pos64[0], e->d_nt[1],
                                                  e->d_nt[0], e->d_sp[1], e->d_sp[0], e->d_rep_of, rbf,
                                                  gact);
     GC_LAUNCH_CHECK("engine skin re-sort");
@@ -1769,8 +1773,10 @@ int skin_lists_t(gc_engine* e, const int* gate, const int* rbflag, const unsigne
     const GcRep& R0 = e->hreps[0];
     {
         auto L = [&](auto kern, int tbs) {
-            // a batch: a fixed grid (2048 groups' worth of 256 lanes) striding over the groups
-            const int grid = ENS ? std::min(xcd_grid(blocks_for(N, tbs)), GATED_MAX_GROUPS * (256 / tbs))
+            // a batch: a fixed grid (1024 groups' worth of 256 lanes) striding over the
+            // groups (64 sheared replicas: list rebuild 283 us per step with 2048,
+            // 253 with 1024, 308 with 512, 311 and 361 with 4096 and 8192)
+            const int grid = ENS ? std::min(xcd_grid(blocks_for(N, tbs)), GATED_MAX_GROUPS / 2 * (256 / tbs))
                                  : xcd_grid(blocks_for(N, tbs));
             kern<<<grid, tbs, 0, s>>>(N, R0, e->d_reps, e->d_rep_of, e->d_posid,
                 e->d_velrho, e->d_offs, e->d_key, e->d_nl, e->d_nlc, e->d_nlover, gate, rbflag, e->d_rbpos,
