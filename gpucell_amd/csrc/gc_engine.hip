@@ -1094,13 +1094,15 @@ __global__ __launch_bounds__(BS) void k2_agg(int nm, GcRep R0, const GcRep* __re
 
 // ---------------------------------------------------------------------------
 // Skin mode (Verlet lists).  Between rebuilds the slots, cube keys, CurPos
-// and the moving-slot map stay as the last rebuild left them, positions are
-// advanced in place and not wrapped (they stay continuous in that rebuild's
-// image frame, so the image codes of the lists stay valid; they are wrapped
-// at the next rebuild and on export), and the lists hold every pair within
-// h + skin at the rebuild.  Every particle then moved at most the summed
-// largest displacement since, so while that sum is <= skin / 2 every pair
-// within h is in the lists.
+// and the moving-slot map stay as the last rebuild left them, and the lists
+// hold every pair within h + skin at the rebuild.  Positions are advanced
+// and wrapped in place every step; each particle carries its wraps since the
+// rebuild (the wrap code in the id word of posid), so a list entry's image
+// shift S becomes S + (o_j - o_i) L, the shift of the wrapped pair.  A pair
+// not in the lists was more than h + skin apart, so it cannot be within h
+// while its relative displacement since the rebuild is at most skin: the
+// rebuild test bounds that by every |D_i| <= skin / 2 or by the bounding box
+// of all D_i (a common translation moves no pair).
 
 __device__ __forceinline__ float wave_maxf(float v)
 {

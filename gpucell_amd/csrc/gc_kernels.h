@@ -30,8 +30,9 @@ struct GcRep {
     int loc_start[4];   // the same blocks' first ids in the replica's own numbering
     // Verlet skin mode (skin > 0): neighbour lists hold every pair within
     // h + skin at the last rebuild (list_r2 = (h + skin)^2); a rebuild is
-    // due once the sum over steps of the largest displacement exceeds
-    // half_skin (skin / 2 less a rounding margin).  0 in per-step mode.
+    // due once some displacement since the rebuild exceeds half_skin (skin
+    // / 2 less a rounding margin) and the displacements' bounding box is
+    // wider than twice that.  0 in per-step mode.
     float list_r2;
     float half_skin;
 };
