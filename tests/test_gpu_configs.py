@@ -116,6 +116,29 @@ def test_suspension_1m_skin(gpu):
     _run(casegen.suspension_1m(), "1M suspension", warm=14, checked=2, skin=0.2)
 
 
+def test_suspension_1m_skin_long_run(gpu):
+    """3000 steps in the skin mode (past the start-up transient, with
+    rebuilds every ~10 steps first and further apart later), graph replay:
+    the state stays finite, no list overflows, and the step after is still
+    the FP64 oracle's."""
+    case = casegen.suspension_1m()
+    e = Engine(case, skin=0.2)
+    e.step(3000, graph=True)
+    info = e.info()
+    print(f"1M suspension, skin 0.2: {info.nrebuild} list rebuilds in {info.step} steps, "
+          f"{info.nl_overflow} list overflows", flush=True)
+    assert info.fault == 0 and info.nl_overflow == 0
+    assert 3000 // 40 <= info.nrebuild <= 3000 // 4
+    o = orc.Oracle(case)
+    o.initial_state()
+    S = e.state()
+    assert np.isfinite(S["pos"]).all() and np.isfinite(S["velrhop"]).all()
+    e.step(1)
+    emax, l2, _ = gu.step_parity(o, case, S, e.state(), info, label="1M suspension step 3001")
+    print(f"1M suspension step 3001: max per-particle force error {emax:.3e}, relative L2 {l2:.3e}")
+    e.close()
+
+
 @pytest.mark.timeout(600)
 def test_stenosis_morse_skin(gpu):
     case = casegen.stenosis_2m()
