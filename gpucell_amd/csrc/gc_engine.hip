@@ -74,25 +74,6 @@ inline int gated_grid(int nb) { return std::min(xcd_grid(nb), GATED_MAX_GROUPS);
 // replicas: rebuild 252 -> 239 us per step; a full 10M rebuild is faster
 // with the whole grid, 126 against 130 us per step amortised).
 inline int gated_grid(int nb, bool batch) { return std::min(xcd_grid(nb), batch ? GATED_MAX_GROUPS / 2 : GATED_MAX_GROUPS); }
-// Cube order: rows of nx cubes grouped into tiles of 2^tys y-rows holding
-// at most CUBE_TILE_CUBES cubes, so that a cube's neighbours in the rows at
-// z - 1 and z + 1 are at most that many cubes away in slot order instead of
-// nx ny, and are still in the L2 of the XCD that sweeps them when they are
-// read again (measured on the 10M-particle duct, 242 x 92 x 92 cubes, skin
-// mode: 2.65 ms/step untiled, 2.70 / 2.63 / 2.60 / 2.57 / 2.59 / 2.62 with
-// tiles of 2, 4, 8, 16, 32, 64 rows).  A grid whose whole y range fits one
-// tile keeps the plain order.
-constexpr int CUBE_TILE_CUBES = 4096;
-inline int cube_tile_shift(int nx, int ny)
-{
-    int s = 0;
-    while (s < 20 && (2 << s) * nx <= CUBE_TILE_CUBES) ++s;   // 2^s rows of nx cubes <= CUBE_TILE_CUBES
-    if ((1 << s) >= ny) {   // one tile: 2^s >= ny
-        s = 0;
-        while ((1 << s) < ny) ++s;
-    }
-    return s;
-}
 // a batch's rebuild: activity bytes per 64-wide chunk of slots (then of
 // cubes), each table padded to whole 32-bit words
 inline int chunks64(int n) { return (n + 63) / 64; }
