@@ -850,7 +850,6 @@ __global__ __launch_bounds__(TBS) void k2_force(int nmov, GcRep R0, const GcRep*
     const float ih = 1.0f / P.h;
     float fx = 0.f, fy = 0.f, fz = 0.f;
     auto pair = [&](int code, const float4 pj, const float4 vj) {
-        int j = code >> 6;
         const int oj = VL ? woff_of(pj.w) : 0;
         float rx, ry, rz;
         if (__ballot((code & 63) | oj | oi)) {   // an image neighbour in the wave (x + 0 is x: same bits)
@@ -1713,7 +1712,6 @@ int skin_rebuild_t(gc_engine* e, const int* gate, const int* rbflag)
 {
     const int N = e->N;
     hipStream_t s = e->s;
-    const int gb = xcd_grid(blocks_for(N, BS));
     const GcRep& R0 = e->hreps[0];
     const int mfree = e->ms == nullptr;
     const unsigned char* gact = ENS ? e->d_gact : nullptr;   // activity of the slot chunks
