@@ -13,7 +13,7 @@ neighbour bits the density pass sets.
   summation order, noise off);
 * graph replay == eager launches, bit for bit, rebuilds included;
 * each replica of a batch == its solo run, bit for bit (each replica decides
-  its own rebuilds);
+  its own rebuilds), also with replicas only a few waves long;
 * membrane substepping under the skin mode (its displacements join the
   rebuild bound);
 * particles that start outside a periodic face (the first lists are built
@@ -130,6 +130,33 @@ def test_skin_ensemble_replicas_match_solo_runs(gpu, graph):
     info = ens.info()
     print(f"ensemble: {info.nrebuild} rebuilds over {len(cases)} replicas")
     assert info.nrebuild == total >= len(cases)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_skin_batch_of_small_replicas_matches_solo_runs(gpu, graph):
+    """Replicas of 216-512 particles (a few waves each) beside a larger one:
+    many 64-slot chunks and waves hold two replicas, one rebuilding in a
+    step and the other not (the batch re-sorts and rebuilds the lists of the
+    rebuilding ones only, lane by lane in such a wave), and each replica
+    still evolves bit for bit as its solo run."""
+    cases = [casegen.periodic_box(n=6, n_rbc=0, seed=21, physics=util.replace_physics(kT=0.02, seed=21)),
+             casegen.periodic_box(n=7, n_rbc=0, seed=22, physics=util.replace_physics(kT=0.005, seed=22)),
+             ensemble_cases()[0],
+             casegen.periodic_box(n=8, n_rbc=0, seed=23, physics=util.replace_physics(kT=0.03, seed=23)),
+             casegen.periodic_box(n=6, n_rbc=0, seed=24, physics=util.replace_physics(kT=0.01, seed=24))]
+    nsteps, skin = 24, 0.03
+    ens = Ensemble(cases, skin=skin)
+    ens.step(nsteps, graph=graph)
+    counts = []
+    for r, c in enumerate(cases):
+        solo = Engine(c, skin=skin)
+        solo.step(nsteps, graph=graph)
+        _same(ens.state(r), solo.state(), f"replica {r}")
+        counts.append(solo.info().nrebuild)
+        solo.close()
+    print(f"small replicas: rebuilds per replica {counts}")
+    assert ens.info().nrebuild == sum(counts)
+    assert min(counts) >= 1 and len(set(counts)) > 1   # the replicas rebuild at different steps
 
 
 @pytest.mark.parametrize("mode", [1])
