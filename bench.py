@@ -198,8 +198,9 @@ def main():
     a = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(one suspension per GPU; see the sharding design)", file=sys.stderr)
+        print("bench.py measures one GPU: the engine has no multi-GPU path "
+              "(a suspension runs on one GPU; across GPUs it would be sharded by slabs of cubes)",
+              file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

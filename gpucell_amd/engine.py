@@ -1,6 +1,6 @@
 """Python handle on the C++ engine (gc_Engine* in libgpucell_hip.so).
 
-Mirrors the reference's host driver (class Particle, src/Particle.cuh:8-167):
+Mirrors the reference's host class (class Particle, src/Particle.cuh:8-167):
 create = InitPar/SetPar, step = DrivePar + UpdatePar, state = the arrays
 ParticlesDataReturn copies back (un-sorted to id order).
 """

@@ -15,7 +15,7 @@ def pytest_configure(config):
 
 
 def _ensure_built():
-    """Build only what is missing (the driver builds before running tests)."""
+    """Build only what is missing (a fresh checkout is built before its tests run)."""
     lib = os.path.join(ROOT, "gpucell_amd", "libgpucell_hip.so")
     orc = os.path.join(ROOT, "oracle", "liboracle.so")
     if not os.path.exists(orc):

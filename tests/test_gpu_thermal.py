@@ -1,4 +1,4 @@
-"""Thermal run of the engine against the FP64 oracle's (north-star check of
+"""Thermal run of the engine against the FP64 oracle's (a check of the
 deformation, flow and aggregation statistics): the 32^3 reverse-Poiseuille
 box with eight cells at kT = 0.01 (tests.thermal_util), 2000 steps, 51
 samples from step 1000.  The oracle's samples are the fixture
