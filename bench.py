@@ -69,22 +69,29 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def duct(c):
+    """The duct's interior size from the case's box, and its size with the
+    three frozen wall layers on each walled face (unit lattice)."""
+    b = [float(v) for v in c.bbox]
+    L, H, W = b[1] - b[0], b[3] - b[2], b[5] - b[4]
+    return f"{L:.0f}x{H:.0f}x{W:.0f} duct ({H + 6:.0f}x{W + 6:.0f} with 3 frozen wall layers)"
+
+
 def make_case(name):
     from gpucell_amd import casegen
     if name == "suspension_1m":
-        return casegen.suspension_1m(), ("suspension_1m_100rbc: 230x60x60 duct (66x66 with 3 "
-                                         "frozen wall layers), 100 RBCs x 642 vertices at 20% "
-                                         "hematocrit, Poiseuille body force, aggregation off")
+        c = casegen.suspension_1m()
+        return c, (f"suspension_1m_100rbc: {duct(c)}, {c.nCell} RBCs x 642 vertices at "
+                   f"{100 * c.hematocrit:.0f}% hematocrit, Poiseuille body force, aggregation off")
     if name == "stenosis_2m":
         c = casegen.stenosis_2m()
-        return c, ("stenosis_2m_300rbc: 300x80x80 duct (86x86 with walls) with a 50% cosine "
-                   f"stenosis, 300 RBCs at {100 * c.hematocrit:.0f}% hematocrit (aligned lattice "
-                   "slots), Morse De=0.3kT, r0=0.3, beta=1.5")
+        return c, (f"stenosis_2m_300rbc: {duct(c)} with a 50% cosine stenosis, {c.nCell} RBCs at "
+                   f"{100 * c.hematocrit:.0f}% hematocrit (aligned lattice slots), Morse De=0.3kT, "
+                   "r0=0.3, beta=1.5")
     if name == "suspension_10m":
         c = casegen.suspension_10m()
-        return c, ("suspension_10m_2000rbc: 411x150x150 duct (156x156 with 3 frozen wall "
-                   f"layers), 2000 RBCs x 642 vertices at {100 * c.hematocrit:.0f}% hematocrit, "
-                   "Poiseuille body force, aggregation off")
+        return c, (f"suspension_10m_2000rbc: {duct(c)}, {c.nCell} RBCs x 642 vertices at "
+                   f"{100 * c.hematocrit:.0f}% hematocrit, Poiseuille body force, aggregation off")
     if name == "box32":
         return casegen.periodic_box(), "periodic 32^3 box, one RBC, reverse Poiseuille"
     if name == "ensemble64":

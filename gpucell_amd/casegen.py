@@ -547,8 +547,10 @@ def suspension_1m(seed: int = 20261203, physics: Physics | None = None, noise: b
     """1M fluid+wall particles (1,015,188 of 1,097,712 lattice sites remain
     around the cells), 100 RBCs at 20 % hematocrit (diameter 21.0), 3 wall
     layers: a 252 x 60 x 60 duct (66 x 66 with walls)."""
-    return channel(252, 60, 100, hematocrit=0.20, seed=seed, physics=physics, noise=noise,
-                   name="suspension_1m_100rbc")
+    c = channel(252, 60, 100, hematocrit=0.20, seed=seed, physics=physics, noise=noise,
+                name="suspension_1m_100rbc")
+    c.hematocrit = 0.20
+    return c
 
 
 def stenosis_2m(seed: int = 20261203, physics: Physics | None = None, noise: bool = True) -> Case:
